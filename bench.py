@@ -1,0 +1,194 @@
+"""Benchmark: localizations/sec of the plain LM fit on one MI355X.
+
+Workload (default): 10,000 frames of 256x256 uint16, 50 emitters/frame (500k ROIs),
+sigma uniform 1.1-1.6 px, photons log-uniform 200-5000, background uniform 50-150 ADU,
+Poisson noise, 9x9 windows, symmetric 5-parameter LM (initialPSFWidth 1.35).
+One step = one fit of all 500k candidates with frames and candidates resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  The CPU baseline is the FP64 oracle (frame-parallel thread pool,
+one frame per task, as the reference's PALMAnalysisController) on a bounded sample of
+the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from localizer_amd import abi, native, synth  # noqa: E402
+
+METRIC = "localizations/sec on 1 MI355X: plain LM vs net-initialized LM at 1e-3 px"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# Algorithmic work per ROI (symmetric model): W = N[(1+T) F_f + (1+A)(F_J + F_NE)] + T F_solve
+F_F, F_J, F_NE, F_SOLVE = 12.0, 6.0, 40.0, 300.0
+F_F7, F_J7, F_NE7, F_SOLVE7 = 16.0, 30.0, 70.0, 700.0
+
+
+def algorithmic_flops(fits, cands, default_h):
+    h = np.where(cands["half_width"] > 0, cands["half_width"], default_h).astype(np.float64)
+    N = (2 * h + 1) ** 2
+    T = fits["trials"].astype(np.float64)
+    A = fits["accepted_steps"].astype(np.float64)
+    started = fits["iterations"] > 0
+    sym = fits["model"] == abi.MODEL_SYMMETRIC
+    w = np.where(sym, N * ((1 + T) * F_F + (1 + A) * (F_J + F_NE)) + T * F_SOLVE,
+                 N * ((1 + T) * F_F7 + (1 + A) * (F_J7 + F_NE7)) + T * F_SOLVE7)
+    return float(np.sum(w[started]))
+
+
+def algorithmic_bytes(fits, cands, default_h):
+    """uint16 window pixels + candidate record + result record, per ROI."""
+    h = np.where(cands["half_width"] > 0, cands["half_width"], default_h).astype(np.float64)
+    N = (2 * h + 1) ** 2
+    return float(np.sum(2 * N + abi.CANDIDATE_DTYPE.itemsize + abi.FIT_DTYPE.itemsize))
+
+
+def cpu_baseline(w, frames, cands, target_seconds, nthreads):
+    """Oracle throughput on a bounded prefix of the workload (whole frames)."""
+    from oracle import pyoracle as po
+
+    def run(nf):
+        sel = cands["frame"] < nf
+        t0 = time.perf_counter()
+        out = po.fit_batch(frames[:nf], cands[sel], abi.MODEL_SYMMETRIC, w.initial_psf_width,
+                           default_half_width=w.half_width[0], nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        return int(np.sum(out["status"] == 0)), int(sel.sum()), dt
+
+    nf = min(50, w.nframes)
+    acc, n, dt = run(nf)
+    if dt < target_seconds and nf < w.nframes:
+        nf = int(min(w.nframes, max(nf, nf * target_seconds / max(dt, 1e-3))))
+        acc, n, dt = run(nf)
+    return {
+        "value": acc / dt,
+        "unit": "localizations/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"{nf} of {w.nframes} frames ({n} ROIs, {acc} accepted) in {dt:.2f} s, FP64 oracle, "
+        f"{nthreads} threads, one frame per task",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=0, help="override the frame count (testing)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    w = synth.PLAIN_GPU_256
+    if args.frames:
+        w = w.scaled(args.frames)
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    frames, truth, cands = synth.generate(w, nthreads=nthreads)
+    print(f"generated {w.nframes} frames, {cands.size} candidates in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    d_frames = torch.from_numpy(frames.view(np.int16)).to(dev)
+    d_cands = torch.from_numpy(cands.view(np.uint8)).to(dev)
+    d_out = torch.zeros(cands.size * abi.FIT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    ctx = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, 1.0, w.half_width[0])
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ctx.fit_device(d_frames.data_ptr(), w.nx, w.ny, w.nframes, d_cands.data_ptr(), cands.size,
+                       d_out.data_ptr(), stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        step()
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    kernel_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+
+    fits = d_out.cpu().numpy().view(abi.FIT_DTYPE)
+    accepted = int(np.sum(fits["status"] == 0))
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = accepted * args.steps / elapsed
+    flops = algorithmic_flops(fits, cands, w.half_width[0])
+    nbytes = algorithmic_bytes(fits, cands, w.half_width[0])
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    print(f"accepted {accepted}/{cands.size}, {ms_per_step:.3f} ms/step, event {kernel_ms:.3f} ms, "
+          f"{achieved:.2f} TFLOP/s algorithmic, mean trials {fits['trials'].mean():.2f}",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, frames, cands, args.cpu_seconds, nthreads)
+        print(f"cpu baseline {cpu['value']:.0f} loc/s ({cpu['sample']})", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "localizations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name,
+            "frames": w.nframes,
+            "frame_px": f"{w.nx}x{w.ny}",
+            "rois": int(cands.size),
+            "window": f"{2 * w.half_width[0] + 1}x{2 * w.half_width[0] + 1}",
+            "model": "symmetric 5-parameter LM",
+            "initial_psf_width": w.initial_psf_width,
+            "method": "plain LM (centroid start)",
+            "accepted": accepted,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel_ms": kernel_ms,
+            "algorithmic_flops_per_launch": flops,
+            "algorithmic_bytes_per_launch": nbytes,
+            "gather_gbs": nbytes / (kernel_ms * 1e-3) / 1e9,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

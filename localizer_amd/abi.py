@@ -1,0 +1,106 @@
+"""numpy views of the C-ABI records declared in include/localizer_hip.h.
+
+``CANDIDATE_DTYPE`` mirrors ``LzrCandidate`` (the reference ``Particle``,
+source/Storage.h:90-101, plus frame / window / model) and ``FIT_DTYPE`` mirrors
+``LzrFit`` (the fields of ``LocalizedPosition_2DGauss`` and
+``LocalizedPosition_Ellipsoidal2DGauss``, source/Storage.h:117-186).
+"""
+import numpy as np
+
+MODEL_SYMMETRIC = 0
+MODEL_ELLIPSOIDAL = 1
+MODEL_ELLIPSOIDAL_SYMMETRIC_PSF = 2
+
+MAX_HALF_WIDTH = 15
+MAX_PARAMS = 7
+
+ACCEPTED = 0
+REJECT_EDGE = 1
+REJECT_AMPLITUDE = 2
+REJECT_POSITION = 3
+REJECT_WIDTH = 4
+REJECT_EIGEN = 5
+REJECT_ANISOTROPIC = 6
+REJECT_PSF_WIDTH = 7
+REJECT_INVALID = 8
+
+LM_SUCCESS = 0
+LM_CONTINUE = -2
+LM_FAILURE = -1
+LM_ENOPROG = 27
+LM_ETOLF = 29
+LM_ETOLX = 30
+LM_ETOLG = 31
+
+OPT_MAX_ITERATIONS = 1
+OPT_EPSABS = 2
+OPT_EPSREL = 3
+OPT_DIAG_UPDATE_ROWS = 4
+OPT_WAVES_PER_CU = 5
+OPT_MAX_HALF_WIDTH = 6
+
+CANDIDATE_DTYPE = np.dtype(
+    [
+        ("x", "<f8"),
+        ("y", "<f8"),
+        ("intensity", "<f8"),
+        ("background", "<f8"),
+        ("frame", "<i4"),
+        ("half_width", "<i2"),
+        ("model", "<i2"),
+    ],
+    align=True,
+)
+
+FIT_DTYPE = np.dtype(
+    [
+        ("params", "<f8", (MAX_PARAMS,)),
+        ("deviations", "<f8", (MAX_PARAMS,)),
+        ("integral", "<f8"),
+        ("integral_deviation", "<f8"),
+        ("sd1", "<f8"),
+        ("sd2", "<f8"),
+        ("theta", "<f8"),
+        ("sd1_deviation", "<f8"),
+        ("sd2_deviation", "<f8"),
+        ("chi", "<f8"),
+        ("status", "<i4"),
+        ("lm_status", "<i4"),
+        ("iterations", "<i2"),
+        ("trials", "<i2"),
+        ("accepted_steps", "<i2"),
+        ("model", "<i2"),
+    ],
+    align=True,
+)
+
+assert CANDIDATE_DTYPE.itemsize == 40
+assert FIT_DTYPE.itemsize == 192
+
+# Truth records written by the synthetic generator (localizer_amd/csrc/lzr_synth.h).
+EMITTER_DTYPE = np.dtype(
+    [
+        ("x", "<f8"),
+        ("y", "<f8"),
+        ("photons", "<f8"),
+        ("sigma_x", "<f8"),
+        ("sigma_y", "<f8"),
+        ("amplitude", "<f8"),
+        ("background", "<f8"),
+        ("frame", "<i4"),
+        ("model", "<i2"),
+        ("half_width", "<i2"),
+    ],
+    align=True,
+)
+
+
+def n_params(model):
+    """Number of fitted parameters of a model code."""
+    return 5 if int(model) == MODEL_SYMMETRIC else 7
+
+
+def default_half_width(initial_psf_width):
+    """The reference window: cutoff_radius = ceil(4 * initialPSFWidth)
+    (source/Localization.h:68)."""
+    return int(np.ceil(initial_psf_width * 4.0))

@@ -1,0 +1,1009 @@
+/*
+ * lzr_lane.h -- the per-spot Levenberg-Marquardt state machine of the MI355X engine.
+ *
+ * One spot (ROI) is fitted by one GPU lane.  The lane runs the same algorithm as
+ * GSL 1.16 gsl_multifit_fdfsolver_lmsder (source/Localization.cpp:75, :145-155), the
+ * MINPACK lmder trust-region method with automatic column scaling, restated on the
+ * normal equations instead of a QR factorisation of J:
+ *
+ *   MINPACK quantity               here (H = J^T J, g = J^T f, D = diag)
+ *   Gauss-Newton step R P^T x=Q^T f  H x = g                      (Cholesky)
+ *   qrsolv([R; sqrt(par) D])        (H + par D^2) x = g            (Cholesky)
+ *   |R^-T P^T u|^2 (parl, parc)     u^T (H + par D^2)^-1 u
+ *   |D^-1 R^T Q^T f| (paru)         |D^-1 g|
+ *   |R P^T dx| (predicted red.)     sqrt(dx^T H dx)
+ *   rank deficiency (nsing < p)     Cholesky breakdown
+ * so the trial points, trust radius, Levenberg-Marquardt parameter, acceptance
+ * ratio, status codes and the reference driver's test_delta / 200-iteration loop
+ * are those of the reference.  The residuals, Jacobian and normal-equation
+ * accumulation run per pixel in FP32 (FP64 accumulation optional, template Acc);
+ * the p x p solves run in FP64.
+ *
+ * Numerics of the acceptance test.  Near convergence the actual reduction
+ * 1 - |f1|^2/|f0|^2 is ~1e-7, below FP32 resolution of |f|^2.  The pass therefore
+ * evaluates the residual change f1 - f0 = M(theta1) - M(theta0) directly: the change
+ * of the Gaussian exponent is formed from the parameter step (no cancellation) and
+ * the exponential change as e0 * expm1(-d_arg); the SSE change is
+ * sum (f1 - f0)(f1 + f0).  That keeps the ratio test accurate to ~1e-3 relative.
+ *
+ * The file compiles as HIP device code (kernel) and as plain C++ (the CPU
+ * emulation used by the tests to check the logic against the FP64 oracle).
+ */
+#ifndef LZR_LANE_H
+#define LZR_LANE_H
+
+#include <stdint.h>
+
+#include "../../include/localizer_hip.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define LZR_HD __host__ __device__
+#define LZR_INLINE __forceinline__
+#else
+#include <cmath>
+#define LZR_HD
+#define LZR_INLINE inline
+#endif
+
+namespace lzr {
+
+constexpr double kPi = 3.1415926535897932384626433; /* source/Defines.h:97 */
+constexpr double kSqrt2 = 1.4142135623730950488;    /* source/Defines.h:98 */
+constexpr double kDblEpsilon = 2.2204460492503131e-16;
+constexpr double kDblMin = 2.2250738585072014e-308;
+
+enum Phase : int { kIdle = 0, kInit = 1, kTrial = 2, kFinished = 3 };
+
+/* Fitter-wide settings (the reference constructor arguments plus the loop constants
+ * source/Localization.cpp:154-155). */
+struct LaneConfig {
+    double initial_psf_width; /* w */
+    double sigma;             /* residual weight */
+    double epsabs, epsrel;    /* test_delta tolerances */
+    int max_iterations;       /* 200 */
+    int default_half_width;   /* window half-width when a candidate gives none */
+    int max_half_width;       /* largest window this launch holds */
+    int diag_update_rows;     /* -1: p rows (GSL 1.16), 0: all rows (MINPACK) */
+    int model;                /* default model */
+};
+
+constexpr LZR_HD int tri(int i, int j) { return i >= j ? (i * (i + 1)) / 2 + j : (j * (j + 1)) / 2 + i; }
+
+template <int P>
+struct NP {
+    static constexpr int S = P * (P + 1) / 2; /* packed symmetric P x P */
+    static constexpr int Q = P - 1;           /* columns other than the offset b */
+    static constexpr int SQ = Q * (Q + 1) / 2;
+};
+
+/* Per-lane LM state.  x0/y0 are kept window-local (exact integer offsets xoff/yoff);
+ * every quantity the reference computes from absolute coordinates (test_delta,
+ * |D x|) uses xoff + local. */
+template <int P>
+struct LaneState {
+    double th[P];   /* current parameters                           */
+    double dx[P];   /* last step (GSL s->dx)                        */
+    double diag[P]; /* column scaling D                             */
+    float H[NP<P>::S]; /* J^T J at th (packed)                      */
+    float g[P];        /* J^T f at th                               */
+    double delta, par, fnorm, xnorm, gnorm;
+    int xoff, yoff, n, frame;
+    int roi;
+    int phase;
+    int iter_gsl;   /* GSL state->iter: 1 + accepted steps          */
+    int iterations; /* reference loop counter                        */
+    int inner;      /* trials inside the current iterate()          */
+    int trials, accepted;
+};
+
+/* FP32 constants of one pixel pass: current point theta0, trial point theta1. */
+template <int P>
+struct PassIn;
+
+template <>
+struct PassIn<5> {
+    float A0, x0, y0, b0, c0; /* c = 1/(2 r^2) */
+    float dA, dxc, dyc, db, dc;
+    float A1, kr, kx, inv_sigma;
+    double A0d, b0d, c0d, inv_sigma_d; /* FP64 copies for the current-point residual */
+};
+
+template <>
+struct PassIn<7> {
+    float A0, x0, y0, b0, qa0, qc0, qb0; /* exponent a ux^2 + 2c ux uy + b uy^2 */
+    float dA, dxc, dyc, db, dqa, dqc, dqb;
+    float A1, k1, k2, k3, k4, k5, k6, k7, k8, k9, two_rho, inv_sigma;
+    double A0d, b0d, qa0d, qc0d, qb0d, inv_sigma_d;
+};
+
+/* Accumulators of one pass.  The offset column of J is the constant 1/sigma, so its
+ * normal-equation entries come from plain sums. */
+template <int P, typename Acc>
+struct PassAcc {
+    Acc hq[NP<P>::SQ]; /* sum J_i J_j, i,j < P-1 */
+    Acc sj[P - 1];     /* sum J_i                */
+    Acc gq[P - 1];     /* sum J_i f1             */
+    Acc sf;            /* sum f1                 */
+    Acc sse0;          /* sum f0^2               */
+    Acc dsse;          /* sum (f1-f0)(f1+f0)     */
+    LZR_HD LZR_INLINE void zero() {
+#pragma unroll
+        for (int i = 0; i < NP<P>::SQ; ++i) hq[i] = Acc(0);
+#pragma unroll
+        for (int i = 0; i < P - 1; ++i) {
+            sj[i] = Acc(0);
+            gq[i] = Acc(0);
+        }
+        sf = sse0 = dsse = Acc(0);
+    }
+};
+
+LZR_HD LZR_INLINE float fexp(float a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __expf(a);
+#else
+    return std::exp(a);
+#endif
+}
+
+LZR_HD LZR_INLINE double dexp(double a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return exp(a);
+#else
+    return std::exp(a);
+#endif
+}
+
+/* Exponential change e1 - e0 = e0 (exp(-darg) - 1) and e1 = exp(-arg1).  For a small
+ * exponent change the difference comes from the Taylor series of expm1 (full relative
+ * accuracy); for a large one e1 is evaluated directly, where the subtraction e1 - e0
+ * loses at most ~1e-6 relative and e0 may have underflowed. */
+LZR_HD LZR_INLINE void exp_change(float e0, float arg1, float darg, float &e1, float &e0m) {
+    const float a = -darg;
+    const float em = a * (1.0f + a * (0.5f + a * (1.0f / 6.0f + a * (1.0f / 24.0f + a * (1.0f / 120.0f)))));
+    const float e1d = fexp(-arg1);
+    const bool small = darg > -0.0625f && darg < 0.0625f;
+    e0m = small ? e0 * em : e1d - e0;
+    e1 = small ? e0 + e0m : e1d;
+}
+
+/* ---- per-pixel work -------------------------------------------------------- */
+
+/* Residual at the current point, f0 = (A e0 + b - D)/sigma.  With F64 the exponent,
+ * exponential and sum run in FP64 (ux0, uy0 are exact in FP32, so the quadratic form is
+ * exact in FP64); the near-converged gradient J^T f is a small sum of large terms and
+ * inherits the residual's absolute error. */
+template <bool F64>
+LZR_HD LZR_INLINE void residual0(double qd_arg0, float arg0_f, double A0d, double b0d, double isd, float A0,
+                                 float b0, float is, float D, float &e0, float &f0) {
+    if constexpr (F64) {
+        const double e = dexp(-qd_arg0);
+        e0 = (float)e;
+        f0 = (float)((A0d * e + (b0d - (double)D)) * isd);
+    } else {
+        e0 = fexp(-arg0_f);
+        f0 = (A0 * e0 + b0 - D) * is;
+    }
+}
+
+template <bool F64, typename Acc>
+LZR_HD LZR_INLINE void pass_pixel(const PassIn<5> &q, float fi, float fj, float D, PassAcc<5, Acc> &a) {
+    const float ux0 = fi - q.x0, uy0 = fj - q.y0;
+    const float ux1 = ux0 - q.dxc, uy1 = uy0 - q.dyc;
+    const float d0 = ux0 * ux0 + uy0 * uy0;
+    const float d1 = ux1 * ux1 + uy1 * uy1;
+    const float arg0 = d0 * q.c0;
+    float e0, f0;
+    {
+        const double dx = ux0, dy = uy0;
+        residual0<F64>(F64 ? (dx * dx + dy * dy) * q.c0d : 0.0, arg0, q.A0d, q.b0d, q.inv_sigma_d, q.A0, q.b0,
+                       q.inv_sigma, D, e0, f0);
+    }
+    /* exponent change, free of cancellation: (d1-d0) c1 + d0 (c1-c0) */
+    const float dd = -(q.dxc * (ux1 + ux0) + q.dyc * (uy1 + uy0));
+    const float darg = dd * (q.c0 + q.dc) + d0 * q.dc;
+    float e1, e0m;
+    exp_change(e0, arg0 + darg, darg, e1, e0m);
+    const float df = (q.dA * e1 + q.A0 * e0m + q.db) * q.inv_sigma;
+    const float f1 = f0 + df;
+    a.sse0 += Acc(f0) * Acc(f0);
+    a.dsse += Acc(df) * Acc(f0 + f1);
+    /* Jacobian at the trial point, reference columns (A, r, x0, y0) */
+    const float jA = e1 * q.inv_sigma;
+    const float ae = q.A1 * e1;
+    const float jr = q.kr * ae * d1;
+    const float jx = q.kx * ae * ux1;
+    const float jy = q.kx * ae * uy1;
+    const Acc J[4] = {Acc(jA), Acc(jr), Acc(jx), Acc(jy)};
+    const Acc F = Acc(f1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] += J[i] * J[j];
+        a.sj[i] += J[i];
+        a.gq[i] += J[i] * F;
+    }
+    a.sf += F;
+}
+
+template <bool F64, typename Acc>
+LZR_HD LZR_INLINE void pass_pixel(const PassIn<7> &q, float fi, float fj, float D, PassAcc<7, Acc> &a) {
+    const float ux0 = fi - q.x0, uy0 = fj - q.y0;
+    const float ux1 = ux0 - q.dxc, uy1 = uy0 - q.dyc;
+    const float xx0 = ux0 * ux0, yy0 = uy0 * uy0, xy0 = ux0 * uy0;
+    const float xx1 = ux1 * ux1, yy1 = uy1 * uy1, xy1 = ux1 * uy1;
+    const float arg0 = q.qa0 * xx0 + 2.0f * q.qc0 * xy0 + q.qb0 * yy0;
+    float e0, f0;
+    {
+        const double dx = ux0, dy = uy0;
+        residual0<F64>(F64 ? q.qa0d * dx * dx + 2.0 * q.qc0d * dx * dy + q.qb0d * dy * dy : 0.0, arg0, q.A0d,
+                       q.b0d, q.inv_sigma_d, q.A0, q.b0, q.inv_sigma, D, e0, f0);
+    }
+    const float dxx = -q.dxc * (ux1 + ux0);
+    const float dyy = -q.dyc * (uy1 + uy0);
+    const float dxy = -(ux1 * q.dyc + uy0 * q.dxc);
+    const float darg = q.dqa * xx1 + q.qa0 * dxx + 2.0f * (q.dqc * xy1 + q.qc0 * dxy) + q.dqb * yy1 + q.qb0 * dyy;
+    float e1, e0m;
+    exp_change(e0, arg0 + darg, darg, e1, e0m);
+    const float df = (q.dA * e1 + q.A0 * e0m + q.db) * q.inv_sigma;
+    const float f1 = f0 + df;
+    a.sse0 += Acc(f0) * Acc(f0);
+    a.dsse += Acc(df) * Acc(f0 + f1);
+    const float ae = q.A1 * e1;
+    const float jA = e1 * q.inv_sigma;
+    const float jsx = ae * (q.k1 * xx1 - q.k2 * xy1);
+    const float jsy = ae * (q.k3 * yy1 - q.k4 * xy1);
+    const float jx = ae * (q.k5 * ux1 - q.k6 * uy1);
+    const float jy = ae * (q.k7 * uy1 - q.k6 * ux1);
+    const float jr = ae * (q.k8 * xy1 - q.k9 * (xx1 + yy1 - q.two_rho * xy1));
+    const Acc J[6] = {Acc(jA), Acc(jsx), Acc(jsy), Acc(jx), Acc(jy), Acc(jr)};
+    const Acc F = Acc(f1);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] += J[i] * J[j];
+        a.sj[i] += J[i];
+        a.gq[i] += J[i] * F;
+    }
+    a.sf += F;
+}
+
+/* ---- FP64 helpers ---------------------------------------------------------- */
+
+LZR_HD LZR_INLINE double dsqrt(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_sqrt(v);
+#else
+    return std::sqrt(v);
+#endif
+}
+LZR_HD LZR_INLINE double dabs(double v) { return v < 0 ? -v : v; }
+LZR_HD LZR_INLINE double dmax(double a, double b) { return a > b ? a : b; }
+LZR_HD LZR_INLINE double dmin(double a, double b) { return a < b ? a : b; }
+
+/* Cholesky M = L L^T (packed); false on a non-positive pivot. */
+template <int P>
+LZR_HD LZR_INLINE bool cholesky(const double (&M)[NP<P>::S], double (&L)[NP<P>::S], double (&inv_diag)[P]) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+        double s = M[tri(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s -= L[tri(j, k)] * L[tri(j, k)];
+        if (!(s > 0.0)) return false;
+        const double d = dsqrt(s);
+        const double inv = 1.0 / d;
+        L[tri(j, j)] = d;
+        inv_diag[j] = inv;
+#pragma unroll
+        for (int i = j + 1; i < P; ++i) {
+            double t = M[tri(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) t -= L[tri(i, k)] * L[tri(j, k)];
+            L[tri(i, j)] = t * inv;
+        }
+    }
+    return true;
+}
+
+/* y = L^-1 b */
+template <int P>
+LZR_HD LZR_INLINE void forward(const double (&L)[NP<P>::S], const double (&inv_diag)[P], const double (&b)[P],
+                               double (&y)[P]) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        double s = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= L[tri(i, k)] * y[k];
+        y[i] = s * inv_diag[i];
+    }
+}
+
+/* x = L^-T y */
+template <int P>
+LZR_HD LZR_INLINE void backward(const double (&L)[NP<P>::S], const double (&inv_diag)[P], const double (&y)[P],
+                                double (&x)[P]) {
+#pragma unroll
+    for (int i = P - 1; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int k = i + 1; k < P; ++k) s -= L[tri(k, i)] * x[k];
+        x[i] = s * inv_diag[i];
+    }
+}
+
+template <int P>
+LZR_HD LZR_INLINE double scaled_norm(const double (&d)[P], const double (&v)[P]) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) s += (d[i] * v[i]) * (d[i] * v[i]);
+    return dsqrt(s);
+}
+
+template <int P>
+LZR_HD LZR_INLINE double plain_norm(const double (&v)[P]) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) s += v[i] * v[i];
+    return dsqrt(s);
+}
+
+/* Levenberg-Marquardt parameter (MINPACK lmpar on the normal equations).
+ * Returns x with (H + par D^2) x = g; the LM step is -x. */
+template <int P>
+LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], const double (&diag)[P],
+                             double delta, double &par_io, double (&x)[P]) {
+    constexpr int S = NP<P>::S;
+    double L[S], id[P], y[P], u[P];
+    /* Gauss-Newton step.  A Cholesky breakdown is MINPACK's rank-deficient case: the
+     * undamped step is not usable, the lower bound parl is 0 and the damped iteration
+     * below (H + par D^2 is positive definite for par > 0) determines the step. */
+    const bool full_rank = cholesky<P>(H, L, id);
+    double dxnorm, fp;
+    if (full_rank) {
+        forward<P>(L, id, g, y);
+        backward<P>(L, id, y, x);
+        dxnorm = scaled_norm<P>(diag, x);
+        fp = dxnorm - delta;
+        if (fp <= 0.1 * delta) {
+            par_io = 0.0;
+            return;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; ++i) x[i] = 0.0;
+        dxnorm = 0.0;
+        fp = 1.0e300;
+    }
+    double parl = 0.0;
+    if (full_rank) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] / dxnorm);
+        forward<P>(L, id, u, y);
+        const double wn = plain_norm<P>(y);
+        const double phider = wn * wn;
+        parl = phider > 0.0 ? fp / (delta * phider) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i) u[i] = g[i] / diag[i];
+    const double gnorm = plain_norm<P>(u);
+    double paru = gnorm / delta;
+    if (paru == 0.0) paru = kDblMin / dmin(delta, 0.1);
+    double par = par_io;
+    if (par > paru)
+        par = paru;
+    else if (par < parl)
+        par = parl;
+    if (par == 0.0 && full_rank) par = gnorm / dxnorm;
+    double M[S];
+    for (int iter = 1;; ++iter) {
+        if (par == 0.0) par = dmax(0.001 * paru, kDblMin);
+#pragma unroll
+        for (int i = 0; i < S; ++i) M[i] = H[i];
+#pragma unroll
+        for (int i = 0; i < P; ++i) M[tri(i, i)] += par * diag[i] * diag[i];
+        if (!cholesky<P>(M, L, id)) {
+            /* (H + par D^2) lost definiteness in rounding: keep the last x */
+            break;
+        }
+        forward<P>(L, id, g, y);
+        backward<P>(L, id, y, x);
+        dxnorm = scaled_norm<P>(diag, x);
+        const double fp_old = fp;
+        fp = dxnorm - delta;
+        if (dabs(fp) <= 0.1 * delta) break;
+        if (parl == 0.0 && fp <= fp_old && fp_old < 0.0) break;
+        if (iter == 10) break;
+#pragma unroll
+        for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] / dxnorm);
+        forward<P>(L, id, u, y);
+        const double wn = plain_norm<P>(y);
+        const double parc = fp / (delta * wn * wn);
+        if (fp > 0.0) {
+            if (par > parl) parl = par;
+        } else if (fp < 0.0) {
+            if (par < paru) paru = par;
+        }
+        par = dmax(parl, par + parc);
+    }
+    par_io = par;
+}
+
+/* ---- model-specific pieces -------------------------------------------------- */
+
+template <int P>
+LZR_HD LZR_INLINE int x_index() { return P == 5 ? 2 : 3; }
+
+template <int P>
+LZR_HD LZR_INLINE bool width_is_zero(const double (&t)[P]) {
+    return P == 5 ? (t[1] == 0.0) : (t[1] == 0.0 || t[2] == 0.0);
+}
+
+/* Absolute-coordinate parameter vector (for test_delta and |D x|). */
+template <int P>
+LZR_HD LZR_INLINE void absolute(const LaneState<P> &s, const double (&t)[P], double (&a)[P]) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) a[i] = t[i];
+    a[x_index<P>()] += (double)s.xoff;
+    a[x_index<P>() + 1] += (double)s.yoff;
+}
+
+LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[5], const double (&dx)[5], double sigma, PassIn<5> &q) {
+    const double r0 = t0[1], r1 = t0[1] + dx[1];
+    const double A1 = t0[0] + dx[0];
+    const double c0 = 1.0 / (2.0 * r0 * r0), c1 = 1.0 / (2.0 * r1 * r1);
+    q.A0 = (float)t0[0];
+    q.x0 = (float)t0[2];
+    q.y0 = (float)t0[3];
+    q.b0 = (float)t0[4];
+    q.c0 = (float)c0;
+    q.dA = (float)dx[0];
+    q.dxc = (float)dx[2];
+    q.dyc = (float)dx[3];
+    q.db = (float)dx[4];
+    q.dc = (float)(c1 - c0);
+    q.A1 = (float)A1;
+    q.inv_sigma = (float)(1.0 / sigma);
+    q.A0d = t0[0];
+    q.b0d = t0[4];
+    q.c0d = c0;
+    q.inv_sigma_d = 1.0 / sigma;
+    /* d f/d r = A e d^2 / (sqrt(2) r^3 sigma)   (source/Localization.cpp:1151-1166)
+     * d f/d x0 = A e (x - x0) / (r^2 sigma)                                        */
+    q.kr = (float)(1.0 / (kSqrt2 * r1 * r1 * r1 * sigma));
+    q.kx = (float)(1.0 / (r1 * r1 * sigma));
+}
+
+LZR_HD LZR_INLINE void quad_coeffs(double sx, double sy, double rho, double &qa, double &qc, double &qb) {
+    const double pre = 1.0 / (2.0 * (1.0 - rho * rho));
+    qa = pre / (sx * sx);
+    qb = pre / (sy * sy);
+    qc = -pre * rho / (sx * sy);
+}
+
+LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7], double sigma, PassIn<7> &q) {
+    double qa0, qc0, qb0, qa1, qc1, qb1;
+    quad_coeffs(t0[1], t0[2], t0[5], qa0, qc0, qb0);
+    const double sx = t0[1] + dx[1], sy = t0[2] + dx[2], rho = t0[5] + dx[5];
+    quad_coeffs(sx, sy, rho, qa1, qc1, qb1);
+    q.A0 = (float)t0[0];
+    q.x0 = (float)t0[3];
+    q.y0 = (float)t0[4];
+    q.b0 = (float)t0[6];
+    q.qa0 = (float)qa0;
+    q.qc0 = (float)qc0;
+    q.qb0 = (float)qb0;
+    q.dA = (float)dx[0];
+    q.dxc = (float)dx[3];
+    q.dyc = (float)dx[4];
+    q.db = (float)dx[6];
+    q.dqa = (float)(qa1 - qa0);
+    q.dqc = (float)(qc1 - qc0);
+    q.dqb = (float)(qb1 - qb0);
+    q.A1 = (float)(t0[0] + dx[0]);
+    q.inv_sigma = (float)(1.0 / sigma);
+    q.A0d = t0[0];
+    q.b0d = t0[6];
+    q.qa0d = qa0;
+    q.qc0d = qc0;
+    q.qb0d = qb0;
+    q.inv_sigma_d = 1.0 / sigma;
+    /* reference columns (source/Localization.cpp:1275-1280) with den = 2(1-rho^2) sigma */
+    const double omr = 1.0 - rho * rho;
+    const double den = 2.0 * omr * sigma;
+    q.k1 = (float)(2.0 / (den * sx * sx * sx));
+    q.k2 = (float)(2.0 * rho / (den * sx * sx * sy));
+    q.k3 = (float)(2.0 / (den * sy * sy * sy));
+    q.k4 = (float)(2.0 * rho / (den * sx * sy * sy));
+    q.k5 = (float)(2.0 / (den * sx * sx));
+    q.k6 = (float)(2.0 * rho / (den * sx * sy));
+    q.k7 = (float)(2.0 / (den * sy * sy));
+    q.k8 = (float)(1.0 / (sigma * omr * sx * sy));
+    q.k9 = (float)(rho / (sigma * sx * sy * omr * omr));
+    q.two_rho = (float)(2.0 * rho);
+}
+
+/* Column norms of the first `rows` Jacobian rows at t (GSL 1.16 update_diag reads the
+ * first p rows of J).  Pixel k = i + n j, absolute coordinates. */
+template <int P>
+LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double sigma, int rows, double (&cn)[P]) {
+#pragma unroll
+    for (int c = 0; c < P; ++c) cn[c] = 0.0;
+    for (int k = 0; k < rows; ++k) {
+        const int i = k % n, j = k / n;
+        double row[P];
+        if (P == 5) {
+            const double A = t[0], r = t[1];
+            const double ux = i - t[2], uy = j - t[3];
+            const double d2 = ux * ux + uy * uy;
+            const double e = (double)fexp((float)(-d2 / (2.0 * r * r)));
+            row[0] = e / sigma;
+            row[1] = A * e * d2 / (kSqrt2 * r * r * r * sigma);
+            row[2] = A * e * ux / (r * r * sigma);
+            row[3] = A * e * uy / (r * r * sigma);
+            row[4] = 1.0 / sigma;
+        } else {
+            const double A = t[0], sx = t[1], sy = t[2], rho = t[P == 5 ? 0 : 5];
+            const double ux = i - t[3], uy = j - t[4];
+            const double omr = 1.0 - rho * rho, den = 2.0 * omr * sigma;
+            const double q = ux * ux / (sx * sx) + uy * uy / (sy * sy) - 2.0 * rho * ux * uy / (sx * sy);
+            const double e = (double)fexp((float)(-q / (2.0 * omr)));
+            row[0] = e / sigma;
+            row[1] = -A * (2.0 * rho * ux * uy / (sx * sx * sy) - 2.0 * ux * ux / (sx * sx * sx)) * e / den;
+            row[2] = -A * (2.0 * rho * ux * uy / (sx * sy * sy) - 2.0 * uy * uy / (sy * sy * sy)) * e / den;
+            row[3 % P] = -A * (2.0 * rho * uy / (sx * sy) - 2.0 * ux / (sx * sx)) * e / den;
+            row[4 % P] = -A * (2.0 * rho * ux / (sx * sy) - 2.0 * uy / (sy * sy)) * e / den;
+            row[5 % P] = A * (ux * uy / omr / (sx * sy) - rho * (ux * ux + uy * uy - 2.0 * rho * ux * uy) / (sx * sy) / omr / omr) * e / sigma;
+            row[(P - 1)] = 1.0 / sigma;
+        }
+#pragma unroll
+        for (int c = 0; c < P; ++c) cn[c] += row[c] * row[c];
+    }
+#pragma unroll
+    for (int c = 0; c < P; ++c) cn[c] = dsqrt(cn[c] == 0.0 ? 1.0 : cn[c]);
+}
+
+/* Full normal equations from the pass accumulators. */
+template <int P, typename Acc>
+LZR_HD LZR_INLINE void assemble(const PassAcc<P, Acc> &a, int npix, double sigma, float (&H)[NP<P>::S], float (&g)[P]) {
+    const double is = 1.0 / sigma;
+#pragma unroll
+    for (int i = 0; i < P - 1; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) H[tri(i, j)] = (float)a.hq[tri(i, j)];
+        H[tri(P - 1, i)] = (float)((double)a.sj[i] * is);
+        g[i] = (float)a.gq[i];
+    }
+    H[tri(P - 1, P - 1)] = (float)((double)npix * is * is);
+    g[P - 1] = (float)((double)a.sf * is);
+}
+
+template <int P>
+LZR_HD LZR_INLINE void to_double(const float (&Hf)[NP<P>::S], const float (&gf)[P], double (&H)[NP<P>::S], double (&g)[P]) {
+#pragma unroll
+    for (int i = 0; i < NP<P>::S; ++i) H[i] = (double)Hf[i];
+#pragma unroll
+    for (int i = 0; i < P; ++i) g[i] = (double)gf[i];
+}
+
+/* Zeroes an output record field by field (the record may live in device memory). */
+LZR_HD LZR_INLINE void clear_record(LzrFit &o) {
+#pragma unroll
+    for (int i = 0; i < LZR_MAX_PARAMS; ++i) {
+        o.params[i] = 0.0;
+        o.deviations[i] = 0.0;
+    }
+    o.integral = o.integral_deviation = 0.0;
+    o.sd1 = o.sd2 = o.theta = o.sd1_deviation = o.sd2_deviation = 0.0;
+    o.chi = 0.0;
+    o.status = 0;
+    o.lm_status = 0;
+    o.iterations = o.trials = o.accepted_steps = o.model = 0;
+}
+
+/* ---- the state machine -------------------------------------------------------- */
+
+/* Prepares a lane for a new candidate.  Returns false when the candidate is rejected
+ * before fitting (window outside the frame, invalid request); `out` then holds the
+ * final record. */
+template <int P>
+LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const LzrCandidate &c, int roi, int nx,
+                                  int ny, int nframes, int model, LzrFit &out) {
+    s.roi = roi;
+    const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
+    if (h < 1 || h > cfg.max_half_width || c.frame < 0 || c.frame >= nframes) {
+        clear_record(out);
+        out.model = (int16_t)model;
+        out.status = LZR_REJECT_INVALID;
+        return false;
+    }
+    const int n = 2 * h + 1;
+    /* x_offset = floor(x - cutoff_radius) (source/Localization.cpp:112-124) */
+    const double fx = c.x - (double)h, fy = c.y - (double)h;
+    const bool inside = (fx >= 0.0) && (fy >= 0.0) && (fx < 2147483647.0) && (fy < 2147483647.0) &&
+                        ((int)fx + n - 1 <= nx - 1) && ((int)fy + n - 1 <= ny - 1);
+    if (!inside) {
+        clear_record(out);
+        out.model = (int16_t)model;
+        out.status = LZR_REJECT_EDGE;
+        return false;
+    }
+    const int xoff = (int)fx, yoff = (int)fy; /* floor of a non-negative value */
+    s.xoff = xoff;
+    s.yoff = yoff;
+    s.n = n;
+    s.frame = c.frame;
+    const double w = cfg.initial_psf_width;
+    if (P == 5) {
+        s.th[0] = c.intensity;
+        s.th[1] = w;
+        s.th[2] = c.x - xoff;
+        s.th[3] = c.y - yoff;
+        s.th[4 % P] = c.background;
+    } else {
+        s.th[0] = c.intensity;
+        s.th[1] = w;
+        s.th[2] = w;
+        s.th[3] = c.x - xoff;
+        s.th[4] = c.y - yoff;
+        s.th[5 % P] = 0.0;
+        s.th[6 % P] = c.background;
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i) s.dx[i] = 0.0;
+    s.phase = kInit;
+    s.iterations = 0;
+    s.trials = 0;
+    s.accepted = 0;
+    return true;
+}
+
+/* FP32 pass constants for the lane's next evaluation (trial point = th + dx for a
+ * trial, th itself for the initial evaluation). */
+template <int P>
+LZR_HD LZR_INLINE void lane_pass_in(const LaneState<P> &s, double sigma, PassIn<P> &q) {
+    double step[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
+    make_pass_in(s.th, step, sigma, q);
+}
+
+template <int P>
+LZR_HD LZR_INLINE int test_delta(const LaneState<P> &s, const LaneConfig &cfg) {
+    double a[P];
+    absolute<P>(s, s.th, a);
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const double tol = cfg.epsabs + cfg.epsrel * dabs(a[i]);
+        if (!(dabs(s.dx[i]) < tol)) return LZR_LM_CONTINUE;
+    }
+    return LZR_LM_SUCCESS;
+}
+
+/* Proposes the next trial (one lmpar + the first-iteration radius clamp). */
+template <int P>
+LZR_HD LZR_INLINE void propose(LaneState<P> &s) {
+    double H[NP<P>::S], g[P], x[P];
+    to_double<P>(s.H, s.g, H, g);
+    s.inner++;
+    lmpar<P>(H, g, s.diag, s.delta, s.par, x);
+#pragma unroll
+    for (int i = 0; i < P; ++i) s.dx[i] = -x[i];
+    const double pnorm = scaled_norm<P>(s.diag, s.dx);
+    if (s.iter_gsl == 1 && pnorm < s.delta) s.delta = pnorm;
+    s.phase = kTrial;
+}
+
+/* Start of one gsl_multifit_fdfsolver_iterate call of the reference loop
+ * (source/Localization.cpp:148-155).  Returns true when a trial evaluation is needed;
+ * false when the loop ended (status in *st). */
+template <int P>
+LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int *st) {
+    for (;;) {
+        s.iterations++;
+        if (s.fnorm != 0.0) break;
+        /* |f| == 0: iterate() returns success with dx unchanged */
+        const int td = test_delta<P>(s, cfg);
+        if (td != LZR_LM_CONTINUE || s.iterations >= cfg.max_iterations) {
+            *st = td;
+            return false;
+        }
+    }
+    double gmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) gmax = dmax(gmax, dabs((double)s.g[i] / s.diag[i]));
+    s.gnorm = gmax / s.fnorm;
+    s.inner = 0;
+    propose<P>(s);
+    return true;
+}
+
+template <int P>
+LZR_HD LZR_INLINE void finish(LaneState<P> &s, const LaneConfig &cfg, int lm_status, int model, LzrFit &out);
+
+/* Consumes one pass.  Returns true if the lane needs another pass (a new trial), false
+ * when the fit is complete (out written). */
+template <int P, typename Acc>
+LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int model,
+                                       LzrFit &out) {
+    const int npix = s.n * s.n;
+    if (s.phase == kInit) {
+        /* lmsder set(): f, J at the start point */
+        assemble<P, Acc>(a, npix, cfg.sigma, s.H, s.g);
+        s.fnorm = dsqrt((double)a.sse0);
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const double hii = (double)s.H[tri(i, i)];
+            s.diag[i] = dsqrt(hii == 0.0 ? 1.0 : hii);
+        }
+        double absx[P];
+        absolute<P>(s, s.th, absx);
+        s.xnorm = scaled_norm<P>(s.diag, absx);
+        s.delta = s.xnorm > 0.0 ? 100.0 * s.xnorm : 100.0;
+        s.par = 0.0;
+        s.iter_gsl = 1;
+        int st = 0;
+        if (begin_iterate<P>(s, cfg, &st)) return true;
+        finish<P>(s, cfg, st, model, out);
+        return false;
+    }
+    /* trial evaluated at th + dx */
+    s.trials++;
+    const double fn2 = s.fnorm * s.fnorm;
+    /* FP32 overflow (a trial far outside the model's domain, e.g. |rho| > 1) stands for
+     * a residual the FP64 reference finds astronomically large: actred = -1 there. */
+    const double dsse = (double)a.dsse;
+    const bool finite = dsse == dsse && dabs(dsse) <= 1.0e300;
+    const double fnorm1 = finite ? dsqrt(dmax(fn2 + dsse, 0.0)) : 1.0e300;
+    double actred = -1.0;
+    if (0.1 * fnorm1 < s.fnorm) actred = -dsse / fn2; /* = 1 - (fnorm1/fnorm)^2 */
+    double H[NP<P>::S];
+    double gd[P];
+    to_double<P>(s.H, s.g, H, gd);
+    /* |J dx|^2 = dx^T H dx */
+    double jdx2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        double row = 0.0;
+#pragma unroll
+        for (int j = 0; j < P; ++j) row += H[tri(i, j)] * s.dx[j];
+        jdx2 += s.dx[i] * row;
+    }
+    const double pnorm = scaled_norm<P>(s.diag, s.dx);
+    const double t1sq = dmax(jdx2, 0.0) / fn2;
+    const double t2 = (dsqrt(s.par) * pnorm) / s.fnorm;
+    const double prered = t1sq + t2 * t2 / 0.5;
+    const double dirder = -(t1sq + t2 * t2);
+    const double ratio = prered > 0.0 ? actred / prered : 0.0;
+    if (ratio > 0.25) {
+        if (s.par == 0.0 || ratio >= 0.75) {
+            s.delta = pnorm / 0.5;
+            s.par *= 0.5;
+        }
+    } else {
+        double temp = actred >= 0.0 ? 0.5 : 0.5 * dirder / (dirder + 0.5 * actred);
+        if (0.1 * fnorm1 >= s.fnorm || temp < 0.1) temp = 0.1;
+        s.delta = temp * dmin(s.delta, pnorm / 0.1);
+        s.par /= temp;
+    }
+    int st = LZR_LM_CONTINUE;
+    if (ratio >= 1e-4) {
+        /* accepted: x, f, J move to the trial point */
+#pragma unroll
+        for (int i = 0; i < P; ++i) s.th[i] += s.dx[i];
+        s.accepted++;
+        assemble<P, Acc>(a, npix, cfg.sigma, s.H, s.g);
+        double absx[P];
+        absolute<P>(s, s.th, absx);
+        s.xnorm = scaled_norm<P>(s.diag, absx);
+        s.fnorm = fnorm1;
+        s.iter_gsl++;
+        double cn[P];
+        const int rows = cfg.diag_update_rows < 0 ? P : (cfg.diag_update_rows == 0 ? npix : cfg.diag_update_rows);
+        if (rows >= npix) {
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const double hii = (double)s.H[tri(i, i)];
+                cn[i] = dsqrt(hii == 0.0 ? 1.0 : hii);
+            }
+        } else {
+            partial_column_norms<P>(s.th, s.n, cfg.sigma, rows, cn);
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+            if (cn[i] > s.diag[i]) s.diag[i] = cn[i];
+        /* iterate() returned GSL_SUCCESS: the reference tests the step */
+        st = test_delta<P>(s, cfg);
+        if (st == LZR_LM_CONTINUE && s.iterations < cfg.max_iterations) {
+            if (begin_iterate<P>(s, cfg, &st)) return true;
+        }
+        finish<P>(s, cfg, st, model, out);
+        return false;
+    }
+    if (dabs(actred) <= kDblEpsilon && prered <= kDblEpsilon && 0.5 * ratio <= 1.0) {
+        st = LZR_LM_ETOLF;
+    } else if (s.delta <= kDblEpsilon * s.xnorm) {
+        st = LZR_LM_ETOLX;
+    } else if (s.gnorm <= kDblEpsilon) {
+        st = LZR_LM_ETOLG;
+    } else if (s.inner >= 10) {
+        st = LZR_LM_ENOPROG;
+    } else {
+        propose<P>(s);
+        return true;
+    }
+    finish<P>(s, cfg, st, model, out);
+    return false;
+}
+
+/* 2x2 eigen-decomposition of the ellipsoidal covariance (source/Localization.cpp:525-547):
+ * ascending eigenvalues, theta = atan(v0/v1) of the first eigenvector (0 if v1 == 0). */
+LZR_HD LZR_INLINE bool eigen2(double sx, double sy, double rho, double &sd1, double &sd2, double &theta) {
+    const double a = sx * sx, c = sy * sy, b = rho * sx * sy;
+    const double lim = 1.79769313486231570815e+308;
+    if (!(dabs(a) <= lim) || !(dabs(b) <= lim) || !(dabs(c) <= lim)) return false;
+    const double hd = 0.5 * (a - c), mean = 0.5 * (a + c);
+    const double rad = dsqrt(hd * hd + b * b);
+    sd1 = dsqrt(mean - rad);
+    sd2 = dsqrt(mean + rad);
+    if (b == 0.0) {
+        theta = 0.0;
+    } else {
+        const double ratio = hd >= 0.0 ? -b / (hd + rad) : (hd - rad) / b;
+#if defined(__HIP_DEVICE_COMPILE__)
+        theta = atan(ratio);
+#else
+        theta = std::atan(ratio);
+#endif
+    }
+    return true;
+}
+
+LZR_HD LZR_INLINE double dcos(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return cos(v);
+#else
+    return std::cos(v);
+#endif
+}
+LZR_HD LZR_INLINE double dsin(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return sin(v);
+#else
+    return std::sin(v);
+#endif
+}
+
+/* Diagonal of (J^T J)^-1 at the final point (gsl_multifit_covar(J, 0.0, C) diagonal).
+ * Returns false if J^T J is singular. */
+template <int P>
+LZR_HD LZR_INLINE bool covariance_diagonal(const float (&Hf)[NP<P>::S], double (&cdiag)[P]) {
+    double H[NP<P>::S], L[NP<P>::S], id[P];
+#pragma unroll
+    for (int i = 0; i < NP<P>::S; ++i) H[i] = (double)Hf[i];
+    if (!cholesky<P>(H, L, id)) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) cdiag[i] = 0.0;
+        return false;
+    }
+    /* C = L^-T L^-1; C_kk = sum_i (L^-1)_{ik}^2 */
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        double e[P], y[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) e[i] = i == k ? 1.0 : 0.0;
+        forward<P>(L, id, e, y);
+        double s2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < P; ++i) s2 += y[i] * y[i];
+        cdiag[k] = s2;
+    }
+    return true;
+}
+
+/* Post-checks and output record of the reference drivers
+ * (source/Localization.cpp:157-211 symmetric, :506-596 ellipsoidal, :606-631 filter). */
+template <int P>
+LZR_HD LZR_INLINE void finish(LaneState<P> &s, const LaneConfig &cfg, int lm_status, int model, LzrFit &out) {
+    clear_record(out);
+    out.iterations = (int16_t)s.iterations;
+    out.lm_status = lm_status;
+    out.trials = (int16_t)s.trials;
+    out.accepted_steps = (int16_t)s.accepted;
+    out.model = (int16_t)model;
+    out.chi = s.fnorm;
+    double x[P];
+    absolute<P>(s, s.th, x);
+    const double w = cfg.initial_psf_width;
+    const double xo = s.xoff, yo = s.yoff, xm = s.xoff + s.n - 1, ym = s.yoff + s.n - 1;
+    const int h = (s.n - 1) / 2;
+    s.phase = kIdle;
+    if (P == 5) {
+        x[1] = dabs(x[1]);
+#pragma unroll
+        for (int i = 0; i < P; ++i) out.params[i] = x[i];
+        if (x[0] <= 0) {
+            out.status = LZR_REJECT_AMPLITUDE;
+            return;
+        }
+        if (x[2] < xo || x[2] > xm || x[3] < yo || x[3] > ym) {
+            out.status = LZR_REJECT_POSITION;
+            return;
+        }
+        if (x[1] < 0.4 * w || x[1] > w * 1.5) {
+            out.status = LZR_REJECT_WIDTH;
+            return;
+        }
+        double cd[P];
+        covariance_diagonal<P>(s.H, cd);
+        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 5);
+        const double q = s.fnorm / dsqrt(dof);
+        const double c = 1.0 > q ? 1.0 : q;
+#pragma unroll
+        for (int i = 0; i < P; ++i) out.deviations[i] = c * dsqrt(cd[i]);
+        const double width = x[1];
+        out.integral = 2 * kPi * width * width * x[0];
+        const double relA = c * dsqrt(cd[0]) / x[0];
+        const double relW = out.deviations[1] / width;
+        out.integral_deviation = 2 * kPi * out.integral * dsqrt(relA * relA + 2 * relW * relW);
+        out.sd1 = out.sd2 = width;
+        out.sd1_deviation = out.sd2_deviation = out.deviations[1];
+        out.theta = 0.0;
+        out.status = LZR_ACCEPTED;
+        return;
+    }
+    x[1] = dabs(x[1]);
+    x[2] = dabs(x[2]);
+#pragma unroll
+    for (int i = 0; i < P; ++i) out.params[i] = x[i];
+    if (x[0] <= 0) {
+        out.status = LZR_REJECT_AMPLITUDE;
+        return;
+    }
+    if (x[3] < xo || x[3] > xm || x[4 % P] < yo || x[4 % P] > ym) {
+        out.status = LZR_REJECT_POSITION;
+        return;
+    }
+    double sd1, sd2, theta;
+    if (!eigen2(x[1], x[2], x[5 % P], sd1, sd2, theta)) {
+        out.status = LZR_REJECT_EIGEN;
+        return;
+    }
+    if (!(((sd1 >= 0.4 * w) && (sd1 <= 1.5 * w)) || ((sd2 >= 0.4 * w) && (sd2 <= 1.5 * w)))) {
+        out.status = LZR_REJECT_WIDTH;
+        return;
+    }
+    double cd[P];
+    covariance_diagonal<P>(s.H, cd);
+    const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
+    const double q = s.fnorm / dsqrt(dof);
+    const double c = 1.0 > q ? 1.0 : q;
+#pragma unroll
+    for (int i = 0; i < P; ++i) out.deviations[i] = c * dsqrt(cd[i]);
+    const double rho = x[5 % P];
+    const double corr_dev = out.deviations[5 % P];
+    const double sxd = out.deviations[1], syd = out.deviations[2];
+    out.sd1 = sd1;
+    out.sd2 = sd2;
+    out.theta = theta;
+    out.sd1_deviation = sxd * dcos(theta) + syd * dsin(theta);
+    out.sd2_deviation = sxd * dsin(theta) + syd * dcos(theta);
+    const double omr = 1 - rho * rho;
+    out.integral = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2;
+    const double relA = out.deviations[0] / x[0];
+    out.integral_deviation = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2 *
+                             dsqrt(relA * relA + rho * rho * corr_dev * corr_dev / omr / omr +
+                                   out.sd1_deviation * out.sd1_deviation / sd1 / sd1 +
+                                   out.sd2_deviation * out.sd2_deviation / sd2 / sd2);
+    out.status = LZR_ACCEPTED;
+    if (model == LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF) {
+        if ((0.75 * sd1 > sd2) || (1.25 * sd1 < sd2))
+            out.status = LZR_REJECT_ANISOTROPIC;
+        else if (((sd1 < w * 0.7) || (sd1 > w * 1.3)) && ((sd2 < w * 0.7) || (sd2 > w * 1.3)))
+            out.status = LZR_REJECT_PSF_WIDTH;
+    }
+}
+
+}  // namespace lzr
+
+#endif /* LZR_LANE_H */

@@ -1,0 +1,116 @@
+"""ctypes binding of the C ABI (include/localizer_hip.h, localizer_amd/lib/liblocalizer_hip.so).
+
+Every call goes through the HIP library; there is no CPU path.  Loading fails loudly
+when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+
+_LIB = None
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "liblocalizer_hip.so")
+
+# Every function declared in include/localizer_hip.h.
+EXPORTED = (
+    "lzr_create",
+    "lzr_destroy",
+    "lzr_last_error",
+    "lzr_set_option",
+    "lzr_fit_batch",
+    "lzr_fit_batch_device",
+    "lzr_abi_version",
+)
+
+
+class LocalizerError(RuntimeError):
+    """Raised when the C ABI returns -1 (the reference raises std::runtime_error)."""
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise LocalizerError(f"{LIB_PATH} is missing: build it with `python -m localizer_amd.build hip`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.lzr_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.lzr_create.restype = ctypes.c_int
+        L.lzr_destroy.argtypes = [vp]
+        L.lzr_destroy.restype = ctypes.c_int
+        L.lzr_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+        L.lzr_last_error.restype = ctypes.c_int
+        L.lzr_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.lzr_set_option.restype = ctypes.c_int
+        L.lzr_fit_batch.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp]
+        L.lzr_fit_batch.restype = ctypes.c_int
+        L.lzr_fit_batch_device.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp]
+        L.lzr_fit_batch_device.restype = ctypes.c_int
+        L.lzr_abi_version.argtypes = []
+        L.lzr_abi_version.restype = ctypes.c_int
+        if L.lzr_abi_version() != 1:
+            raise LocalizerError("ABI version mismatch")
+        _LIB = L
+    return _LIB
+
+
+def _error(ctx):
+    buf = ctypes.create_string_buffer(1024)
+    lib().lzr_last_error(ctx, buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+class Context:
+    """One fitting context (lzr_create); the Python face of a FitPositions_* instance."""
+
+    def __init__(self, model, initial_psf_width, sigma=1.0, default_half_width=0):
+        h = ctypes.c_void_p()
+        if lib().lzr_create(ctypes.byref(h), int(model), float(initial_psf_width), float(sigma),
+                            int(default_half_width)) != 0:
+            raise LocalizerError(_error(None))
+        self._h = h
+        self.model = int(model)
+        self.initial_psf_width = float(initial_psf_width)
+        self.sigma = float(sigma)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lzr_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, option, value):
+        if lib().lzr_set_option(self._h, int(option), float(value)) != 0:
+            raise LocalizerError(_error(self._h))
+
+    def fit(self, frames, candidates):
+        """Host arrays in, FIT_DTYPE array out (lzr_fit_batch)."""
+        frames = np.ascontiguousarray(frames, np.uint16)
+        if frames.ndim == 2:
+            frames = frames[None]
+        nf, ny, nx = frames.shape
+        cands = np.ascontiguousarray(candidates, abi.CANDIDATE_DTYPE)
+        out = np.zeros(cands.size, abi.FIT_DTYPE)
+        rc = lib().lzr_fit_batch(self._h, frames.ctypes.data, nx, ny, nf, cands.ctypes.data, cands.size,
+                                 out.ctypes.data)
+        if rc < 0:
+            raise LocalizerError(_error(self._h))
+        return out
+
+    def fit_device(self, d_frames, nx, ny, nframes, d_cands, ncands, d_out, stream=0):
+        """Device pointers in (ints), enqueued on `stream` (lzr_fit_batch_device)."""
+        rc = lib().lzr_fit_batch_device(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                        ctypes.c_void_p(d_cands), int(ncands), ctypes.c_void_p(d_out),
+                                        ctypes.c_void_p(stream))
+        if rc < 0:
+            raise LocalizerError(_error(self._h))
+        return rc

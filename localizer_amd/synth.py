@@ -1,0 +1,262 @@
+"""Synthetic single-molecule workloads (ctypes front of csrc/synth.c).
+
+The standard workloads below are the benchmark configurations of this engine:
+frames of uint16 with Poisson noise, candidates at the rounded true positions.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from . import abi
+
+SEED = 0x10CA112E
+_LIB = None
+
+
+class _Spec(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint32),
+        ("config_id", ctypes.c_uint32),
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nframes", ctypes.c_int),
+        ("emitters_per_frame", ctypes.c_int),
+        ("sigma_min", ctypes.c_double),
+        ("sigma_max", ctypes.c_double),
+        ("photons_min", ctypes.c_double),
+        ("photons_max", ctypes.c_double),
+        ("bg_min", ctypes.c_double),
+        ("bg_max", ctypes.c_double),
+        ("bg_gradient", ctypes.c_double),
+        ("half_width_min", ctypes.c_int),
+        ("half_width_max", ctypes.c_int),
+        ("ellipsoidal_fraction", ctypes.c_double),
+        ("sigma_xy_min", ctypes.c_double),
+        ("sigma_xy_max", ctypes.c_double),
+        ("pair_fraction", ctypes.c_double),
+        ("pair_distance_sigmas", ctypes.c_double),
+        ("margin", ctypes.c_int),
+        ("candidate_model_from_truth", ctypes.c_int),
+    ]
+
+
+@dataclass
+class Workload:
+    """One synthetic workload and the fitter settings it is measured with."""
+
+    name: str
+    config_id: int
+    nx: int
+    ny: int
+    nframes: int
+    emitters_per_frame: int
+    sigma: tuple
+    photons: tuple
+    background: tuple
+    half_width: tuple  # (min, max) candidate window half-width
+    initial_psf_width: float
+    model: int = abi.MODEL_SYMMETRIC
+    bg_gradient: float = 0.0
+    ellipsoidal_fraction: float = 0.0
+    sigma_xy: tuple = (1.0, 2.0)
+    pair_fraction: float = 0.0
+    pair_distance_sigmas: float = 2.0
+    candidate_model_from_truth: bool = False
+    seed: int = SEED
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n_rois(self):
+        return self.nframes * self.emitters_per_frame
+
+    def scaled(self, nframes):
+        """The same workload with fewer (or more) frames."""
+        return replace(self, nframes=int(nframes))
+
+
+# Plain method, CPU-runnable: 100 frames of 512x512, 20 emitters/frame, sigma 1.3,
+# photons log-uniform 500-5000, background 100, 7x7 window (h = 3), symmetric model.
+PLAIN_CPU_512 = Workload(
+    name="plain-lm-512px-100frames-7x7",
+    config_id=1,
+    nx=512,
+    ny=512,
+    nframes=100,
+    emitters_per_frame=20,
+    sigma=(1.3, 1.3),
+    photons=(500.0, 5000.0),
+    background=(100.0, 100.0),
+    half_width=(3, 3),
+    initial_psf_width=1.3,
+)
+
+# Plain LM at scale: 10,000 frames of 256x256, 50 emitters/frame (500k ROIs),
+# sigma uniform 1.1-1.6, photons log-uniform 200-5000, background uniform 50-150,
+# 9x9 windows (h = 4), symmetric 5-parameter model.
+PLAIN_GPU_256 = Workload(
+    name="plain-lm-256px-10000frames-9x9",
+    config_id=2,
+    nx=256,
+    ny=256,
+    nframes=10000,
+    emitters_per_frame=50,
+    sigma=(1.1, 1.6),
+    photons=(200.0, 5000.0),
+    background=(50.0, 150.0),
+    half_width=(4, 4),
+    initial_psf_width=1.35,
+)
+
+# Batched uneven workload: window half-widths uniform 2..7, 60% symmetric and 40%
+# ellipsoidal emitters (sigma_x, sigma_y uniform 1.0-2.0) fitted with the matching
+# model; 40,000 frames of 256x256 x 50 emitters = 2M ROIs.
+MIXED_2M = Workload(
+    name="mixed-lm-2M-rois-5x5-to-15x15",
+    config_id=3,
+    nx=256,
+    ny=256,
+    nframes=40000,
+    emitters_per_frame=50,
+    sigma=(1.1, 1.6),
+    photons=(200.0, 5000.0),
+    background=(50.0, 150.0),
+    half_width=(2, 7),
+    initial_psf_width=1.5,
+    ellipsoidal_fraction=0.4,
+    sigma_xy=(1.0, 2.0),
+    candidate_model_from_truth=True,
+)
+
+# Dense end-to-end stream: 1000 frames of 2048x2048, 200 emitters/frame, 20% placed
+# within 2 sigma of a neighbour, photons log-uniform 150-3000, background 100 plus a
+# 0-50 gradient along x, 9x9 windows.
+DENSE_2048 = Workload(
+    name="dense-stream-2048px-1000frames",
+    config_id=5,
+    nx=2048,
+    ny=2048,
+    nframes=1000,
+    emitters_per_frame=200,
+    sigma=(1.3, 1.3),
+    photons=(150.0, 3000.0),
+    background=(100.0, 100.0),
+    half_width=(4, 4),
+    initial_psf_width=1.3,
+    bg_gradient=50.0,
+    pair_fraction=0.2,
+    pair_distance_sigmas=2.0,
+)
+
+WORKLOADS = {w.name: w for w in (PLAIN_CPU_512, PLAIN_GPU_256, MIXED_2M, DENSE_2048)}
+
+
+def _lib_path():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "liblzr_synth.so")
+
+
+def lib():
+    """Loads the generator library (built by localizer_amd.build)."""
+    global _LIB
+    if _LIB is None:
+        path = _lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run localizer_amd.build first")
+        L = ctypes.CDLL(path)
+        L.lzr_synth_generate.argtypes = [
+            ctypes.POINTER(_Spec),
+            ctypes.c_void_p,
+            ctypes.c_void_p,
+            ctypes.c_void_p,
+            ctypes.c_int,
+        ]
+        L.lzr_synth_generate.restype = ctypes.c_int
+        L.lzr_synth_rois.argtypes = [
+            ctypes.POINTER(_Spec),
+            ctypes.c_int,
+            ctypes.c_int,
+            ctypes.c_double,
+            ctypes.c_void_p,
+            ctypes.c_void_p,
+            ctypes.c_void_p,
+            ctypes.c_int,
+        ]
+        L.lzr_synth_rois.restype = ctypes.c_int
+        L.lzr_philox4x32_10.argtypes = [
+            ctypes.POINTER(ctypes.c_uint32),
+            ctypes.POINTER(ctypes.c_uint32),
+            ctypes.POINTER(ctypes.c_uint32),
+        ]
+        L.lzr_philox4x32_10.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _spec(w: Workload):
+    s = _Spec()
+    s.seed = w.seed & 0xFFFFFFFF
+    s.config_id = w.config_id
+    s.nx, s.ny, s.nframes = w.nx, w.ny, w.nframes
+    s.emitters_per_frame = w.emitters_per_frame
+    s.sigma_min, s.sigma_max = w.sigma
+    s.photons_min, s.photons_max = w.photons
+    s.bg_min, s.bg_max = w.background
+    s.bg_gradient = w.bg_gradient
+    s.half_width_min, s.half_width_max = w.half_width
+    s.ellipsoidal_fraction = w.ellipsoidal_fraction
+    s.sigma_xy_min, s.sigma_xy_max = w.sigma_xy
+    s.pair_fraction = w.pair_fraction
+    s.pair_distance_sigmas = w.pair_distance_sigmas
+    s.margin = 0
+    s.candidate_model_from_truth = 1 if w.candidate_model_from_truth else 0
+    return s
+
+
+def philox4x32_10(ctr, key):
+    """One Philox-4x32-10 block (used by the known-answer test)."""
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().lzr_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def generate(w: Workload, nthreads=0, frames_out=None):
+    """Renders the workload: returns (frames[nframes, ny, nx] uint16, truth, candidates)."""
+    n = w.n_rois
+    frames = frames_out if frames_out is not None else np.empty((w.nframes, w.ny, w.nx), np.uint16)
+    assert frames.shape == (w.nframes, w.ny, w.nx) and frames.dtype == np.uint16
+    assert frames.flags.c_contiguous
+    truth = np.zeros(n, abi.EMITTER_DTYPE)
+    cands = np.zeros(n, abi.CANDIDATE_DTYPE)
+    spec = _spec(w)
+    rc = lib().lzr_synth_generate(
+        ctypes.byref(spec), frames.ctypes.data, truth.ctypes.data, cands.ctypes.data, int(nthreads)
+    )
+    if rc != 0:
+        raise ValueError(f"invalid workload {w.name}")
+    return frames, truth, cands
+
+
+def generate_rois(w: Workload, nrois, half_width, max_offset, nthreads=0):
+    """Independent (2h+1)^2 ROIs: returns (rois[nrois, n, n] uint16, truth, candidates);
+    candidate i refers to frame i (its own ROI)."""
+    n = 2 * half_width + 1
+    rois = np.empty((nrois, n, n), np.uint16)
+    truth = np.zeros(nrois, abi.EMITTER_DTYPE)
+    cands = np.zeros(nrois, abi.CANDIDATE_DTYPE)
+    spec = _spec(w)
+    rc = lib().lzr_synth_rois(
+        ctypes.byref(spec),
+        int(nrois),
+        int(half_width),
+        float(max_offset),
+        rois.ctypes.data,
+        truth.ctypes.data,
+        cands.ctypes.data,
+        int(nthreads),
+    )
+    if rc != 0:
+        raise ValueError("invalid ROI request")
+    return rois, truth, cands

@@ -1,0 +1,174 @@
+"""ctypes front of the FP64 oracle (oracle/liblzr_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, never by the product package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from localizer_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("max_iterations", ctypes.c_int),
+        ("epsabs", ctypes.c_double),
+        ("epsrel", ctypes.c_double),
+        ("diag_update_rows", ctypes.c_int),
+        ("inner_cap", ctypes.c_int),
+    ]
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int),
+        ("initial_psf_width", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("default_half_width", ctypes.c_int),
+        ("opt", Options),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liblzr_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: build the oracle (make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.orc_default_options.argtypes = [ctypes.POINTER(Options)]
+        L.orc_model.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_model.restype = ctypes.c_int
+        L.orc_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_jacobian.restype = ctypes.c_int
+        L.orc_nrm2.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_nrm2.restype = ctypes.c_double
+        L.orc_test_delta.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        L.orc_test_delta.restype = ctypes.c_int
+        L.orc_covar.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_covar.restype = None
+        L.orc_lm_run.argtypes = [
+            ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            _dp, ctypes.POINTER(Options), _dp, _dp, _dp, _ip, _ip, _ip, _dp, ctypes.c_int,
+        ]
+        L.orc_lm_run.restype = ctypes.c_int
+        L.orc_eigen2.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_eigen2.restype = ctypes.c_int
+        L.orc_fit_roi.argtypes = [ctypes.POINTER(Settings), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_fit_roi.restype = None
+        L.orc_fit_batch.argtypes = [
+            ctypes.POINTER(Settings), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+        ]
+        L.orc_fit_batch.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def default_options(minpack=False):
+    o = Options()
+    lib().orc_default_options(ctypes.byref(o))
+    if minpack:
+        o.diag_update_rows = 0
+        o.inner_cap = 0
+    return o
+
+
+def settings(model, initial_psf_width, sigma=1.0, default_half_width=0, options=None):
+    s = Settings()
+    s.model = int(model)
+    s.initial_psf_width = float(initial_psf_width)
+    s.sigma = float(sigma)
+    s.default_half_width = int(default_half_width)
+    s.opt = options if options is not None else default_options()
+    return s
+
+
+def model(model_code, theta, data, n, xoff, yoff, sigma=1.0):
+    theta = np.ascontiguousarray(theta, np.float64)
+    data = np.ascontiguousarray(data, np.float64).reshape(-1)
+    f = np.empty(n * n)
+    st = lib().orc_model(int(model_code), _ptr(theta), _ptr(data), n, float(xoff), float(yoff), float(sigma), _ptr(f))
+    return st, f
+
+
+def jacobian(model_code, theta, n, xoff, yoff, sigma=1.0):
+    theta = np.ascontiguousarray(theta, np.float64)
+    p = abi.n_params(model_code)
+    J = np.empty((n * n, p))
+    st = lib().orc_jacobian(int(model_code), _ptr(theta), n, float(xoff), float(yoff), float(sigma), _ptr(J))
+    return st, J
+
+
+def nrm2(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_nrm2(x.size, _ptr(x), 1)
+
+
+def test_delta(dx, x, epsabs=1e-4, epsrel=1e-4):
+    dx = np.ascontiguousarray(dx, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_test_delta(x.size, _ptr(dx), _ptr(x), epsabs, epsrel)
+
+
+def covar(J):
+    J = np.ascontiguousarray(J, np.float64)
+    N, p = J.shape
+    C = np.empty((p, p))
+    lib().orc_covar(N, p, _ptr(J), _ptr(C))
+    return C
+
+
+def lm_run(model_code, data, n, xoff, yoff, theta0, sigma=1.0, options=None, max_trace=0):
+    """Runs the reference LM loop on one window.  Returns a dict with theta, f, J,
+    iterations, lm_status, accepted, trials and (if max_trace) the accepted point after
+    every trial."""
+    p = abi.n_params(model_code)
+    opt = options if options is not None else default_options()
+    data = np.ascontiguousarray(data, np.float64).reshape(-1)
+    theta0 = np.ascontiguousarray(theta0, np.float64)
+    theta = np.empty(p)
+    f = np.empty(n * n)
+    J = np.empty((n * n, p))
+    it, st, acc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    trace = np.full((max(max_trace, 1), p), np.nan)
+    trials = lib().orc_lm_run(
+        int(model_code), _ptr(data), n, float(xoff), float(yoff), float(sigma), _ptr(theta0),
+        ctypes.byref(opt), _ptr(theta), _ptr(f), _ptr(J), ctypes.byref(it), ctypes.byref(st),
+        ctypes.byref(acc), _ptr(trace) if max_trace else None, int(max_trace),
+    )
+    return dict(theta=theta, f=f, J=J, iterations=it.value, lm_status=st.value, accepted=acc.value,
+                trials=trials, trace=trace[: min(trials, max_trace)] if max_trace else None)
+
+
+def eigen2(sx, sy, rho):
+    a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    st = lib().orc_eigen2(sx, sy, rho, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return st, a.value, b.value, c.value
+
+
+def fit_batch(frames, candidates, model_code, initial_psf_width, sigma=1.0, default_half_width=0,
+              nthreads=0, options=None):
+    """Fits every candidate with the frame-parallel pool; returns an array of FIT_DTYPE."""
+    frames = np.ascontiguousarray(frames, np.uint16)
+    if frames.ndim == 2:
+        frames = frames[None]
+    nframes, ny, nx = frames.shape
+    cands = np.ascontiguousarray(candidates, abi.CANDIDATE_DTYPE)
+    out = np.zeros(cands.size, abi.FIT_DTYPE)
+    s = settings(model_code, initial_psf_width, sigma, default_half_width, options)
+    lib().orc_fit_batch(ctypes.byref(s), frames.ctypes.data, nx, ny, nframes, cands.ctypes.data,
+                        cands.size, out.ctypes.data, int(nthreads))
+    return out

@@ -1,0 +1,63 @@
+// Host build of the GPU lane logic (localizer_amd/csrc/lzr_lane.h) for the CPU tests:
+// every candidate runs the same state machine and the same FP32 pixel pass as one GPU
+// lane, sequentially.  TEST INFRASTRUCTURE ONLY -- the product never calls it.
+#include <cstring>
+
+#include "../../localizer_amd/csrc/lzr_lane.h"
+
+using namespace lzr;
+
+template <int P, typename Acc, bool F64>
+static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int ny, int nframes,
+                    const LzrCandidate &c, int idx, int model, LzrFit &out) {
+    LaneState<P> s;
+    memset(&out, 0, sizeof out);
+    if (!lane_start<P>(s, cfg, c, idx, nx, ny, nframes, model, out)) return;
+    const int n = s.n;
+    float D[(2 * LZR_MAX_HALF_WIDTH + 1) * (2 * LZR_MAX_HALF_WIDTH + 1)];
+    const uint16_t *src = frames + ((size_t)s.frame * ny + s.yoff) * nx + s.xoff;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) D[i + n * j] = (float)src[(size_t)j * nx + i];
+    for (;;) {
+        PassIn<P> q;
+        lane_pass_in<P>(s, cfg.sigma, q);
+        PassAcc<P, Acc> a;
+        a.zero();
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) pass_pixel<F64>(q, (float)i, (float)j, D[i + n * j], a);
+        if (!lane_after_pass<P, Acc>(s, cfg, a, model, out)) return;
+    }
+}
+
+extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, int max_h, int diag_rows,
+                             int max_iter, double epsabs, double epsrel, int acc_double,
+                             const uint16_t *frames, int nx, int ny, int nframes,
+                             const LzrCandidate *cands, int ncands, LzrFit *out) {
+    LaneConfig cfg;
+    cfg.initial_psf_width = w;
+    cfg.sigma = sigma;
+    cfg.epsabs = epsabs;
+    cfg.epsrel = epsrel;
+    cfg.max_iterations = max_iter;
+    cfg.default_half_width = default_h;
+    cfg.max_half_width = max_h;
+    cfg.diag_update_rows = diag_rows;
+    cfg.model = model;
+    int accepted = 0;
+    for (int i = 0; i < ncands; ++i) {
+        const int m = cands[i].model >= 0 ? cands[i].model : model;
+        const bool f64 = acc_double & 2;
+        const bool dacc = acc_double & 1;
+#define LZR_EMU_CALL(PP, AA, FF) fit_one<PP, AA, FF>(cfg, frames, nx, ny, nframes, cands[i], i, m, out[i])
+        if (m == LZR_MODEL_SYMMETRIC) {
+            if (dacc) { if (f64) LZR_EMU_CALL(5, double, true); else LZR_EMU_CALL(5, double, false); }
+            else { if (f64) LZR_EMU_CALL(5, float, true); else LZR_EMU_CALL(5, float, false); }
+        } else {
+            if (dacc) { if (f64) LZR_EMU_CALL(7, double, true); else LZR_EMU_CALL(7, double, false); }
+            else { if (f64) LZR_EMU_CALL(7, float, true); else LZR_EMU_CALL(7, float, false); }
+        }
+#undef LZR_EMU_CALL
+        accepted += out[i].status == LZR_ACCEPTED;
+    }
+    return accepted;
+}
