@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--frames", type=int, default=0, help="override the frame count (testing)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--occupancy", type=int, default=0, help="fitting-kernel register budget (1 or 2 waves/SIMD)")
+    ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent grid size per CU (0: resident max)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -111,6 +113,10 @@ def main():
     d_out = torch.zeros(cands.size * abi.FIT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
     ctx = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, 1.0, w.half_width[0])
+    if args.occupancy:
+        ctx.set_option(abi.OPT_OCCUPANCY, args.occupancy)
+    if args.waves_per_cu:
+        ctx.set_option(abi.OPT_WAVES_PER_CU, args.waves_per_cu)
     stream = torch.cuda.current_stream()
 
     def step():

@@ -38,6 +38,7 @@ OPT_EPSREL = 3
 OPT_DIAG_UPDATE_ROWS = 4
 OPT_WAVES_PER_CU = 5
 OPT_MAX_HALF_WIDTH = 6
+OPT_OCCUPANCY = 7
 
 CANDIDATE_DTYPE = np.dtype(
     [

@@ -36,10 +36,6 @@
         if (e_ != hipSuccess) throw std::string(#call ": ") + hipGetErrorString(e_);         \
     } while (0)
 
-#ifndef LZR_WAVES_PER_EU
-#define LZR_WAVES_PER_EU 1
-#endif
-
 namespace lzr {
 
 struct KernelArgs {
@@ -60,12 +56,36 @@ __device__ __forceinline__ int model_params(int model) { return model == LZR_MOD
 __global__ void classify_kernel(const LzrCandidate *cands, int ncands, int default_model, int *order,
                                 unsigned *counters) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int cls = -1;
+    if (i < ncands) {
+        int model = cands[i].model;
+        if (model < 0) model = default_model;
+        cls = model_params(model) == 5 ? 0 : 1;
+    }
+    /* one atomic per wave and class */
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const unsigned long long m = __ballot(cls == c);
+        if (!m) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&counters[c], (unsigned)__popcll(m));
+        base = __shfl(base, leader, 64);
+        if (cls == c) {
+            const unsigned slot = base + (unsigned)__popcll(m & below);
+            order[c == 0 ? slot : ncands - 1 - slot] = i;
+        }
+    }
+}
+
+/* Turns the raw end states into the reference records (post-checks, covariance,
+ * 2x2 eigen-decomposition), one thread per candidate. */
+__global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, LzrFit *out, LaneConfig cfg) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ncands) return;
-    int model = cands[i].model;
-    if (model < 0) model = default_model;
-    const int cls = model_params(model) == 5 ? 0 : 1;
-    const unsigned slot = atomicAdd(&counters[cls], 1u);
-    order[cls == 0 ? slot : ncands - 1 - slot] = i;
+    finish_any(out[i], cfg, cands[i]);
 }
 
 __global__ void reset_counters(unsigned *counters) {
@@ -78,8 +98,9 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-template <int P, typename Acc, bool F64>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LZR_WAVES_PER_EU))) lm_persistent(KernelArgs args, int cls, int ncands_total) {
+/* W = minimum waves per SIMD the register allocation must allow (occupancy target). */
+template <int P, typename Acc, bool F64, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) lm_persistent(KernelArgs args, int cls, int ncands_total) {
     extern __shared__ uint16_t tile[];
     const int lane = threadIdx.x;
     const unsigned count = args.count[cls];
@@ -158,7 +179,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(LZR_WAV
 
         /* ---- LM logic ---- */
         if (active) {
-            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, args.out[s.roi]))
+            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, *reinterpret_cast<LzrRawFit *>(args.out + s.roi)))
                 s.phase = open ? kIdle : kFinished;
         }
         __syncthreads();
@@ -177,6 +198,7 @@ struct LzrContext {
     int device = 0;
     LaneConfig cfg{};
     int waves_per_cu = 0;
+    int occupancy5 = 2; /* register budget of the 5-parameter kernel: 1 or 2 waves/SIMD */
     int num_cus = 0;
     std::string err;
     std::mutex mu;
@@ -206,10 +228,10 @@ static void ensure(T *&ptr, size_t &cap, size_t need) {
     cap = n;
 }
 
-template <int P, typename Acc, bool F64>
+template <int P, typename Acc, bool F64, int W>
 static int grid_for(LzrContext *ctx, size_t lds) {
     int per_cu = 0;
-    LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, F64>, 64, lds));
+    LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, F64, W>, 64, lds));
     if (per_cu < 1) throw std::string("fitting kernel cannot be resident with this window size");
     if (ctx->waves_per_cu > 0) per_cu = std::min(per_cu, ctx->waves_per_cu);
     return per_cu * ctx->num_cus;
@@ -238,10 +260,17 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.cfg = ctx->cfg;
     const int nmax = 2 * ctx->cfg.max_half_width + 1;
     const size_t lds = (size_t)nmax * nmax * 64 * sizeof(uint16_t);
-    const int g5 = grid_for<5, AccT, kF64Residual>(ctx, lds);
-    hipLaunchKernelGGL((lm_persistent<5, AccT, kF64Residual>), dim3(g5), dim3(64), lds, stream, a, 0, ncands);
-    const int g7 = grid_for<7, AccT, kF64Residual>(ctx, lds);
-    hipLaunchKernelGGL((lm_persistent<7, AccT, kF64Residual>), dim3(g7), dim3(64), lds, stream, a, 1, ncands);
+    if (ctx->occupancy5 == 1) {
+        const int g5 = grid_for<5, AccT, kF64Residual, 1>(ctx, lds);
+        hipLaunchKernelGGL((lm_persistent<5, AccT, kF64Residual, 1>), dim3(g5), dim3(64), lds, stream, a, 0, ncands);
+    } else {
+        const int g5 = grid_for<5, AccT, kF64Residual, 2>(ctx, lds);
+        hipLaunchKernelGGL((lm_persistent<5, AccT, kF64Residual, 2>), dim3(g5), dim3(64), lds, stream, a, 0, ncands);
+    }
+    const int g7 = grid_for<7, AccT, kF64Residual, 1>(ctx, lds);
+    hipLaunchKernelGGL((lm_persistent<7, AccT, kF64Residual, 1>), dim3(g7), dim3(64), lds, stream, a, 1, ncands);
+    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, out,
+                       ctx->cfg);
     LZR_CHECK(hipGetLastError());
 }
 
@@ -345,6 +374,10 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
     case LZR_OPT_WAVES_PER_CU:
         if (!(value >= 0 && value <= 64)) return fail(ctx, "waves per CU out of range");
         ctx->waves_per_cu = (int)value;
+        return 0;
+    case LZR_OPT_OCCUPANCY:
+        if (!(value == 1 || value == 2)) return fail(ctx, "occupancy must be 1 or 2");
+        ctx->occupancy5 = (int)value;
         return 0;
     case LZR_OPT_MAX_HALF_WIDTH:
         if (!(value >= 1 && value <= LZR_MAX_HALF_WIDTH)) return fail(ctx, "max half-width out of range");

@@ -77,6 +77,20 @@ struct NP {
     static constexpr int SQ = Q * (Q + 1) / 2;
 };
 
+/* Raw end state of one fit, written by the fitting kernel into the caller's LzrFit
+ * slot (same size and layout of the trailing fields) and turned into the reference
+ * record by the epilogue (finish_record).  th is in absolute coordinates. */
+struct LzrRawFit {
+    double th[LZR_MAX_PARAMS];
+    float H[28]; /* J^T J at th, packed (P(P+1)/2 used) */
+    double chi;
+    int32_t status; /* kPending */
+    int32_t lm_status;
+    int16_t iterations, trials, accepted_steps, model;
+};
+static_assert(sizeof(LzrRawFit) == sizeof(LzrFit), "raw record must overlay LzrFit");
+constexpr int32_t kPending = -1;
+
 /* Per-lane LM state.  x0/y0 are kept window-local (exact integer offsets xoff/yoff);
  * every quantity the reference computes from absolute coordinates (test_delta,
  * |D x|) uses xoff + local. */
@@ -718,14 +732,31 @@ LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int
     return true;
 }
 
+/* Stores the raw end state of the lane's fit (the reference's x, J^T J, |f| and loop
+ * counters) for the epilogue. */
 template <int P>
-LZR_HD LZR_INLINE void finish(LaneState<P> &s, const LaneConfig &cfg, int lm_status, int model, LzrFit &out);
+LZR_HD LZR_INLINE void store_final(LaneState<P> &s, int lm_status, int model, LzrRawFit &o) {
+    double x[P];
+    absolute<P>(s, s.th, x);
+#pragma unroll
+    for (int i = 0; i < P; ++i) o.th[i] = x[i];
+#pragma unroll
+    for (int i = 0; i < NP<P>::S; ++i) o.H[i] = s.H[i];
+    o.chi = s.fnorm;
+    o.status = kPending;
+    o.lm_status = lm_status;
+    o.iterations = (int16_t)s.iterations;
+    o.trials = (int16_t)s.trials;
+    o.accepted_steps = (int16_t)s.accepted;
+    o.model = (int16_t)model;
+    s.phase = kIdle;
+}
 
 /* Consumes one pass.  Returns true if the lane needs another pass (a new trial), false
  * when the fit is complete (out written). */
 template <int P, typename Acc>
 LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int model,
-                                       LzrFit &out) {
+                                       LzrRawFit &out) {
     const int npix = s.n * s.n;
     if (s.phase == kInit) {
         /* lmsder set(): f, J at the start point */
@@ -744,7 +775,7 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
         s.iter_gsl = 1;
         int st = 0;
         if (begin_iterate<P>(s, cfg, &st)) return true;
-        finish<P>(s, cfg, st, model, out);
+        store_final<P>(s, st, model, out);
         return false;
     }
     /* trial evaluated at th + dx */
@@ -817,7 +848,7 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
         if (st == LZR_LM_CONTINUE && s.iterations < cfg.max_iterations) {
             if (begin_iterate<P>(s, cfg, &st)) return true;
         }
-        finish<P>(s, cfg, st, model, out);
+        store_final<P>(s, st, model, out);
         return false;
     }
     if (dabs(actred) <= kDblEpsilon && prered <= kDblEpsilon && 0.5 * ratio <= 1.0) {
@@ -832,7 +863,7 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
         propose<P>(s);
         return true;
     }
-    finish<P>(s, cfg, st, model, out);
+    store_final<P>(s, st, model, out);
     return false;
 }
 
@@ -902,22 +933,30 @@ LZR_HD LZR_INLINE bool covariance_diagonal(const float (&Hf)[NP<P>::S], double (
 }
 
 /* Post-checks and output record of the reference drivers
- * (source/Localization.cpp:157-211 symmetric, :506-596 ellipsoidal, :606-631 filter). */
+ * (source/Localization.cpp:157-211 symmetric, :506-596 ellipsoidal, :606-631 filter),
+ * from the raw end state r of candidate c. */
 template <int P>
-LZR_HD LZR_INLINE void finish(LaneState<P> &s, const LaneConfig &cfg, int lm_status, int model, LzrFit &out) {
-    clear_record(out);
-    out.iterations = (int16_t)s.iterations;
-    out.lm_status = lm_status;
-    out.trials = (int16_t)s.trials;
-    out.accepted_steps = (int16_t)s.accepted;
-    out.model = (int16_t)model;
-    out.chi = s.fnorm;
-    double x[P];
-    absolute<P>(s, s.th, x);
+LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, LzrFit &out) {
+    const int model = r.model;
+    const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
+    const int n = 2 * h + 1;
+    const double xo = (double)(int)(c.x - (double)h), yo = (double)(int)(c.y - (double)h);
+    const double xm = xo + n - 1, ym = yo + n - 1;
     const double w = cfg.initial_psf_width;
-    const double xo = s.xoff, yo = s.yoff, xm = s.xoff + s.n - 1, ym = s.yoff + s.n - 1;
-    const int h = (s.n - 1) / 2;
-    s.phase = kIdle;
+    double x[P];
+    float Hf[NP<P>::S];
+#pragma unroll
+    for (int i = 0; i < P; ++i) x[i] = r.th[i];
+#pragma unroll
+    for (int i = 0; i < NP<P>::S; ++i) Hf[i] = r.H[i];
+    const double chi = r.chi;
+    clear_record(out);
+    out.iterations = r.iterations;
+    out.lm_status = r.lm_status;
+    out.trials = r.trials;
+    out.accepted_steps = r.accepted_steps;
+    out.model = (int16_t)model;
+    out.chi = chi;
     if (P == 5) {
         x[1] = dabs(x[1]);
 #pragma unroll
@@ -935,15 +974,15 @@ LZR_HD LZR_INLINE void finish(LaneState<P> &s, const LaneConfig &cfg, int lm_sta
             return;
         }
         double cd[P];
-        covariance_diagonal<P>(s.H, cd);
+        covariance_diagonal<P>(Hf, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 5);
-        const double q = s.fnorm / dsqrt(dof);
-        const double c = 1.0 > q ? 1.0 : q;
+        const double q = chi / dsqrt(dof);
+        const double cs = 1.0 > q ? 1.0 : q;
 #pragma unroll
-        for (int i = 0; i < P; ++i) out.deviations[i] = c * dsqrt(cd[i]);
+        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
         const double width = x[1];
         out.integral = 2 * kPi * width * width * x[0];
-        const double relA = c * dsqrt(cd[0]) / x[0];
+        const double relA = cs * dsqrt(cd[0]) / x[0];
         const double relW = out.deviations[1] / width;
         out.integral_deviation = 2 * kPi * out.integral * dsqrt(relA * relA + 2 * relW * relW);
         out.sd1 = out.sd2 = width;
@@ -974,12 +1013,12 @@ LZR_HD LZR_INLINE void finish(LaneState<P> &s, const LaneConfig &cfg, int lm_sta
         return;
     }
     double cd[P];
-    covariance_diagonal<P>(s.H, cd);
+    covariance_diagonal<P>(Hf, cd);
     const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
-    const double q = s.fnorm / dsqrt(dof);
-    const double c = 1.0 > q ? 1.0 : q;
+    const double q = chi / dsqrt(dof);
+    const double cs = 1.0 > q ? 1.0 : q;
 #pragma unroll
-    for (int i = 0; i < P; ++i) out.deviations[i] = c * dsqrt(cd[i]);
+    for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
     const double rho = x[5 % P];
     const double corr_dev = out.deviations[5 % P];
     const double sxd = out.deviations[1], syd = out.deviations[2];
@@ -1002,6 +1041,18 @@ LZR_HD LZR_INLINE void finish(LaneState<P> &s, const LaneConfig &cfg, int lm_sta
         else if (((sd1 < w * 0.7) || (sd1 > w * 1.3)) && ((sd2 < w * 0.7) || (sd2 > w * 1.3)))
             out.status = LZR_REJECT_PSF_WIDTH;
     }
+}
+
+/* Epilogue entry: finishes a record written by store_final (no-op for records already
+ * final, i.e. rejected before fitting). */
+LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c) {
+    LzrRawFit r;
+    __builtin_memcpy(&r, &rec, sizeof r);
+    if (r.status != kPending) return;
+    if (r.model == LZR_MODEL_SYMMETRIC)
+        finish_record<5>(r, cfg, c, rec);
+    else
+        finish_record<7>(r, cfg, c, rec);
 }
 
 }  // namespace lzr

@@ -32,6 +32,13 @@ class LocalizerError(RuntimeError):
 def lib():
     global _LIB
     if _LIB is None:
+        # torch ships its own libamdhip64 (same SONAME).  Loading it first makes this
+        # library bind to that runtime instead of starting a second HIP runtime in the
+        # process, which would leave torch without a device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise LocalizerError(f"{LIB_PATH} is missing: build it with `python -m localizer_amd.build hip`")
         L = ctypes.CDLL(LIB_PATH)

@@ -26,6 +26,22 @@ def _ensure_built():
 _ensure_built()
 
 
+def pytest_collection_modifyitems(config, items):
+    """GPU tests need a HIP device; on a machine without one they are skipped."""
+    try:
+        import torch
+
+        has_gpu = torch.cuda.is_available()
+    except Exception:  # pragma: no cover
+        has_gpu = False
+    if has_gpu:
+        return
+    skip = pytest.mark.skip(reason="no HIP device")
+    for item in items:
+        if "gpu" in item.keywords:
+            item.add_marker(skip)
+
+
 @pytest.fixture(scope="session")
 def root():
     return ROOT

@@ -25,7 +25,10 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
         a.zero();
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) pass_pixel<F64>(q, (float)i, (float)j, D[i + n * j], a);
-        if (!lane_after_pass<P, Acc>(s, cfg, a, model, out)) return;
+        if (!lane_after_pass<P, Acc>(s, cfg, a, model, *reinterpret_cast<LzrRawFit *>(&out))) {
+            finish_any(out, cfg, c);
+            return;
+        }
     }
 }
 
