@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--occupancy", type=int, default=0, help="fitting-kernel register budget (1 or 2 waves/SIMD)")
     ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent grid size per CU (0: resident max)")
+    ap.add_argument("--rounds", type=int, default=0, help="fitting-kernel rounds (fits parked in between)")
+    ap.add_argument("--park-threshold", type=int, default=-1, help="busy lanes at which a wave parks its fits")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -117,6 +119,10 @@ def main():
         ctx.set_option(abi.OPT_OCCUPANCY, args.occupancy)
     if args.waves_per_cu:
         ctx.set_option(abi.OPT_WAVES_PER_CU, args.waves_per_cu)
+    if args.rounds:
+        ctx.set_option(abi.OPT_ROUNDS, args.rounds)
+    if args.park_threshold >= 0:
+        ctx.set_option(abi.OPT_PARK_THRESHOLD, args.park_threshold)
     stream = torch.cuda.current_stream()
 
     def step():

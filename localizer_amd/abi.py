@@ -39,6 +39,8 @@ OPT_DIAG_UPDATE_ROWS = 4
 OPT_WAVES_PER_CU = 5
 OPT_MAX_HALF_WIDTH = 6
 OPT_OCCUPANCY = 7
+OPT_ROUNDS = 8
+OPT_PARK_THRESHOLD = 9
 
 CANDIDATE_DTYPE = np.dtype(
     [

@@ -71,6 +71,14 @@ def build_hip():
     os.replace(tmp, out)
 
 
+def build_dropin_driver():
+    """C++ test driver of the drop-in classes (links the HIP library)."""
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", os.path.join(cpp, "dropin_main"),
+          os.path.join(cpp, "dropin_main.cpp"), f"-L{LIB}", "-llocalizer_hip",
+          "-Wl,-rpath,$ORIGIN/../../localizer_amd/lib", "-lpthread"])
+
+
 def build_host_libs():
     build_synth()
     build_oracle()
@@ -81,10 +89,11 @@ def build_all():
     build_host_libs()
     build_ref()
     build_hip()
+    build_dropin_driver()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "host": build_host_libs, "ref": build_ref,
-         "emu": build_emu, "oracle": build_oracle, "synth": build_synth}[w]()
+         "emu": build_emu, "oracle": build_oracle, "synth": build_synth, "dropin": build_dropin_driver}[w]()

@@ -51,33 +51,72 @@ struct KernelArgs {
 
 __device__ __forceinline__ int model_params(int model) { return model == LZR_MODEL_SYMMETRIC ? 5 : 7; }
 
-/* Splits candidates into the 5- and 7-parameter queues. counters[0..1] = sizes,
- * counters[2..3] = queue heads (zeroed here for the fitting kernels). */
-__global__ void classify_kernel(const LzrCandidate *cands, int ncands, int default_model, int *order,
-                                unsigned *counters) {
+/* ---- work ordering ------------------------------------------------------------
+ * Candidates are queued by (parameter count, start intensity): the 5-parameter class
+ * first, and inside each class dimmest first.  LM iteration counts fall steeply with
+ * the spot's intensity, so the long fits start early and the end of the queue holds
+ * short ones; lanes then run out of work at nearly the same time.  Counting sort:
+ * histogram -> exclusive scan -> scatter.  counters layout: [0] class-5 count,
+ * [1] class-7 count, [2] [3] queue heads, [4 .. 4+kKeys) histogram, then cursors. */
+constexpr int kKeysPerClass = 128; /* 8 buckets per octave of intensity */
+constexpr int kKeys = 2 * kKeysPerClass;
+constexpr int kCounterWords = 4 + 2 * kKeys;
+
+__device__ __forceinline__ int sort_key(const LzrCandidate &c, int default_model) {
+    const int model = c.model < 0 ? default_model : c.model;
+    const int cls = model_params(model) == 5 ? 0 : 1;
+    const float I = (float)c.intensity;
+    int b = I > 1.0f ? (int)(__log2f(I) * 8.0f) : 0;
+    b = b < 0 ? 0 : (b >= kKeysPerClass ? kKeysPerClass - 1 : b);
+    return cls * kKeysPerClass + b;
+}
+
+__global__ void histogram_kernel(const LzrCandidate *cands, int ncands, int default_model, unsigned *counters) {
+    __shared__ unsigned h[kKeys];
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    int cls = -1;
+    if (i < ncands) atomicAdd(&h[sort_key(cands[i], default_model)], 1u);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
+        if (h[t]) atomicAdd(&counters[4 + t], h[t]);
+}
+
+/* one block of kKeys threads */
+__global__ void scan_kernel(unsigned *counters) {
+    __shared__ unsigned v[kKeys];
+    const int t = threadIdx.x;
+    const unsigned x = counters[4 + t];
+    v[t] = x;
+    __syncthreads();
+    for (int off = 1; off < kKeys; off <<= 1) {
+        const unsigned add = t >= off ? v[t - off] : 0u;
+        __syncthreads();
+        v[t] += add;
+        __syncthreads();
+    }
+    counters[4 + kKeys + t] = v[t] - x; /* exclusive prefix = bucket start */
+    if (t == kKeysPerClass - 1) counters[0] = v[t];
+    if (t == kKeys - 1) counters[1] = v[t] - v[kKeysPerClass - 1];
+}
+
+__global__ void scatter_kernel(const LzrCandidate *cands, int ncands, int default_model, unsigned *counters,
+                               int *order) {
+    __shared__ unsigned h[kKeys], base[kKeys];
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int key = 0;
+    unsigned local = 0;
     if (i < ncands) {
-        int model = cands[i].model;
-        if (model < 0) model = default_model;
-        cls = model_params(model) == 5 ? 0 : 1;
+        key = sort_key(cands[i], default_model);
+        local = atomicAdd(&h[key], 1u);
     }
-    /* one atomic per wave and class */
-    const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const unsigned long long m = __ballot(cls == c);
-        if (!m) continue;
-        const int leader = __ffsll((long long)m) - 1;
-        unsigned base = 0;
-        if (lane == leader) base = atomicAdd(&counters[c], (unsigned)__popcll(m));
-        base = __shfl(base, leader, 64);
-        if (cls == c) {
-            const unsigned slot = base + (unsigned)__popcll(m & below);
-            order[c == 0 ? slot : ncands - 1 - slot] = i;
-        }
-    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
+        if (h[t]) base[t] = atomicAdd(&counters[4 + kKeys + t], h[t]);
+    __syncthreads();
+    if (i < ncands) order[base[key] + local] = i;
 }
 
 /* Turns the raw end states into the reference records (post-checks, covariance,
@@ -88,8 +127,8 @@ __global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, LzrFit *o
     finish_any(out[i], cfg, cands[i]);
 }
 
-__global__ void reset_counters(unsigned *counters) {
-    if (threadIdx.x < 4) counters[threadIdx.x] = 0u;
+__global__ void reset_counters(unsigned *counters, int n) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) counters[t] = 0u;
 }
 
 __device__ __forceinline__ int wave_max(int v) {
@@ -98,19 +137,48 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+/* One fit in flight between two pixel passes: everything a lane needs to resume it. */
+template <int P>
+struct Parked {
+    LaneState<P> s;
+    int model;
+};
+
+/* One round of the fitting kernel.  Round 0 takes fresh candidates from the sorted
+ * queue; later rounds resume fits parked by the previous round.  A wave parks its
+ * remaining fits (and exits) once the queue is drained and at most `threshold` of its
+ * lanes are still busy, so the end of a round does not leave waves running with a
+ * few long fits; the next round packs the parked fits 64 to a wave.  The last round
+ * has threshold 0 and runs every fit to completion. */
+struct RoundArgs {
+    int resume;
+    const void *in; /* Parked<P>[] */
+    const unsigned *in_count;
+    unsigned *in_head;
+    void *out; /* Parked<P>[] */
+    unsigned *out_count;
+    int threshold;
+};
+
 /* W = minimum waves per SIMD the register allocation must allow (occupancy target). */
 template <int P, typename Acc, bool F64, int W>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) lm_persistent(KernelArgs args, int cls, int ncands_total) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
+lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     extern __shared__ uint16_t tile[];
     const int lane = threadIdx.x;
-    const unsigned count = args.count[cls];
-    const int *order = args.order + (cls == 0 ? 0 : ncands_total - (int)count);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned count = r.resume ? *r.in_count : args.count[cls];
+    unsigned *head = r.resume ? r.in_head : args.head + cls;
+    const int *order = args.order + (cls == 0 ? 0 : args.count[0]);
+    const Parked<P> *pin = static_cast<const Parked<P> *>(r.in);
+    Parked<P> *pout = static_cast<Parked<P> *>(r.out);
 
     LaneState<P> s;
     s.phase = kIdle;
     s.n = 0;
     int model = 0;
-    bool open = true; /* wave-uniform: the queue may still hold work */
+    bool fresh = false; /* this lane's window must be gathered */
+    bool open = true;   /* wave-uniform: the queue may still hold work */
 
     for (;;) {
         /* ---- refill idle lanes from the queue ---- */
@@ -120,19 +188,27 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) lm
             const int cnt = __popcll(want);
             const int leader = __ffsll((long long)want) - 1;
             unsigned base = 0;
-            if (lane == leader) base = atomicAdd(args.head + cls, (unsigned)cnt);
+            if (lane == leader) base = atomicAdd(head, (unsigned)cnt);
             base = __shfl(base, leader, 64);
             if (base + (unsigned)cnt >= count) open = false;
             if (base >= count) break;
             if (s.phase == kIdle) {
-                const unsigned rank = __popcll(want & ((1ull << lane) - 1ull));
-                const unsigned q = base + rank;
+                const unsigned q = base + (unsigned)__popcll(want & below);
                 if (q < count) {
-                    const int idx = order[q];
-                    const LzrCandidate c = args.cands[idx];
-                    model = c.model >= 0 ? c.model : args.cfg.model;
-                    if (!lane_start<P>(s, args.cfg, c, idx, args.nx, args.ny, args.nframes, model, args.out[idx]))
-                        s.phase = kIdle;
+                    if (r.resume) {
+                        const Parked<P> &pk = pin[q];
+                        s = pk.s;
+                        model = pk.model;
+                        fresh = true;
+                    } else {
+                        const int idx = order[q];
+                        const LzrCandidate c = args.cands[idx];
+                        model = c.model >= 0 ? c.model : args.cfg.model;
+                        if (lane_start<P>(s, args.cfg, c, idx, args.nx, args.ny, args.nframes, model, args.out[idx]))
+                            fresh = true;
+                        else
+                            s.phase = kIdle;
+                    }
                 } else {
                     s.phase = kFinished;
                 }
@@ -140,11 +216,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) lm
         }
         if (!open && s.phase == kIdle) s.phase = kFinished;
 
-        /* ---- gather the windows of freshly started lanes ---- */
-        unsigned long long fresh = __ballot(s.phase == kInit);
-        while (fresh) {
-            const int L = __ffsll((long long)fresh) - 1;
-            fresh &= fresh - 1ull;
+        /* ---- gather the windows of newly loaded lanes ---- */
+        unsigned long long gmask = __ballot(fresh);
+        while (gmask) {
+            const int L = __ffsll((long long)gmask) - 1;
+            gmask &= gmask - 1ull;
             const int fL = __shfl(s.frame, L, 64);
             const int xL = __shfl(s.xoff, L, 64);
             const int yL = __shfl(s.yoff, L, 64);
@@ -155,10 +231,26 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) lm
                 tile[k * 64 + L] = src[(size_t)j * args.nx + i];
             }
         }
+        fresh = false;
         __syncthreads();
 
         const bool active = s.phase == kInit || s.phase == kTrial;
-        if (!__any(active)) break;
+        const unsigned long long amask = __ballot(active);
+        if (!amask) break;
+
+        /* ---- park the stragglers of a drained queue ---- */
+        if (!open && r.threshold > 0 && __popcll(amask) <= r.threshold) {
+            const int leader = __ffsll((long long)amask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(r.out_count, (unsigned)__popcll(amask));
+            base = __shfl(base, leader, 64);
+            if (active) {
+                Parked<P> &pk = pout[base + (unsigned)__popcll(amask & below)];
+                pk.s = s;
+                pk.model = model;
+            }
+            break;
+        }
 
         /* ---- one pixel pass ---- */
         PassIn<P> q;
@@ -186,6 +278,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) lm
     }
 }
 
+__global__ void reset_park(unsigned *ctr) {
+    if (threadIdx.x < 2) ctr[threadIdx.x] = 0u;
+}
+
 }  // namespace lzr
 
 using namespace lzr;
@@ -198,7 +294,12 @@ struct LzrContext {
     int device = 0;
     LaneConfig cfg{};
     int waves_per_cu = 0;
-    int occupancy5 = 2; /* register budget of the 5-parameter kernel: 1 or 2 waves/SIMD */
+    int occupancy5 = 1; /* register budget of the 5-parameter kernel: 1 or 2 waves/SIMD */
+    int rounds = 1;          /* launches per class; fits are parked between them */
+    int park_threshold = 16; /* park a wave's fits when this few lanes are still busy */
+    void *d_park[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}; /* [class][buffer] */
+    size_t park_cap[2][2] = {{0, 0}, {0, 0}};
+    unsigned *d_park_ctr = nullptr; /* [class][buffer] x (count, head) */
     int num_cus = 0;
     std::string err;
     std::mutex mu;
@@ -229,6 +330,9 @@ static void ensure(T *&ptr, size_t &cap, size_t need) {
 }
 
 template <int P, typename Acc, bool F64, int W>
+static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream);
+
+template <int P, typename Acc, bool F64, int W>
 static int grid_for(LzrContext *ctx, size_t lds) {
     int per_cu = 0;
     LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, F64, W>, 64, lds));
@@ -237,16 +341,49 @@ static int grid_for(LzrContext *ctx, size_t lds) {
     return per_cu * ctx->num_cus;
 }
 
+template <int P, typename Acc, bool F64, int W>
+static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
+    const int grid = grid_for<P, Acc, F64, W>(ctx, lds);
+    const size_t cap = (size_t)grid * 64;
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->park_cap[cls][b] < cap) {
+            if (ctx->d_park[cls][b]) LZR_CHECK(hipFree(ctx->d_park[cls][b]));
+            ctx->d_park[cls][b] = nullptr;
+            ctx->park_cap[cls][b] = 0;
+            LZR_CHECK(hipMalloc(&ctx->d_park[cls][b], cap * sizeof(Parked<P>)));
+            ctx->park_cap[cls][b] = cap;
+        }
+    }
+    unsigned *ctr = ctx->d_park_ctr + 4 * cls; /* buffer b: count ctr[2b], head ctr[2b+1] */
+    for (int round = 0; round < ctx->rounds; ++round) {
+        const int ob = round & 1, ib = ob ^ 1;
+        hipLaunchKernelGGL(reset_park, dim3(1), dim3(64), 0, stream, ctr + 2 * ob);
+        RoundArgs r;
+        r.resume = round > 0;
+        r.in = ctx->d_park[cls][ib];
+        r.in_count = ctr + 2 * ib;
+        r.in_head = ctr + 2 * ib + 1;
+        r.out = ctx->d_park[cls][ob];
+        r.out_count = ctr + 2 * ob;
+        r.threshold = round == ctx->rounds - 1 ? 0 : ctx->park_threshold;
+        hipLaunchKernelGGL((lm_persistent<P, Acc, F64, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
+    }
+}
+
 typedef double AccT;           /* normal-equation accumulation type      */
 constexpr bool kF64Residual = true; /* current-point residual in FP64 */
 
 static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int nframes,
                    const LzrCandidate *cands, int ncands, LzrFit *out, hipStream_t stream) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
-    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, stream, ctx->d_counters);
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
     const int tpb = 256;
-    hipLaunchKernelGGL(classify_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands,
-                       ctx->cfg.model, ctx->d_order, ctx->d_counters);
+    const int nb = (ncands + tpb - 1) / tpb;
+    hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(tpb), 0, stream, cands, ncands, ctx->cfg.model,
+                       ctx->d_counters);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kKeys), 0, stream, ctx->d_counters);
+    hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(tpb), 0, stream, cands, ncands, ctx->cfg.model,
+                       ctx->d_counters, ctx->d_order);
     KernelArgs a;
     a.frames = frames;
     a.nx = nx;
@@ -260,15 +397,11 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.cfg = ctx->cfg;
     const int nmax = 2 * ctx->cfg.max_half_width + 1;
     const size_t lds = (size_t)nmax * nmax * 64 * sizeof(uint16_t);
-    if (ctx->occupancy5 == 1) {
-        const int g5 = grid_for<5, AccT, kF64Residual, 1>(ctx, lds);
-        hipLaunchKernelGGL((lm_persistent<5, AccT, kF64Residual, 1>), dim3(g5), dim3(64), lds, stream, a, 0, ncands);
-    } else {
-        const int g5 = grid_for<5, AccT, kF64Residual, 2>(ctx, lds);
-        hipLaunchKernelGGL((lm_persistent<5, AccT, kF64Residual, 2>), dim3(g5), dim3(64), lds, stream, a, 0, ncands);
-    }
-    const int g7 = grid_for<7, AccT, kF64Residual, 1>(ctx, lds);
-    hipLaunchKernelGGL((lm_persistent<7, AccT, kF64Residual, 1>), dim3(g7), dim3(64), lds, stream, a, 1, ncands);
+    if (ctx->occupancy5 == 1)
+        run_class<5, AccT, kF64Residual, 1>(ctx, a, 0, lds, stream);
+    else
+        run_class<5, AccT, kF64Residual, 2>(ctx, a, 0, lds, stream);
+    run_class<7, AccT, kF64Residual, 1>(ctx, a, 1, lds, stream);
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, out,
                        ctx->cfg);
     LZR_CHECK(hipGetLastError());
@@ -310,7 +443,8 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         hipDeviceProp_t prop;
         LZR_CHECK(hipGetDeviceProperties(&prop, ctx->device));
         ctx->num_cus = prop.multiProcessorCount;
-        LZR_CHECK(hipMalloc((void **)&ctx->d_counters, 4 * sizeof(unsigned)));
+        LZR_CHECK(hipMalloc((void **)&ctx->d_counters, kCounterWords * sizeof(unsigned)));
+        LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 8 * sizeof(unsigned)));
     } catch (const std::string &e) {
         g_create_error = "lzr_create: " + e;
         delete ctx;
@@ -336,6 +470,9 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_out);
     (void)hipFree(ctx->d_order);
     (void)hipFree(ctx->d_counters);
+    (void)hipFree(ctx->d_park_ctr);
+    for (int c = 0; c < 2; ++c)
+        for (int b = 0; b < 2; ++b) (void)hipFree(ctx->d_park[c][b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -378,6 +515,14 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
     case LZR_OPT_OCCUPANCY:
         if (!(value == 1 || value == 2)) return fail(ctx, "occupancy must be 1 or 2");
         ctx->occupancy5 = (int)value;
+        return 0;
+    case LZR_OPT_ROUNDS:
+        if (!(value >= 1 && value <= 16)) return fail(ctx, "rounds must be in 1..16");
+        ctx->rounds = (int)value;
+        return 0;
+    case LZR_OPT_PARK_THRESHOLD:
+        if (!(value >= 0 && value <= 64)) return fail(ctx, "park threshold must be in 0..64");
+        ctx->park_threshold = (int)value;
         return 0;
     case LZR_OPT_MAX_HALF_WIDTH:
         if (!(value >= 1 && value <= LZR_MAX_HALF_WIDTH)) return fail(ctx, "max half-width out of range");

@@ -1,0 +1,247 @@
+/*
+ * LocalizationHIP.h -- drop-in replacements for the reference's LM fitters, running on
+ * the MI355X through the C ABI of include/localizer_hip.h.
+ *
+ *   FitPositions_SymmetricGaussianHIP(initialPSFWidth, sigma)      <- FitPositions_SymmetricGaussian
+ *   FitPositions_EllipsoidalGaussianHIP(initialPSFWidth, sigma)    <- FitPositions_EllipsoidalGaussian
+ *   FitPositions_EllipsoidalGaussian_SymmetricPSFHIP(w, sigma)     <- FitPositions_EllipsoidalGaussian_SymmetricPSF
+ *   (source/Localization.h:55-121)
+ *
+ * Same constructor arguments, the same virtual fit_positions(ImagePtr, list<Particle>)
+ * returning the same concrete container type, the same side effect (rejected particles
+ * are erased from the caller's list; for the symmetric-PSF variant only the ellipsoidal
+ * fit's rejections are, source/Localization.cpp:606-631), the same error convention
+ * (std::runtime_error, source/Errors.h).  Instances are safe to call from several
+ * threads at once (PALMAnalysisController shares one fitter between its workers,
+ * source/PALMAnalysis.cpp:234): calls on one instance are serialised on the device.
+ *
+ * Compile inside the reference code base with LZR_WITH_LOCALIZER_HEADERS defined (the
+ * reference's Localization.h supplies FitPositions, Particle, Image and the
+ * containers); without it, the mirror types of StorageHIP.h are used.
+ *
+ * fit_positions_batch() is the throughput form: many frames, one device launch.
+ */
+#ifndef LZR_LOCALIZATION_HIP_H
+#define LZR_LOCALIZATION_HIP_H
+
+#include <cmath>
+#include <cstdint>
+#include <list>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef LZR_WITH_LOCALIZER_HEADERS
+#include "Localization.h"
+#else
+#include "StorageHIP.h"
+#endif
+
+#include "../../include/localizer_hip.h"
+
+namespace lzr_dropin {
+
+inline std::string last_error(const LzrContext *ctx) {
+    char buf[512];
+    lzr_last_error(ctx, buf, (int)sizeof buf);
+    return std::string(buf);
+}
+
+/* Copies an FP64 frame (column-major, rows = x) into a uint16 buffer; the GPU path
+ * fits camera counts, so every pixel must be an integer in [0, 65535]. */
+inline void append_frame(const Image &img, std::vector<uint16_t> &dst) {
+    const long nx = img.rows(), ny = img.cols();
+    const size_t base = dst.size();
+    dst.resize(base + (size_t)nx * ny);
+    for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+            const double v = img(x, y);
+            if (!(v >= 0.0 && v <= 65535.0) || v != std::floor(v))
+                throw std::runtime_error("LocalizationHIP: frame pixels must be integer counts in [0, 65535]");
+            dst[base + (size_t)y * nx + x] = (uint16_t)v;
+        }
+}
+
+class FitterCore {
+public:
+    FitterCore(int model, double initialPSFWidth, double sigma) : model_(model) {
+        if (lzr_create(&ctx_, model, initialPSFWidth, sigma, 0) != 0)
+            throw std::runtime_error("LocalizationHIP: " + last_error(nullptr));
+    }
+    ~FitterCore() {
+        if (ctx_) lzr_destroy(ctx_);
+    }
+    FitterCore(const FitterCore &) = delete;
+    FitterCore &operator=(const FitterCore &) = delete;
+
+    LzrContext *context() { return ctx_; }
+
+    /* Fits several frames (same size) with their particle lists in one launch. */
+    template <class Container>
+    std::vector<std::shared_ptr<LocalizedPositionsContainer> > fit_many(
+        const std::vector<ImagePtr> &images, const std::vector<std::shared_ptr<std::list<Particle> > > &lists) {
+        if (images.size() != lists.size()) throw std::runtime_error("LocalizationHIP: frames and lists differ in count");
+        std::vector<std::shared_ptr<LocalizedPositionsContainer> > out;
+        std::vector<uint16_t> stack;
+        std::vector<LzrCandidate> cands;
+        long nx = -1, ny = -1;
+        for (size_t f = 0; f < images.size(); ++f) {
+            out.push_back(std::shared_ptr<LocalizedPositionsContainer>(new Container()));
+            if (lists[f]->empty()) continue;
+            if (nx < 0) {
+                nx = images[f]->rows();
+                ny = images[f]->cols();
+            } else if (images[f]->rows() != nx || images[f]->cols() != ny) {
+                throw std::runtime_error("LocalizationHIP: frames in one batch must share their size");
+            }
+            const int fi = (int)(stack.size() / ((size_t)nx * ny));
+            append_frame(*images[f], stack);
+            for (const Particle &p : *lists[f]) {
+                LzrCandidate c;
+                c.x = p.x;
+                c.y = p.y;
+                c.intensity = p.intensity;
+                c.background = p.background;
+                c.frame = fi;
+                c.half_width = 0;
+                c.model = -1;
+                cands.push_back(c);
+            }
+        }
+        if (cands.empty()) return out;
+        std::vector<LzrFit> res(cands.size());
+        const int nframes = (int)(stack.size() / ((size_t)nx * ny));
+        if (lzr_fit_batch(ctx_, stack.data(), (int)nx, (int)ny, nframes, cands.data(), (int)cands.size(),
+                          res.data()) < 0)
+            throw std::runtime_error("LocalizationHIP: " + last_error(ctx_));
+        size_t k = 0;
+        for (size_t f = 0; f < images.size(); ++f) {
+            auto it = lists[f]->begin();
+            while (it != lists[f]->end()) {
+                const LzrFit &r = res[k++];
+                if (r.status == LZR_ACCEPTED) {
+                    add_record(*out[f], r);
+                    ++it;
+                } else if (erases(r.status)) {
+                    it = lists[f]->erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        return out;
+    }
+
+    template <class Container>
+    std::shared_ptr<LocalizedPositionsContainer> fit_one(const ImagePtr image,
+                                                         std::shared_ptr<std::list<Particle> > particles) {
+        std::vector<ImagePtr> im(1, image);
+        std::vector<std::shared_ptr<std::list<Particle> > > li(1, particles);
+        return fit_many<Container>(im, li)[0];
+    }
+
+private:
+    /* The symmetric-PSF filter's rejections stay in the caller's list (the reference
+     * erases only the ellipsoidal fit's rejections). */
+    bool erases(int status) const {
+        if (model_ == LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF)
+            return status != LZR_REJECT_ANISOTROPIC && status != LZR_REJECT_PSF_WIDTH;
+        return true;
+    }
+
+    void add_record(LocalizedPositionsContainer &c, const LzrFit &r) const {
+        if (model_ == LZR_MODEL_SYMMETRIC) {
+            std::shared_ptr<LocalizedPosition_2DGauss> p(new LocalizedPosition_2DGauss());
+            p->integral = r.integral;
+            p->width = r.params[1];
+            p->xPosition = r.params[2];
+            p->yPosition = r.params[3];
+            p->background = r.params[4];
+            p->integralDeviation = r.integral_deviation;
+            p->widthDeviation = r.deviations[1];
+            p->xPositionDeviation = r.deviations[2];
+            p->yPositionDeviation = r.deviations[3];
+            p->backgroundDeviation = r.deviations[4];
+            c.addPosition(p);
+        } else {
+            std::shared_ptr<LocalizedPosition_Ellipsoidal2DGauss> p(new LocalizedPosition_Ellipsoidal2DGauss());
+            p->integral = r.integral;
+            p->stdDev1 = r.sd1;
+            p->stdDev2 = r.sd2;
+            p->xPosition = r.params[3];
+            p->yPosition = r.params[4];
+            p->theta = r.theta;
+            p->background = r.params[6];
+            p->integralDeviation = r.integral_deviation;
+            p->stdDev1Deviation = r.sd1_deviation;
+            p->stdDev2Deviation = r.sd2_deviation;
+            p->xPositionDeviation = r.deviations[3];
+            p->yPositionDeviation = r.deviations[4];
+            p->thetaDeviation = 0.0;
+            p->backgroundDeviation = r.deviations[6];
+            c.addPosition(p);
+        }
+    }
+
+    LzrContext *ctx_ = nullptr;
+    int model_;
+};
+
+}  // namespace lzr_dropin
+
+class FitPositions_SymmetricGaussianHIP : public FitPositions {
+public:
+    FitPositions_SymmetricGaussianHIP(double initialPSFWidth_rhs, double sigma_rhs)
+        : core_(LZR_MODEL_SYMMETRIC, initialPSFWidth_rhs, sigma_rhs) {}
+    std::shared_ptr<LocalizedPositionsContainer> fit_positions(const ImagePtr image,
+                                                               std::shared_ptr<std::list<Particle> > particles) {
+        return core_.fit_one<LocalizedPositionsContainer_2DGauss>(image, particles);
+    }
+    std::vector<std::shared_ptr<LocalizedPositionsContainer> > fit_positions_batch(
+        const std::vector<ImagePtr> &images, const std::vector<std::shared_ptr<std::list<Particle> > > &lists) {
+        return core_.fit_many<LocalizedPositionsContainer_2DGauss>(images, lists);
+    }
+    LzrContext *context() { return core_.context(); }
+
+private:
+    lzr_dropin::FitterCore core_;
+};
+
+class FitPositions_EllipsoidalGaussianHIP : public FitPositions {
+public:
+    FitPositions_EllipsoidalGaussianHIP(double initialPSFWidth_rhs, double sigma_rhs)
+        : core_(LZR_MODEL_ELLIPSOIDAL, initialPSFWidth_rhs, sigma_rhs) {}
+    std::shared_ptr<LocalizedPositionsContainer> fit_positions(const ImagePtr image,
+                                                               std::shared_ptr<std::list<Particle> > particles) {
+        return core_.fit_one<LocalizedPositionsContainer_Ellipsoidal2DGaussian>(image, particles);
+    }
+    std::vector<std::shared_ptr<LocalizedPositionsContainer> > fit_positions_batch(
+        const std::vector<ImagePtr> &images, const std::vector<std::shared_ptr<std::list<Particle> > > &lists) {
+        return core_.fit_many<LocalizedPositionsContainer_Ellipsoidal2DGaussian>(images, lists);
+    }
+    LzrContext *context() { return core_.context(); }
+
+private:
+    lzr_dropin::FitterCore core_;
+};
+
+class FitPositions_EllipsoidalGaussian_SymmetricPSFHIP : public FitPositions {
+public:
+    FitPositions_EllipsoidalGaussian_SymmetricPSFHIP(double initialPSFWidth_rhs, double sigma_rhs)
+        : core_(LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF, initialPSFWidth_rhs, sigma_rhs) {}
+    std::shared_ptr<LocalizedPositionsContainer> fit_positions(const ImagePtr image,
+                                                               std::shared_ptr<std::list<Particle> > particles) {
+        return core_.fit_one<LocalizedPositionsContainer_Ellipsoidal2DGaussian>(image, particles);
+    }
+    std::vector<std::shared_ptr<LocalizedPositionsContainer> > fit_positions_batch(
+        const std::vector<ImagePtr> &images, const std::vector<std::shared_ptr<std::list<Particle> > > &lists) {
+        return core_.fit_many<LocalizedPositionsContainer_Ellipsoidal2DGaussian>(images, lists);
+    }
+    LzrContext *context() { return core_.context(); }
+
+private:
+    lzr_dropin::FitterCore core_;
+};
+
+#endif

@@ -1,0 +1,91 @@
+"""The C++ drop-in classes (localizer_amd/dropin/LocalizationHIP.h): they compile
+against the reference's own headers (same FitPositions / Particle / container
+interface), and on the GPU they return the oracle's records and erase exactly the
+particles the reference erases (source/Localization.cpp:122-180, :606-631)."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from localizer_amd import abi, synth
+from oracle import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "dropin_main")
+REF = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "source")), reason="reference sources not present")
+def test_dropin_compiles_against_reference_headers():
+    src = ('#include "%s/localizer_amd/dropin/LocalizationHIP.h"\n'
+           "int main() { FitPositions *f = new FitPositions_SymmetricGaussianHIP(1.3, 1.0); delete f;\n"
+           "  f = new FitPositions_EllipsoidalGaussianHIP(1.3, 1.0); delete f;\n"
+           "  f = new FitPositions_EllipsoidalGaussian_SymmetricPSFHIP(1.3, 1.0); delete f; return 0; }\n" % ROOT)
+    cmd = ["g++", "-std=c++11", "-fsyntax-only", "-DLZR_WITH_LOCALIZER_HEADERS", "-include", "memory", "-include",
+           "cstring", "-DHAVE_INLINE", f"-I{REF}/include", f"-I{REF}/include/eigen3", f"-I{REF}/source",
+           f"-I{REF}/libsource/LNBTIFF", "-x", "c++", "-"]
+    subprocess.run(cmd, input=src.encode(), check=True)
+
+
+def test_dropin_driver_is_built():
+    assert os.path.exists(DRIVER), "build with `python -m localizer_amd.build`"
+
+
+def _write_input(path, frames, cands):
+    nf, ny, nx = frames.shape
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iii", nx, ny, nf))
+        f.write(frames.astype("<u2").tobytes())
+        for fi in range(nf):
+            c = cands[cands["frame"] == fi]
+            f.write(struct.pack("<i", c.size))
+            f.write(np.stack([c["x"], c["y"], c["intensity"], c["background"]], 1).astype("<f8").tobytes())
+
+
+def _read_output(path, nf):
+    raw = open(path, "rb").read()
+    pos, out = 0, []
+    for _ in range(nf):
+        r, c = struct.unpack_from("<ii", raw, pos)
+        pos += 8
+        m = np.frombuffer(raw, "<f8", r * c, pos).reshape(c, r).T.copy()
+        pos += 8 * r * c
+        (s,) = struct.unpack_from("<i", raw, pos)
+        pos += 4
+        surv = np.frombuffer(raw, "<f8", 2 * s, pos).reshape(s, 2).copy()
+        pos += 16 * s
+        out.append((m, surv))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,mode", [(0, "single"), (0, "threads"), (0, "batch"), (1, "batch"), (2, "single")])
+def test_dropin_matches_oracle(tmp_path, model, mode):
+    print(f"case: drop-in model={model} mode={mode}", flush=True)
+    w = synth.PLAIN_GPU_256.scaled(12)
+    frames, _, cands = synth.generate(w)
+    cands["half_width"] = 0  # the drop-in always fits the reference window ceil(4 w)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, frames, cands)
+    w_psf = 1.35
+    subprocess.run([DRIVER, str(model), str(w_psf), "1.0", mode, fin, fout], check=True, timeout=120)
+    got = _read_output(fout, frames.shape[0])
+    # the reference window: cutoff_radius = ceil(4 w) (no explicit half-width)
+    ref = po.fit_batch(frames, cands, model, w_psf)
+    xcol = 3 if model == 0 else 4
+    for fi in range(frames.shape[0]):
+        sel = cands["frame"] == fi
+        r = ref[sel]
+        acc = r["status"] == 0
+        m, surv = got[fi]
+        assert m.shape[0] == acc.sum()
+        if m.shape[0] == acc.sum():
+            xref = r["params"][acc][:, 2 if model == 0 else 3]
+            assert np.mean(np.abs(m[:, xcol] - xref) <= 1e-3) >= 0.99
+            assert np.all(m[:, 0] == fi)
+        erased = (r["status"] != 0)
+        if model == abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF:
+            erased &= ~np.isin(r["status"], [abi.REJECT_ANISOTROPIC, abi.REJECT_PSF_WIDTH])
+        assert surv.shape[0] == int((~erased).sum())
