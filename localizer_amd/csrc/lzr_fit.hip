@@ -164,7 +164,11 @@ struct RoundArgs {
 template <int P, typename Acc, bool F64, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_persistent(KernelArgs args, int cls, RoundArgs r) {
-    extern __shared__ uint16_t tile[];
+    extern __shared__ double lds_words[];
+    /* LDS: per-column FP64 factors [nmax][64], then the pixel tile [nmax*nmax][64] */
+    const int nmax = 2 * args.cfg.max_half_width + 1;
+    double *xcol = lds_words;
+    uint16_t *tile = reinterpret_cast<uint16_t *>(lds_words + nmax * 64);
     const int lane = threadIdx.x;
     const unsigned long long below = (1ull << lane) - 1ull;
     const unsigned count = r.resume ? *r.in_count : args.count[cls];
@@ -257,17 +261,28 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         PassAcc<P, Acc> acc;
         acc.zero();
         const int n = active ? s.n : 0;
-        if (active) lane_pass_in<P>(s, args.cfg.sigma, q);
-        const int nw = wave_max(n);
-        for (int j = 0; j < nw; ++j) {
-            const float fj = (float)j;
-            for (int i = 0; i < nw; ++i) {
-                if (i < n && j < n) {
-                    const float D = (float)tile[(i + j * n) * 64 + lane];
-                    pass_pixel<F64>(q, (float)i, fj, D, acc);
-                }
-            }
+        if (active) {
+            lane_pass_in<P>(s, args.cfg.sigma, q);
+        } else {
+            /* idle lanes run the loop on benign constants; their sums are discarded */
+            double z[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) z[k] = 1.0;
+            make_pass_in(z, z, 1.0, q);
         }
+        const int nw = wave_max(n);
+        const bool mixed = __any(active && n != nw);
+        struct {
+            const uint16_t *t;
+            int n;
+            __device__ float operator()(int i, int j) const { return (float)t[(i + j * n) * 64]; }
+        } pix{tile + lane, active ? n : nw};
+        struct {
+            double *c;
+            __device__ void set(int i, double v) { c[i * 64] = v; }
+            __device__ double get(int i) const { return c[i * 64]; }
+        } xb{xcol + lane};
+        window_pass(q, n, nw, mixed, pix, xb, acc);
 
         /* ---- LM logic ---- */
         if (active) {
@@ -396,7 +411,7 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.out = out;
     a.cfg = ctx->cfg;
     const int nmax = 2 * ctx->cfg.max_half_width + 1;
-    const size_t lds = (size_t)nmax * nmax * 64 * sizeof(uint16_t);
+    const size_t lds = (size_t)nmax * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * sizeof(uint16_t);
     if (ctx->occupancy5 == 1)
         run_class<5, AccT, kF64Residual, 1>(ctx, a, 0, lds, stream);
     else

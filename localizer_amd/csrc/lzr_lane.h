@@ -201,21 +201,19 @@ LZR_HD LZR_INLINE void residual0(double qd_arg0, float arg0_f, double A0d, doubl
     }
 }
 
-template <bool F64, typename Acc>
-LZR_HD LZR_INLINE void pass_pixel(const PassIn<5> &q, float fi, float fj, float D, PassAcc<5, Acc> &a) {
-    const float ux0 = fi - q.x0, uy0 = fj - q.y0;
-    const float ux1 = ux0 - q.dxc, uy1 = uy0 - q.dyc;
+/* One pixel of the symmetric model.  ux0/uy0: pixel minus current centre (exact in
+ * FP32), e0d: exp(-c0 (ux0^2 + uy0^2)) in FP64 (from separable factors). */
+template <typename Acc>
+LZR_HD LZR_INLINE void pixel_sym(const PassIn<5> &q, float ux0, float uy0, float uy1, float ddy, float D, double e0d,
+                                 PassAcc<5, Acc> &a) {
+    const float ux1 = ux0 - q.dxc;
     const float d0 = ux0 * ux0 + uy0 * uy0;
     const float d1 = ux1 * ux1 + uy1 * uy1;
     const float arg0 = d0 * q.c0;
-    float e0, f0;
-    {
-        const double dx = ux0, dy = uy0;
-        residual0<F64>(F64 ? (dx * dx + dy * dy) * q.c0d : 0.0, arg0, q.A0d, q.b0d, q.inv_sigma_d, q.A0, q.b0,
-                       q.inv_sigma, D, e0, f0);
-    }
+    const float e0 = (float)e0d;
+    const float f0 = (float)((q.A0d * e0d + (q.b0d - (double)D)) * q.inv_sigma_d);
     /* exponent change, free of cancellation: (d1-d0) c1 + d0 (c1-c0) */
-    const float dd = -(q.dxc * (ux1 + ux0) + q.dyc * (uy1 + uy0));
+    const float dd = ddy - q.dxc * (ux1 + ux0);
     const float darg = dd * (q.c0 + q.dc) + d0 * q.dc;
     float e1, e0m;
     exp_change(e0, arg0 + darg, darg, e1, e0m);
@@ -241,8 +239,8 @@ LZR_HD LZR_INLINE void pass_pixel(const PassIn<5> &q, float fi, float fj, float 
     a.sf += F;
 }
 
-template <bool F64, typename Acc>
-LZR_HD LZR_INLINE void pass_pixel(const PassIn<7> &q, float fi, float fj, float D, PassAcc<7, Acc> &a) {
+template <typename Acc>
+LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D, PassAcc<7, Acc> &a) {
     const float ux0 = fi - q.x0, uy0 = fj - q.y0;
     const float ux1 = ux0 - q.dxc, uy1 = uy0 - q.dyc;
     const float xx0 = ux0 * ux0, yy0 = uy0 * uy0, xy0 = ux0 * uy0;
@@ -251,8 +249,8 @@ LZR_HD LZR_INLINE void pass_pixel(const PassIn<7> &q, float fi, float fj, float 
     float e0, f0;
     {
         const double dx = ux0, dy = uy0;
-        residual0<F64>(F64 ? q.qa0d * dx * dx + 2.0 * q.qc0d * dx * dy + q.qb0d * dy * dy : 0.0, arg0, q.A0d,
-                       q.b0d, q.inv_sigma_d, q.A0, q.b0, q.inv_sigma, D, e0, f0);
+        residual0<true>(q.qa0d * dx * dx + 2.0 * q.qc0d * dx * dy + q.qb0d * dy * dy, arg0, q.A0d, q.b0d,
+                        q.inv_sigma_d, q.A0, q.b0, q.inv_sigma, D, e0, f0);
     }
     const float dxx = -q.dxc * (ux1 + ux0);
     const float dyy = -q.dyc * (uy1 + uy0);
@@ -281,6 +279,42 @@ LZR_HD LZR_INLINE void pass_pixel(const PassIn<7> &q, float fi, float fj, float 
         a.gq[i] += J[i] * F;
     }
     a.sf += F;
+}
+
+/* One pass over a lane's window: residuals at the current point, residual change and
+ * Jacobian at the trial point, normal equations.  nloop >= n is the loop bound shared
+ * by the wave; `guard` masks pixels beyond the lane's own n (mixed window sizes).
+ * pix(i, j) returns pixel (i, j); xb holds one FP64 value per column.
+ * Symmetric model: exp(-c0 (ux^2 + uy^2)) = Ex[i] Ey[j], 2n FP64 exponentials per pass. */
+template <typename Acc, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void window_pass(const PassIn<5> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
+                                   PassAcc<5, Acc> &a) {
+    for (int i = 0; i < nloop; ++i) {
+        const double ux = (double)((float)i - q.x0);
+        xb.set(i, dexp(-q.c0d * ux * ux));
+    }
+    for (int j = 0; j < nloop; ++j) {
+        const float uy0 = (float)j - q.y0;
+        const float uy1 = uy0 - q.dyc;
+        const float ddy = -q.dyc * (uy1 + uy0);
+        const double Ey = dexp(-q.c0d * (double)uy0 * (double)uy0);
+        if (guard) {
+            for (int i = 0; i < nloop; ++i)
+                if (i < n && j < n) pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, pix(i, j), xb.get(i) * Ey, a);
+        } else {
+            for (int i = 0; i < nloop; ++i) pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, pix(i, j), xb.get(i) * Ey, a);
+        }
+    }
+}
+
+template <typename Acc, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void window_pass(const PassIn<7> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &,
+                                   PassAcc<7, Acc> &a) {
+    for (int j = 0; j < nloop; ++j) {
+        const float fj = (float)j;
+        for (int i = 0; i < nloop; ++i)
+            if (!guard || (i < n && j < n)) pixel_ell(q, (float)i, fj, pix(i, j), a);
+    }
 }
 
 /* ---- FP64 helpers ---------------------------------------------------------- */

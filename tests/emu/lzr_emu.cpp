@@ -23,8 +23,17 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
         lane_pass_in<P>(s, cfg.sigma, q);
         PassAcc<P, Acc> a;
         a.zero();
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i) pass_pixel<F64>(q, (float)i, (float)j, D[i + n * j], a);
+        struct {
+            const float *D;
+            int n;
+            float operator()(int i, int j) const { return D[i + n * j]; }
+        } pix{D, n};
+        struct {
+            double v[2 * LZR_MAX_HALF_WIDTH + 1];
+            void set(int i, double x) { v[i] = x; }
+            double get(int i) const { return v[i]; }
+        } xb;
+        window_pass(q, n, n, false, pix, xb, a);
         if (!lane_after_pass<P, Acc>(s, cfg, a, model, *reinterpret_cast<LzrRawFit *>(&out))) {
             finish_any(out, cfg, c);
             return;

@@ -62,7 +62,12 @@ def _read_output(path, nf):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("model,mode", [(0, "single"), (0, "threads"), (0, "batch"), (1, "batch"), (2, "single")])
-def test_dropin_matches_oracle(tmp_path, model, mode):
+def test_dropin_matches_c_abi_and_oracle(tmp_path, model, mode):
+    """The C++ layer returns exactly the C ABI's accepted fits (in particle order) and
+    erases exactly the particles the reference erases for those statuses; the C ABI
+    itself agrees with the oracle to the parity bar."""
+    from localizer_amd import native
+
     print(f"case: drop-in model={model} mode={mode}", flush=True)
     w = synth.PLAIN_GPU_256.scaled(12)
     frames, _, cands = synth.generate(w)
@@ -72,20 +77,26 @@ def test_dropin_matches_oracle(tmp_path, model, mode):
     w_psf = 1.35
     subprocess.run([DRIVER, str(model), str(w_psf), "1.0", mode, fin, fout], check=True, timeout=120)
     got = _read_output(fout, frames.shape[0])
-    # the reference window: cutoff_radius = ceil(4 w) (no explicit half-width)
-    ref = po.fit_batch(frames, cands, model, w_psf)
+    with native.Context(model, w_psf) as ctx:
+        capi = ctx.fit(frames, cands)
+    xi = 2 if model == 0 else 3
     xcol = 3 if model == 0 else 4
     for fi in range(frames.shape[0]):
-        sel = cands["frame"] == fi
-        r = ref[sel]
+        r = capi[cands["frame"] == fi]
         acc = r["status"] == 0
         m, surv = got[fi]
         assert m.shape[0] == acc.sum()
-        if m.shape[0] == acc.sum():
-            xref = r["params"][acc][:, 2 if model == 0 else 3]
-            assert np.mean(np.abs(m[:, xcol] - xref) <= 1e-3) >= 0.99
-            assert np.all(m[:, 0] == fi)
-        erased = (r["status"] != 0)
+        np.testing.assert_array_equal(m[:, xcol], r["params"][acc][:, xi])
+        assert np.all(m[:, 0] == fi)
+        erased = r["status"] != 0
         if model == abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF:
             erased &= ~np.isin(r["status"], [abi.REJECT_ANISOTROPIC, abi.REJECT_PSF_WIDTH])
-        assert surv.shape[0] == int((~erased).sum())
+        c = cands[cands["frame"] == fi]
+        np.testing.assert_array_equal(surv[:, 0], c["x"][~erased])
+    ref = po.fit_batch(frames, cands, model, w_psf)
+    agree = np.mean(ref["status"] == capi["status"])
+    both = (ref["status"] == 0) & (capi["status"] == 0)
+    dxy = np.abs(ref["params"][both][:, xi:xi + 2] - capi["params"][both][:, xi:xi + 2]).max(axis=1)
+    print(f"drop-in model={model}: status agreement {agree:.4f}, xy within 1e-3: {np.mean(dxy <= 1e-3):.4f}")
+    assert agree >= 0.99
+    assert np.mean(dxy <= 1e-3) >= 0.98
