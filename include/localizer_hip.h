@@ -18,6 +18,7 @@
  *   LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF
  *                          FitPositions_EllipsoidalGaussian_SymmetricPSF
  *                                                           (source/Localization.cpp:606-631)
+ *   LZR_MODEL_FIXED_WIDTH  FitPositions_FixedWidthGaussian  (source/Localization.cpp:222-388)
  * The C++ classes in localizer_amd/dropin/LocalizationHIP.h wrap this ABI with the
  * reference constructors and container types.
  */
@@ -40,6 +41,7 @@
 #define LZR_MODEL_SYMMETRIC 0
 #define LZR_MODEL_ELLIPSOIDAL 1
 #define LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF 2
+#define LZR_MODEL_FIXED_WIDTH 3 /* FitPositions_FixedWidthGaussian (source/Localization.cpp:222-388) */
 
 /* Maximum window half-width: n = 2h+1 <= 31. */
 #define LZR_MAX_HALF_WIDTH 15
@@ -56,6 +58,7 @@
 #define LZR_REJECT_ANISOTROPIC 6 /* symmetric-PSF filter: sd ratio (Localization.cpp:617-619)    */
 #define LZR_REJECT_PSF_WIDTH 7   /* symmetric-PSF filter: widths   (Localization.cpp:622-625)    */
 #define LZR_REJECT_INVALID 8   /* candidate refers to a missing frame / bad window size         */
+#define LZR_REJECT_DISTANCE 9  /* fixed width: position moved > 2w from the start (Localization.cpp:349) */
 
 /* GSL-equivalent status that ended the LM loop (gsl_errno.h:50-84). */
 #define LZR_LM_SUCCESS 0       /* test_delta satisfied                                         */
@@ -120,6 +123,7 @@ typedef struct LzrContext LzrContext;
  * Returns 0, or -1 (no context is created; lzr_last_error(NULL, ...) explains). */
 LZR_EXPORT int lzr_create(LzrContext **ctx, int model, double initial_psf_width, double sigma,
                           int default_half_width);
+/* (also FitPositions_FixedWidthGaussian(initialPSFWidth, sigma), source/Localization.h:83-95) */
 
 /* Releases a context and its device memory.  Returns 0 or -1.
  * Replaces the fitter's destructor and LocalizerFree (source/LocalizerDLL.cpp:131-133). */

@@ -10,6 +10,7 @@ import numpy as np
 MODEL_SYMMETRIC = 0
 MODEL_ELLIPSOIDAL = 1
 MODEL_ELLIPSOIDAL_SYMMETRIC_PSF = 2
+MODEL_FIXED_WIDTH = 3
 
 MAX_HALF_WIDTH = 15
 MAX_PARAMS = 7
@@ -23,6 +24,7 @@ REJECT_EIGEN = 5
 REJECT_ANISOTROPIC = 6
 REJECT_PSF_WIDTH = 7
 REJECT_INVALID = 8
+REJECT_DISTANCE = 9
 
 LM_SUCCESS = 0
 LM_CONTINUE = -2
@@ -100,7 +102,8 @@ EMITTER_DTYPE = np.dtype(
 
 def n_params(model):
     """Number of fitted parameters of a model code."""
-    return 5 if int(model) == MODEL_SYMMETRIC else 7
+    m = int(model)
+    return 5 if m == MODEL_SYMMETRIC else (4 if m == MODEL_FIXED_WIDTH else 7)
 
 
 def default_half_width(initial_psf_width):

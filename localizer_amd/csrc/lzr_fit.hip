@@ -16,8 +16,8 @@
  *     current and trial points, the residual change, the Jacobian at the trial point
  *     and its normal equations, all in registers (lzr_lane.h);
  *   - the per-lane LM logic (lmpar, ratio test, status) runs in FP64 between passes.
- * Candidates are split by parameter count (5: symmetric, 7: ellipsoidal) on the device
- * and each class runs its own kernel instantiation.
+ * Candidates are split by parameter count (5: symmetric, 7: ellipsoidal, 4: fixed width)
+ * on the device and each class runs its own kernel instantiation.
  */
 #include <hip/hip_runtime.h>
 
@@ -49,22 +49,28 @@ struct KernelArgs {
     LaneConfig cfg;
 };
 
-__device__ __forceinline__ int model_params(int model) { return model == LZR_MODEL_SYMMETRIC ? 5 : 7; }
+/* Queue class of a model: 0 symmetric (5 parameters), 1 ellipsoidal and the
+ * symmetric-PSF filter (7), 2 fixed width (4). */
+constexpr int kClasses = 3;
+__host__ __device__ __forceinline__ int model_class(int model) {
+    return model == LZR_MODEL_SYMMETRIC ? 0 : (model == LZR_MODEL_FIXED_WIDTH ? 2 : 1);
+}
 
 /* ---- work ordering ------------------------------------------------------------
- * Candidates are queued by (parameter count, start intensity): the 5-parameter class
- * first, and inside each class dimmest first.  LM iteration counts fall steeply with
- * the spot's intensity, so the long fits start early and the end of the queue holds
- * short ones; lanes then run out of work at nearly the same time.  Counting sort:
- * histogram -> exclusive scan -> scatter.  counters layout: [0] class-5 count,
- * [1] class-7 count, [2] [3] queue heads, [4 .. 4+kKeys) histogram, then cursors. */
+ * Candidates are queued by (class, start intensity): class 0 first, and inside each
+ * class dimmest first.  LM iteration counts fall steeply with the spot's intensity, so
+ * the long fits start early and the end of the queue holds short ones; lanes then run
+ * out of work at nearly the same time.  Counting sort: histogram -> exclusive scan ->
+ * scatter.  counters layout: [0, kClasses) class counts, [kClasses, 2 kClasses) queue
+ * heads, [kHist, kHist + kKeys) histogram, then the per-key cursors. */
 constexpr int kKeysPerClass = 128; /* 8 buckets per octave of intensity */
-constexpr int kKeys = 2 * kKeysPerClass;
-constexpr int kCounterWords = 4 + 2 * kKeys;
+constexpr int kKeys = kClasses * kKeysPerClass;
+constexpr int kHist = 2 * kClasses;
+constexpr int kCounterWords = kHist + 2 * kKeys;
 
 __device__ __forceinline__ int sort_key(const LzrCandidate &c, int default_model) {
     const int model = c.model < 0 ? default_model : c.model;
-    const int cls = model_params(model) == 5 ? 0 : 1;
+    const int cls = model_class(model);
     const float I = (float)c.intensity;
     int b = I > 1.0f ? (int)(__log2f(I) * 8.0f) : 0;
     b = b < 0 ? 0 : (b >= kKeysPerClass ? kKeysPerClass - 1 : b);
@@ -79,14 +85,14 @@ __global__ void histogram_kernel(const LzrCandidate *cands, int ncands, int defa
     if (i < ncands) atomicAdd(&h[sort_key(cands[i], default_model)], 1u);
     __syncthreads();
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
-        if (h[t]) atomicAdd(&counters[4 + t], h[t]);
+        if (h[t]) atomicAdd(&counters[kHist + t], h[t]);
 }
 
 /* one block of kKeys threads */
 __global__ void scan_kernel(unsigned *counters) {
     __shared__ unsigned v[kKeys];
     const int t = threadIdx.x;
-    const unsigned x = counters[4 + t];
+    const unsigned x = counters[kHist + t];
     v[t] = x;
     __syncthreads();
     for (int off = 1; off < kKeys; off <<= 1) {
@@ -95,9 +101,11 @@ __global__ void scan_kernel(unsigned *counters) {
         v[t] += add;
         __syncthreads();
     }
-    counters[4 + kKeys + t] = v[t] - x; /* exclusive prefix = bucket start */
-    if (t == kKeysPerClass - 1) counters[0] = v[t];
-    if (t == kKeys - 1) counters[1] = v[t] - v[kKeysPerClass - 1];
+    counters[kHist + kKeys + t] = v[t] - x; /* exclusive prefix = bucket start */
+    if ((t + 1) % kKeysPerClass == 0) {
+        const int c = t / kKeysPerClass;
+        counters[c] = v[t] - (c > 0 ? v[c * kKeysPerClass - 1] : 0u);
+    }
 }
 
 __global__ void scatter_kernel(const LzrCandidate *cands, int ncands, int default_model, unsigned *counters,
@@ -114,7 +122,7 @@ __global__ void scatter_kernel(const LzrCandidate *cands, int ncands, int defaul
     }
     __syncthreads();
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
-        if (h[t]) base[t] = atomicAdd(&counters[4 + kKeys + t], h[t]);
+        if (h[t]) base[t] = atomicAdd(&counters[kHist + kKeys + t], h[t]);
     __syncthreads();
     if (i < ncands) order[base[key] + local] = i;
 }
@@ -173,7 +181,9 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     const unsigned long long below = (1ull << lane) - 1ull;
     const unsigned count = r.resume ? *r.in_count : args.count[cls];
     unsigned *head = r.resume ? r.in_head : args.head + cls;
-    const int *order = args.order + (cls == 0 ? 0 : args.count[0]);
+    unsigned first = 0; /* queue entries of the classes before this one */
+    for (int c = 0; c < cls; ++c) first += args.count[c];
+    const int *order = args.order + first;
     const Parked<P> *pin = static_cast<const Parked<P> *>(r.in);
     Parked<P> *pout = static_cast<Parked<P> *>(r.out);
 
@@ -262,13 +272,13 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         acc.zero();
         const int n = active ? s.n : 0;
         if (active) {
-            lane_pass_in<P>(s, args.cfg.sigma, q);
+            lane_pass_in<P>(s, args.cfg, q);
         } else {
             /* idle lanes run the loop on benign constants; their sums are discarded */
             double z[P];
 #pragma unroll
             for (int k = 0; k < P; ++k) z[k] = 1.0;
-            make_pass_in(z, z, 1.0, q);
+            make_pass_in(z, z, 1.0, 1.0, q);
         }
         const int nw = wave_max(n);
         const bool mixed = __any(active && n != nw);
@@ -312,8 +322,8 @@ struct LzrContext {
     int occupancy5 = 1; /* register budget of the 5-parameter kernel: 1 or 2 waves/SIMD */
     int rounds = 1;          /* launches per class; fits are parked between them */
     int park_threshold = 16; /* park a wave's fits when this few lanes are still busy */
-    void *d_park[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}}; /* [class][buffer] */
-    size_t park_cap[2][2] = {{0, 0}, {0, 0}};
+    void *d_park[kClasses][2] = {}; /* [class][buffer] */
+    size_t park_cap[kClasses][2] = {};
     unsigned *d_park_ctr = nullptr; /* [class][buffer] x (count, head) */
     int num_cus = 0;
     std::string err;
@@ -407,7 +417,7 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.cands = cands;
     a.order = ctx->d_order;
     a.count = ctx->d_counters;
-    a.head = ctx->d_counters + 2;
+    a.head = ctx->d_counters + kClasses;
     a.out = out;
     a.cfg = ctx->cfg;
     const int nmax = 2 * ctx->cfg.max_half_width + 1;
@@ -417,6 +427,7 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     else
         run_class<5, AccT, kF64Residual, 2>(ctx, a, 0, lds, stream);
     run_class<7, AccT, kF64Residual, 1>(ctx, a, 1, lds, stream);
+    run_class<4, AccT, kF64Residual, 2>(ctx, a, 2, lds, stream);
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, out,
                        ctx->cfg);
     LZR_CHECK(hipGetLastError());
@@ -437,7 +448,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
     }
     *out = nullptr;
     if (model != LZR_MODEL_SYMMETRIC && model != LZR_MODEL_ELLIPSOIDAL &&
-        model != LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF) {
+        model != LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF && model != LZR_MODEL_FIXED_WIDTH) {
         g_create_error = "lzr_create: unknown model";
         return -1;
     }
@@ -459,7 +470,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         LZR_CHECK(hipGetDeviceProperties(&prop, ctx->device));
         ctx->num_cus = prop.multiProcessorCount;
         LZR_CHECK(hipMalloc((void **)&ctx->d_counters, kCounterWords * sizeof(unsigned)));
-        LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 8 * sizeof(unsigned)));
+        LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 4 * kClasses * sizeof(unsigned)));
     } catch (const std::string &e) {
         g_create_error = "lzr_create: " + e;
         delete ctx;
@@ -486,7 +497,7 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_order);
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_park_ctr);
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < kClasses; ++c)
         for (int b = 0; b < 2; ++b) (void)hipFree(ctx->d_park[c][b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -123,6 +123,15 @@ struct PassIn<5> {
     double A0d, b0d, c0d, inv_sigma_d; /* FP64 copies for the current-point residual */
 };
 
+/* Fixed width (FitPositions_FixedWidthGaussian): c = 1/(2 w^2) is a constant. */
+template <>
+struct PassIn<4> {
+    float A0, x0, y0, b0, c0;
+    float dA, dxc, dyc, db;
+    float A1, kx, inv_sigma;
+    double A0d, b0d, c0d, inv_sigma_d;
+};
+
 template <>
 struct PassIn<7> {
     float A0, x0, y0, b0, qa0, qc0, qb0; /* exponent a ux^2 + 2c ux uy + b uy^2 */
@@ -239,6 +248,39 @@ LZR_HD LZR_INLINE void pixel_sym(const PassIn<5> &q, float ux0, float uy0, float
     a.sf += F;
 }
 
+/* One pixel of the fixed-width model: the symmetric pixel with dc = 0 and no width
+ * column (source/Localization.cpp:1183-1225 keeps columns A, x0, y0, b). */
+template <typename Acc>
+LZR_HD LZR_INLINE void pixel_sym(const PassIn<4> &q, float ux0, float uy0, float uy1, float ddy, float D, double e0d,
+                                 PassAcc<4, Acc> &a) {
+    const float ux1 = ux0 - q.dxc;
+    const float d0 = ux0 * ux0 + uy0 * uy0;
+    const float arg0 = d0 * q.c0;
+    const float e0 = (float)e0d;
+    const float f0 = (float)((q.A0d * e0d + (q.b0d - (double)D)) * q.inv_sigma_d);
+    const float darg = (ddy - q.dxc * (ux1 + ux0)) * q.c0;
+    float e1, e0m;
+    exp_change(e0, arg0 + darg, darg, e1, e0m);
+    const float df = (q.dA * e1 + q.A0 * e0m + q.db) * q.inv_sigma;
+    const float f1 = f0 + df;
+    a.sse0 += Acc(f0) * Acc(f0);
+    a.dsse += Acc(df) * Acc(f0 + f1);
+    const float jA = e1 * q.inv_sigma;
+    const float ae = q.A1 * e1;
+    const float jx = q.kx * ae * ux1;
+    const float jy = q.kx * ae * uy1;
+    const Acc J[3] = {Acc(jA), Acc(jx), Acc(jy)};
+    const Acc F = Acc(f1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] += J[i] * J[j];
+        a.sj[i] += J[i];
+        a.gq[i] += J[i] * F;
+    }
+    a.sf += F;
+}
+
 template <typename Acc>
 LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D, PassAcc<7, Acc> &a) {
     const float ux0 = fi - q.x0, uy0 = fj - q.y0;
@@ -285,10 +327,11 @@ LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D
  * Jacobian at the trial point, normal equations.  nloop >= n is the loop bound shared
  * by the wave; `guard` masks pixels beyond the lane's own n (mixed window sizes).
  * pix(i, j) returns pixel (i, j); xb holds one FP64 value per column.
- * Symmetric model: exp(-c0 (ux^2 + uy^2)) = Ex[i] Ey[j], 2n FP64 exponentials per pass. */
-template <typename Acc, typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void window_pass(const PassIn<5> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
-                                   PassAcc<5, Acc> &a) {
+ * Symmetric and fixed-width models: exp(-c0 (ux^2 + uy^2)) = Ex[i] Ey[j], 2n FP64
+ * exponentials per pass. */
+template <int P, typename Acc, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void window_pass_separable(const PassIn<P> &q, int n, int nloop, bool guard, const Pix &pix,
+                                             XBuf &xb, PassAcc<P, Acc> &a) {
     for (int i = 0; i < nloop; ++i) {
         const double ux = (double)((float)i - q.x0);
         xb.set(i, dexp(-q.c0d * ux * ux));
@@ -305,6 +348,18 @@ LZR_HD LZR_INLINE void window_pass(const PassIn<5> &q, int n, int nloop, bool gu
             for (int i = 0; i < nloop; ++i) pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, pix(i, j), xb.get(i) * Ey, a);
         }
     }
+}
+
+template <typename Acc, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void window_pass(const PassIn<5> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
+                                   PassAcc<5, Acc> &a) {
+    window_pass_separable<5>(q, n, nloop, guard, pix, xb, a);
+}
+
+template <typename Acc, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void window_pass(const PassIn<4> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
+                                   PassAcc<4, Acc> &a) {
+    window_pass_separable<4>(q, n, nloop, guard, pix, xb, a);
 }
 
 template <typename Acc, typename Pix, typename XBuf>
@@ -479,13 +534,10 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
 
 /* ---- model-specific pieces -------------------------------------------------- */
 
+/* Index of x0 in the parameter vector: (A, x0, y0, b), (A, r, x0, y0, b),
+ * (A, sx, sy, x0, y0, rho, b). */
 template <int P>
-LZR_HD LZR_INLINE int x_index() { return P == 5 ? 2 : 3; }
-
-template <int P>
-LZR_HD LZR_INLINE bool width_is_zero(const double (&t)[P]) {
-    return P == 5 ? (t[1] == 0.0) : (t[1] == 0.0 || t[2] == 0.0);
-}
+LZR_HD LZR_INLINE int x_index() { return P == 4 ? 1 : (P == 5 ? 2 : 3); }
 
 /* Absolute-coordinate parameter vector (for test_delta and |D x|). */
 template <int P>
@@ -496,7 +548,30 @@ LZR_HD LZR_INLINE void absolute(const LaneState<P> &s, const double (&t)[P], dou
     a[x_index<P>() + 1] += (double)s.yoff;
 }
 
-LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[5], const double (&dx)[5], double sigma, PassIn<5> &q) {
+/* w: the fixed width of the 4-parameter model (unused by the others). */
+LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[4], const double (&dx)[4], double sigma, double w,
+                                    PassIn<4> &q) {
+    const double c0 = 1.0 / (2.0 * w * w);
+    q.A0 = (float)t0[0];
+    q.x0 = (float)t0[1];
+    q.y0 = (float)t0[2];
+    q.b0 = (float)t0[3];
+    q.c0 = (float)c0;
+    q.dA = (float)dx[0];
+    q.dxc = (float)dx[1];
+    q.dyc = (float)dx[2];
+    q.db = (float)dx[3];
+    q.A1 = (float)(t0[0] + dx[0]);
+    q.inv_sigma = (float)(1.0 / sigma);
+    q.A0d = t0[0];
+    q.b0d = t0[3];
+    q.c0d = c0;
+    q.inv_sigma_d = 1.0 / sigma;
+    q.kx = (float)(1.0 / (w * w * sigma));
+}
+
+LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[5], const double (&dx)[5], double sigma, double,
+                                    PassIn<5> &q) {
     const double r0 = t0[1], r1 = t0[1] + dx[1];
     const double A1 = t0[0] + dx[0];
     const double c0 = 1.0 / (2.0 * r0 * r0), c1 = 1.0 / (2.0 * r1 * r1);
@@ -529,7 +604,8 @@ LZR_HD LZR_INLINE void quad_coeffs(double sx, double sy, double rho, double &qa,
     qc = -pre * rho / (sx * sy);
 }
 
-LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7], double sigma, PassIn<7> &q) {
+LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7], double sigma, double,
+                                    PassIn<7> &q) {
     double qa0, qc0, qb0, qa1, qc1, qb1;
     quad_coeffs(t0[1], t0[2], t0[5], qa0, qc0, qb0);
     const double sx = t0[1] + dx[1], sy = t0[2] + dx[2], rho = t0[5] + dx[5];
@@ -572,15 +648,24 @@ LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7]
 }
 
 /* Column norms of the first `rows` Jacobian rows at t (GSL 1.16 update_diag reads the
- * first p rows of J).  Pixel k = i + n j, absolute coordinates. */
+ * first p rows of J).  Pixel k = i + n j, window coordinates; w is the fixed width. */
 template <int P>
-LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double sigma, int rows, double (&cn)[P]) {
+LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double sigma, double w, int rows,
+                                            double (&cn)[P]) {
 #pragma unroll
     for (int c = 0; c < P; ++c) cn[c] = 0.0;
     for (int k = 0; k < rows; ++k) {
         const int i = k % n, j = k / n;
         double row[P];
-        if (P == 5) {
+        if constexpr (P == 4) {
+            const double A = t[0];
+            const double ux = i - t[1], uy = j - t[2];
+            const double e = (double)fexp((float)(-(ux * ux + uy * uy) / (2.0 * w * w)));
+            row[0] = e / sigma;
+            row[1] = A * e * ux / (w * w * sigma);
+            row[2] = A * e * uy / (w * w * sigma);
+            row[3] = 1.0 / sigma;
+        } else if constexpr (P == 5) {
             const double A = t[0], r = t[1];
             const double ux = i - t[2], uy = j - t[3];
             const double d2 = ux * ux + uy * uy;
@@ -591,7 +676,7 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
             row[3] = A * e * uy / (r * r * sigma);
             row[4] = 1.0 / sigma;
         } else {
-            const double A = t[0], sx = t[1], sy = t[2], rho = t[P == 5 ? 0 : 5];
+            const double A = t[0], sx = t[1], sy = t[2], rho = t[5];
             const double ux = i - t[3], uy = j - t[4];
             const double omr = 1.0 - rho * rho, den = 2.0 * omr * sigma;
             const double q = ux * ux / (sx * sx) + uy * uy / (sy * sy) - 2.0 * rho * ux * uy / (sx * sy);
@@ -599,10 +684,10 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
             row[0] = e / sigma;
             row[1] = -A * (2.0 * rho * ux * uy / (sx * sx * sy) - 2.0 * ux * ux / (sx * sx * sx)) * e / den;
             row[2] = -A * (2.0 * rho * ux * uy / (sx * sy * sy) - 2.0 * uy * uy / (sy * sy * sy)) * e / den;
-            row[3 % P] = -A * (2.0 * rho * uy / (sx * sy) - 2.0 * ux / (sx * sx)) * e / den;
-            row[4 % P] = -A * (2.0 * rho * ux / (sx * sy) - 2.0 * uy / (sy * sy)) * e / den;
-            row[5 % P] = A * (ux * uy / omr / (sx * sy) - rho * (ux * ux + uy * uy - 2.0 * rho * ux * uy) / (sx * sy) / omr / omr) * e / sigma;
-            row[(P - 1)] = 1.0 / sigma;
+            row[3] = -A * (2.0 * rho * uy / (sx * sy) - 2.0 * ux / (sx * sx)) * e / den;
+            row[4] = -A * (2.0 * rho * ux / (sx * sy) - 2.0 * uy / (sy * sy)) * e / den;
+            row[5] = A * (ux * uy / omr / (sx * sy) - rho * (ux * ux + uy * uy - 2.0 * rho * ux * uy) / (sx * sy) / omr / omr) * e / sigma;
+            row[6] = 1.0 / sigma;
         }
 #pragma unroll
         for (int c = 0; c < P; ++c) cn[c] += row[c] * row[c];
@@ -682,20 +767,25 @@ LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const 
     s.n = n;
     s.frame = c.frame;
     const double w = cfg.initial_psf_width;
-    if (P == 5) {
+    if constexpr (P == 4) {
+        s.th[0] = c.intensity;
+        s.th[1] = c.x - xoff;
+        s.th[2] = c.y - yoff;
+        s.th[3] = c.background;
+    } else if constexpr (P == 5) {
         s.th[0] = c.intensity;
         s.th[1] = w;
         s.th[2] = c.x - xoff;
         s.th[3] = c.y - yoff;
-        s.th[4 % P] = c.background;
+        s.th[4] = c.background;
     } else {
         s.th[0] = c.intensity;
         s.th[1] = w;
         s.th[2] = w;
         s.th[3] = c.x - xoff;
         s.th[4] = c.y - yoff;
-        s.th[5 % P] = 0.0;
-        s.th[6 % P] = c.background;
+        s.th[5] = 0.0;
+        s.th[6] = c.background;
     }
 #pragma unroll
     for (int i = 0; i < P; ++i) s.dx[i] = 0.0;
@@ -709,11 +799,11 @@ LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const 
 /* FP32 pass constants for the lane's next evaluation (trial point = th + dx for a
  * trial, th itself for the initial evaluation). */
 template <int P>
-LZR_HD LZR_INLINE void lane_pass_in(const LaneState<P> &s, double sigma, PassIn<P> &q) {
+LZR_HD LZR_INLINE void lane_pass_in(const LaneState<P> &s, const LaneConfig &cfg, PassIn<P> &q) {
     double step[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
-    make_pass_in(s.th, step, sigma, q);
+    make_pass_in(s.th, step, cfg.sigma, cfg.initial_psf_width, q);
 }
 
 template <int P>
@@ -872,7 +962,7 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
                 cn[i] = dsqrt(hii == 0.0 ? 1.0 : hii);
             }
         } else {
-            partial_column_norms<P>(s.th, s.n, cfg.sigma, rows, cn);
+            partial_column_norms<P>(s.th, s.n, cfg.sigma, cfg.initial_psf_width, rows, cn);
         }
 #pragma unroll
         for (int i = 0; i < P; ++i)
@@ -967,7 +1057,8 @@ LZR_HD LZR_INLINE bool covariance_diagonal(const float (&Hf)[NP<P>::S], double (
 }
 
 /* Post-checks and output record of the reference drivers
- * (source/Localization.cpp:157-211 symmetric, :506-596 ellipsoidal, :606-631 filter),
+ * (source/Localization.cpp:157-211 symmetric, :336-378 fixed width, :506-596 ellipsoidal,
+ * :606-631 filter),
  * from the raw end state r of candidate c. */
 template <int P>
 LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, LzrFit &out) {
@@ -991,7 +1082,37 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
     out.accepted_steps = r.accepted_steps;
     out.model = (int16_t)model;
     out.chi = chi;
-    if (P == 5) {
+    if constexpr (P == 4) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) out.params[i] = x[i];
+        if (x[0] <= 0) {
+            out.status = LZR_REJECT_AMPLITUDE;
+            return;
+        }
+        if (x[1] < xo || x[1] > xm || x[2] < yo || x[2] > ym) {
+            out.status = LZR_REJECT_POSITION;
+            return;
+        }
+        /* fitted position within 2 w of the candidate (:349-353) */
+        if (dabs(x[1] - c.x) > 2.0 * w || dabs(x[2] - c.y) > 2.0 * w) {
+            out.status = LZR_REJECT_DISTANCE;
+            return;
+        }
+        double cd[P];
+        covariance_diagonal<P>(Hf, cd);
+        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 4);
+        const double q = chi / dsqrt(dof);
+        const double cs = 1.0 > q ? 1.0 : q;
+#pragma unroll
+        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
+        out.integral = 2 * kPi * w * w * x[0];
+        /* the reference scales the amplitude's relative error by 2 pi I w^2 (:371-372) */
+        const double relA = cs * dsqrt(cd[0]) / x[0];
+        out.integral_deviation = 2 * kPi * out.integral * w * w * dsqrt(relA * relA);
+        out.sd1 = out.sd2 = w;
+        out.status = LZR_ACCEPTED;
+        return;
+    } else if constexpr (P == 5) {
         x[1] = dabs(x[1]);
 #pragma unroll
         for (int i = 0; i < P; ++i) out.params[i] = x[i];
@@ -1024,56 +1145,57 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
         out.theta = 0.0;
         out.status = LZR_ACCEPTED;
         return;
-    }
-    x[1] = dabs(x[1]);
-    x[2] = dabs(x[2]);
+    } else {
+        x[1] = dabs(x[1]);
+        x[2] = dabs(x[2]);
 #pragma unroll
-    for (int i = 0; i < P; ++i) out.params[i] = x[i];
-    if (x[0] <= 0) {
-        out.status = LZR_REJECT_AMPLITUDE;
-        return;
-    }
-    if (x[3] < xo || x[3] > xm || x[4 % P] < yo || x[4 % P] > ym) {
-        out.status = LZR_REJECT_POSITION;
-        return;
-    }
-    double sd1, sd2, theta;
-    if (!eigen2(x[1], x[2], x[5 % P], sd1, sd2, theta)) {
-        out.status = LZR_REJECT_EIGEN;
-        return;
-    }
-    if (!(((sd1 >= 0.4 * w) && (sd1 <= 1.5 * w)) || ((sd2 >= 0.4 * w) && (sd2 <= 1.5 * w)))) {
-        out.status = LZR_REJECT_WIDTH;
-        return;
-    }
-    double cd[P];
-    covariance_diagonal<P>(Hf, cd);
-    const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
-    const double q = chi / dsqrt(dof);
-    const double cs = 1.0 > q ? 1.0 : q;
+        for (int i = 0; i < P; ++i) out.params[i] = x[i];
+        if (x[0] <= 0) {
+            out.status = LZR_REJECT_AMPLITUDE;
+            return;
+        }
+        if (x[3] < xo || x[3] > xm || x[4] < yo || x[4] > ym) {
+            out.status = LZR_REJECT_POSITION;
+            return;
+        }
+        double sd1, sd2, theta;
+        if (!eigen2(x[1], x[2], x[5], sd1, sd2, theta)) {
+            out.status = LZR_REJECT_EIGEN;
+            return;
+        }
+        if (!(((sd1 >= 0.4 * w) && (sd1 <= 1.5 * w)) || ((sd2 >= 0.4 * w) && (sd2 <= 1.5 * w)))) {
+            out.status = LZR_REJECT_WIDTH;
+            return;
+        }
+        double cd[P];
+        covariance_diagonal<P>(Hf, cd);
+        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
+        const double q = chi / dsqrt(dof);
+        const double cs = 1.0 > q ? 1.0 : q;
 #pragma unroll
-    for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
-    const double rho = x[5 % P];
-    const double corr_dev = out.deviations[5 % P];
-    const double sxd = out.deviations[1], syd = out.deviations[2];
-    out.sd1 = sd1;
-    out.sd2 = sd2;
-    out.theta = theta;
-    out.sd1_deviation = sxd * dcos(theta) + syd * dsin(theta);
-    out.sd2_deviation = sxd * dsin(theta) + syd * dcos(theta);
-    const double omr = 1 - rho * rho;
-    out.integral = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2;
-    const double relA = out.deviations[0] / x[0];
-    out.integral_deviation = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2 *
-                             dsqrt(relA * relA + rho * rho * corr_dev * corr_dev / omr / omr +
-                                   out.sd1_deviation * out.sd1_deviation / sd1 / sd1 +
-                                   out.sd2_deviation * out.sd2_deviation / sd2 / sd2);
-    out.status = LZR_ACCEPTED;
-    if (model == LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF) {
-        if ((0.75 * sd1 > sd2) || (1.25 * sd1 < sd2))
-            out.status = LZR_REJECT_ANISOTROPIC;
-        else if (((sd1 < w * 0.7) || (sd1 > w * 1.3)) && ((sd2 < w * 0.7) || (sd2 > w * 1.3)))
-            out.status = LZR_REJECT_PSF_WIDTH;
+        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
+        const double rho = x[5];
+        const double corr_dev = out.deviations[5];
+        const double sxd = out.deviations[1], syd = out.deviations[2];
+        out.sd1 = sd1;
+        out.sd2 = sd2;
+        out.theta = theta;
+        out.sd1_deviation = sxd * dcos(theta) + syd * dsin(theta);
+        out.sd2_deviation = sxd * dsin(theta) + syd * dcos(theta);
+        const double omr = 1 - rho * rho;
+        out.integral = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2;
+        const double relA = out.deviations[0] / x[0];
+        out.integral_deviation = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2 *
+                                 dsqrt(relA * relA + rho * rho * corr_dev * corr_dev / omr / omr +
+                                       out.sd1_deviation * out.sd1_deviation / sd1 / sd1 +
+                                       out.sd2_deviation * out.sd2_deviation / sd2 / sd2);
+        out.status = LZR_ACCEPTED;
+        if (model == LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF) {
+            if ((0.75 * sd1 > sd2) || (1.25 * sd1 < sd2))
+                out.status = LZR_REJECT_ANISOTROPIC;
+            else if (((sd1 < w * 0.7) || (sd1 > w * 1.3)) && ((sd2 < w * 0.7) || (sd2 > w * 1.3)))
+                out.status = LZR_REJECT_PSF_WIDTH;
+        }
     }
 }
 
@@ -1085,6 +1207,8 @@ LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrC
     if (r.status != kPending) return;
     if (r.model == LZR_MODEL_SYMMETRIC)
         finish_record<5>(r, cfg, c, rec);
+    else if (r.model == LZR_MODEL_FIXED_WIDTH)
+        finish_record<4>(r, cfg, c, rec);
     else
         finish_record<7>(r, cfg, c, rec);
 }

@@ -5,6 +5,7 @@
  *   FitPositions_SymmetricGaussianHIP(initialPSFWidth, sigma)      <- FitPositions_SymmetricGaussian
  *   FitPositions_EllipsoidalGaussianHIP(initialPSFWidth, sigma)    <- FitPositions_EllipsoidalGaussian
  *   FitPositions_EllipsoidalGaussian_SymmetricPSFHIP(w, sigma)     <- FitPositions_EllipsoidalGaussian_SymmetricPSF
+ *   FitPositions_FixedWidthGaussianHIP(initialPSFWidth, sigma)     <- FitPositions_FixedWidthGaussian
  *   (source/Localization.h:55-121)
  *
  * Same constructor arguments, the same virtual fit_positions(ImagePtr, list<Particle>)
@@ -164,6 +165,17 @@ private:
             p->yPositionDeviation = r.deviations[3];
             p->backgroundDeviation = r.deviations[4];
             c.addPosition(p);
+        } else if (model_ == LZR_MODEL_FIXED_WIDTH) {
+            std::shared_ptr<LocalizedPosition_2DGaussFixedWidth> p(new LocalizedPosition_2DGaussFixedWidth());
+            p->integral = r.integral;
+            p->xPosition = r.params[1];
+            p->yPosition = r.params[2];
+            p->background = r.params[3];
+            p->integralDeviation = r.integral_deviation;
+            p->xPositionDeviation = r.deviations[1];
+            p->yPositionDeviation = r.deviations[2];
+            p->backgroundDeviation = r.deviations[3];
+            c.addPosition(p);
         } else {
             std::shared_ptr<LocalizedPosition_Ellipsoidal2DGauss> p(new LocalizedPosition_Ellipsoidal2DGauss());
             p->integral = r.integral;
@@ -237,6 +249,24 @@ public:
     std::vector<std::shared_ptr<LocalizedPositionsContainer> > fit_positions_batch(
         const std::vector<ImagePtr> &images, const std::vector<std::shared_ptr<std::list<Particle> > > &lists) {
         return core_.fit_many<LocalizedPositionsContainer_Ellipsoidal2DGaussian>(images, lists);
+    }
+    LzrContext *context() { return core_.context(); }
+
+private:
+    lzr_dropin::FitterCore core_;
+};
+
+class FitPositions_FixedWidthGaussianHIP : public FitPositions {
+public:
+    FitPositions_FixedWidthGaussianHIP(double initialPSFWidth_rhs, double sigma_rhs)
+        : core_(LZR_MODEL_FIXED_WIDTH, initialPSFWidth_rhs, sigma_rhs) {}
+    std::shared_ptr<LocalizedPositionsContainer> fit_positions(const ImagePtr image,
+                                                               std::shared_ptr<std::list<Particle> > particles) {
+        return core_.fit_one<LocalizedPositionsContainer_2DGaussFixedWidth>(image, particles);
+    }
+    std::vector<std::shared_ptr<LocalizedPositionsContainer> > fit_positions_batch(
+        const std::vector<ImagePtr> &images, const std::vector<std::shared_ptr<std::list<Particle> > > &lists) {
+        return core_.fit_many<LocalizedPositionsContainer_2DGaussFixedWidth>(images, lists);
     }
     LzrContext *context() { return core_.context(); }
 

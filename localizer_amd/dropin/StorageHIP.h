@@ -8,9 +8,11 @@
  *   Image / ImagePtr                        source/Defines.h:44-45 (column-major, rows = x)
  *   Particle                                source/Storage.h:90-101
  *   LocalizedPosition_2DGauss               source/Storage.h:117-138
+ *   LocalizedPosition_2DGaussFixedWidth     source/Storage.h:140-159
  *   LocalizedPosition_Ellipsoidal2DGauss    source/Storage.h:161-186
- *   LocalizedPositionsContainer(_2DGauss, _Ellipsoidal2DGaussian)
- *                                           source/Storage.h:268-480, source/Storage.cpp:174-345, :614-700
+ *   LocalizedPositionsContainer(_2DGauss, _2DGaussFixedWidth, _Ellipsoidal2DGaussian)
+ *                                           source/Storage.h:268-480, source/Storage.cpp:174-345,
+ *                                           :405-476, :614-700
  * Only the members the fitting path and its callers use are provided.
  */
 #ifndef LZR_STORAGE_HIP_H
@@ -47,6 +49,7 @@ typedef std::shared_ptr<Image> ImagePtr;
 const double PI = 3.1415926535897932384626433;
 
 const size_t LOCALIZED_POSITIONS_TYPE_2DGAUSS = 0;
+const size_t LOCALIZED_POSITIONS_TYPE_2DGAUSS_FIXED_WIDTH = 1;
 const size_t LOCALIZED_POSITIONS_TYPE_ELLIPSOIDAL2DGAUSS = 5;
 
 class Particle {
@@ -83,6 +86,22 @@ public:
     double yPositionDeviation;
     double backgroundDeviation;
     size_t getPositionType() const { return LOCALIZED_POSITIONS_TYPE_2DGAUSS; }
+};
+
+class LocalizedPosition_2DGaussFixedWidth : public LocalizedPosition {
+public:
+    LocalizedPosition_2DGaussFixedWidth() : frameNumber(0), nFramesPresent(1) {}
+    size_t frameNumber;
+    size_t nFramesPresent;
+    double integral;
+    double xPosition;
+    double yPosition;
+    double background;
+    double integralDeviation;
+    double xPositionDeviation;
+    double yPositionDeviation;
+    double backgroundDeviation;
+    size_t getPositionType() const { return LOCALIZED_POSITIONS_TYPE_2DGAUSS_FIXED_WIDTH; }
 };
 
 class LocalizedPosition_Ellipsoidal2DGauss : public LocalizedPosition {
@@ -194,6 +213,45 @@ public:
               << "\t" << p.background << "\t" << p.integralDeviation << "\t" << p.widthDeviation << "\t"
               << p.xPositionDeviation << "\t" << p.yPositionDeviation << "\t" << p.backgroundDeviation << "\t"
               << p.nFramesPresent << "\n";
+            if (f.fail()) throw std::runtime_error("Error writing to the localized positions file");
+        }
+    }
+};
+
+class LocalizedPositionsContainer_2DGaussFixedWidth
+    : public LzrTypedContainer<LocalizedPosition_2DGaussFixedWidth, LOCALIZED_POSITIONS_TYPE_2DGAUSS_FIXED_WIDTH, 10> {
+public:
+    double getIntegralDeviation(size_t i) const { return positionsVector.at(i).integralDeviation; }
+    double getXPositionDeviation(size_t i) const { return positionsVector.at(i).xPositionDeviation; }
+    double getYPositionDeviation(size_t i) const { return positionsVector.at(i).yPositionDeviation; }
+    double getBackgroundDeviation(size_t i) const { return positionsVector.at(i).backgroundDeviation; }
+    /* N x 10 matrix, columns of source/Storage.cpp:405-425 */
+    ImagePtr getLocalizedPositionsAsMatrix() const {
+        const size_t n = positionsVector.size();
+        ImagePtr m(new Image((long)n, 10));
+        for (size_t i = 0; i < n; ++i) {
+            const LocalizedPosition_2DGaussFixedWidth &p = positionsVector[i];
+            const double row[10] = {(double)p.frameNumber, p.integral, p.xPosition, p.yPosition, p.background,
+                                    p.integralDeviation, p.xPositionDeviation, p.yPositionDeviation,
+                                    p.backgroundDeviation, (double)p.nFramesPresent};
+            for (int c = 0; c < 10; ++c) (*m)((long)i, c) = row[c];
+        }
+        return m;
+    }
+    /* tab-separated text, layout of source/Storage.cpp:427-476 */
+    void writePositionsToFile(std::string filePath, std::string header) const {
+        if (positionsVector.empty()) throw std::runtime_error("No positions localized!");
+        std::ofstream f(filePath.c_str(), std::ios::trunc);
+        if (f.fail()) throw std::runtime_error("Unable to create the output file");
+        f << "Localized positions using symmetric 2D Gauss fitting with fixed PSF width\n" << header
+          << "\nDATA FOLLOWS\n";
+        f << "First frame\tIntegrated intensity\tX position (pixel)\tY position (pixel)\tBackground\t"
+             "Intensity deviation\tX position deviation (pixel)\tY position deviation (pixel)\t"
+             "Background deviation\tNumber of frames where this emitter is present\n";
+        for (const auto &p : positionsVector) {
+            f << p.frameNumber << "\t" << p.integral << "\t" << p.xPosition << "\t" << p.yPosition << "\t"
+              << p.background << "\t" << p.integralDeviation << "\t" << p.xPositionDeviation << "\t"
+              << p.yPositionDeviation << "\t" << p.backgroundDeviation << "\t" << p.nFramesPresent << "\n";
             if (f.fail()) throw std::runtime_error("Error writing to the localized positions file");
         }
     }

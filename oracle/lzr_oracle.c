@@ -169,6 +169,31 @@ int orc_jacobian(int model, const double *theta, int n, double xoff, double yoff
     return jacobian_ellipsoidal(theta, n, xoff, yoff, sigma, J);
 }
 
+/* FitFunction_/Jacobian_FixedWidthGaussian (source/Localization.cpp:1050-1076,
+ * :1183-1225): the symmetric model at r = width, Jacobian columns A, x0, y0, b. */
+int orc_model_fixed_width(const double *theta, double width, const double *data, int n, double xoff,
+                          double yoff, double sigma, double *f) {
+    const double t5[5] = {theta[0], width, theta[1], theta[2], theta[3]};
+    return model_symmetric(t5, data, n, xoff, yoff, sigma, f);
+}
+
+int orc_jacobian_fixed_width(const double *theta, double width, int n, double xoff, double yoff,
+                             double sigma, double *J) {
+    const double t5[5] = {theta[0], width, theta[1], theta[2], theta[3]};
+    double *J5 = (double *)malloc(sizeof(double) * 5 * n * n);
+    const int st = jacobian_symmetric(t5, n, xoff, yoff, sigma, J5);
+    if (st == ST_OK) {
+        for (int k = 0; k < n * n; ++k) {
+            J[4 * k + 0] = J5[5 * k + 0];
+            J[4 * k + 1] = J5[5 * k + 2];
+            J[4 * k + 2] = J5[5 * k + 3];
+            J[4 * k + 3] = J5[5 * k + 4];
+        }
+    }
+    free(J5);
+    return st;
+}
+
 /* ------------------------------------------------------------------------------ */
 /* Dense linear algebra                                                            */
 /* ------------------------------------------------------------------------------ */
@@ -444,6 +469,7 @@ static void lmpar(int p, const double *R, int ldr, const int *perm, const double
 
 typedef struct Lm {
     int N, p, model, n;
+    double fixed_width; /* LZR_MODEL_FIXED_WIDTH: the width r held at initialPSFWidth */
     const double *data;
     double xoff, yoff, sigma;
     OrcOptions opt;
@@ -459,10 +485,14 @@ typedef struct Lm {
 } Lm;
 
 static int eval_f(Lm *s, const double *t, double *f) {
+    if (s->model == LZR_MODEL_FIXED_WIDTH)
+        return orc_model_fixed_width(t, s->fixed_width, s->data, s->n, s->xoff, s->yoff, s->sigma, f);
     return orc_model(s->model, t, s->data, s->n, s->xoff, s->yoff, s->sigma, f);
 }
 
 static int eval_J(Lm *s, const double *t, double *J) {
+    if (s->model == LZR_MODEL_FIXED_WIDTH)
+        return orc_jacobian_fixed_width(t, s->fixed_width, s->n, s->xoff, s->yoff, s->sigma, J);
     return orc_jacobian(s->model, t, s->n, s->xoff, s->yoff, s->sigma, J);
 }
 
@@ -635,10 +665,11 @@ void orc_covar(int N, int p, const double *J, double *C) {
 int orc_lm_run(int model, const double *data, int n, double xoff, double yoff, double sigma,
                const double *theta0, const OrcOptions *opt, double *theta, double *f_out,
                double *J_out, int *iterations, int *lm_status, int *accepted, double *trace,
-               int max_trace) {
+               int max_trace, double fixed_width) {
     Lm *s = (Lm *)calloc(1, sizeof(Lm));
-    s->model = model == LZR_MODEL_SYMMETRIC ? LZR_MODEL_SYMMETRIC : LZR_MODEL_ELLIPSOIDAL;
-    s->p = s->model == LZR_MODEL_SYMMETRIC ? 5 : 7;
+    s->model = (model == LZR_MODEL_SYMMETRIC || model == LZR_MODEL_FIXED_WIDTH) ? model : LZR_MODEL_ELLIPSOIDAL;
+    s->p = s->model == LZR_MODEL_SYMMETRIC ? 5 : (s->model == LZR_MODEL_FIXED_WIDTH ? 4 : 7);
+    s->fixed_width = fixed_width;
     s->n = n;
     s->N = n * n;
     s->data = data;
@@ -711,7 +742,7 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
                                                               : (int)ceil(w * 4.0));
     if (h < 1 || h > LZR_MAX_HALF_WIDTH ||
         (model != LZR_MODEL_SYMMETRIC && model != LZR_MODEL_ELLIPSOIDAL &&
-         model != LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF)) {
+         model != LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF && model != LZR_MODEL_FIXED_WIDTH)) {
         out->status = LZR_REJECT_INVALID;
         return;
     }
@@ -724,13 +755,19 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
         return;
     }
     const int sym = model == LZR_MODEL_SYMMETRIC;
-    const int p = sym ? 5 : 7;
+    const int fixed = model == LZR_MODEL_FIXED_WIDTH;
+    const int p = sym ? 5 : (fixed ? 4 : 7);
     double data[ORC_MAXN];
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) data[i + n * j] = (double)frame[(long)(yoff + j) * nx + xoff + i];
 
     double t0[ORC_MAXP];
-    if (sym) {
+    if (fixed) {
+        t0[0] = cand->intensity;
+        t0[1] = cand->x;
+        t0[2] = cand->y;
+        t0[3] = cand->background;
+    } else if (sym) {
         t0[0] = cand->intensity;
         t0[1] = w;
         t0[2] = cand->x;
@@ -746,7 +783,8 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
         t0[6] = cand->background;
     }
     Lm *lm = (Lm *)calloc(1, sizeof(Lm));
-    lm->model = sym ? LZR_MODEL_SYMMETRIC : LZR_MODEL_ELLIPSOIDAL;
+    lm->model = sym ? LZR_MODEL_SYMMETRIC : (fixed ? LZR_MODEL_FIXED_WIDTH : LZR_MODEL_ELLIPSOIDAL);
+    lm->fixed_width = w;
     lm->p = p;
     lm->n = n;
     lm->N = n * n;
@@ -773,7 +811,28 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
     double *x = lm->x;
     out->chi = orc_nrm2(lm->N, lm->f, 1);
 
-    if (sym) {
+    if (fixed) {
+        /* source/Localization.cpp:336-378 */
+        memcpy(out->params, x, sizeof(double) * p);
+        if (x[0] <= 0) {
+            out->status = LZR_REJECT_AMPLITUDE;
+        } else if (x[1] < xoff || x[1] > xmax || x[2] < yoff || x[2] > ymax) {
+            out->status = LZR_REJECT_POSITION;
+        } else if ((fabs(x[1] - cand->x) > 2.0 * w) || (fabs(x[2] - cand->y) > 2.0 * w)) {
+            out->status = LZR_REJECT_DISTANCE;
+        } else {
+            double C[ORC_MAXP * ORC_MAXP];
+            orc_covar(lm->N, p, lm->J, C);
+            const double dof = (double)((2 * h - 1) * (2 * h - 1) - 4);
+            const double c = gsl_max(1.0, out->chi / sqrt(dof));
+            for (int k = 0; k < p; ++k) out->deviations[k] = c * sqrt(C[k * p + k]);
+            out->integral = 2 * ORC_PI * w * w * x[0];
+            const double relA = c * sqrt(C[0]) / x[0];
+            out->integral_deviation = 2 * ORC_PI * out->integral * w * w * sqrt(relA * relA);
+            out->sd1 = out->sd2 = w;
+            out->status = LZR_ACCEPTED;
+        }
+    } else if (sym) {
         x[1] = fabs(x[1]);
         memcpy(out->params, x, sizeof(double) * p);
         if (x[0] <= 0) {

@@ -67,6 +67,12 @@ int orc_model(int model, const double *theta, const double *data, int n, double 
 int orc_jacobian(int model, const double *theta, int n, double xoff, double yoff,
                  double sigma, double *J);
 
+/* Fixed-width model: theta = (A, x0, y0, b), r = width; J is N x 4. */
+int orc_model_fixed_width(const double *theta, double width, const double *data, int n, double xoff,
+                          double yoff, double sigma, double *f);
+int orc_jacobian_fixed_width(const double *theta, double width, int n, double xoff, double yoff,
+                             double sigma, double *J);
+
 /* BLAS dnrm2 (scaled sum of squares). */
 double orc_nrm2(int n, const double *x, int stride);
 
@@ -81,11 +87,12 @@ void orc_covar(int N, int p, const double *J, double *C);
 /* Runs the LM loop of the reference driver on one window (model, data n x n at
  * (xoff, yoff)) from theta0.  Writes the final theta, and if trace != NULL the accepted
  * point after every trial evaluation (trace[t*p + i], t < max_trace).  Returns the
- * number of trials; *iterations, *lm_status, *accepted receive the loop counters. */
+ * number of trials; *iterations, *lm_status, *accepted receive the loop counters.
+ * fixed_width is the width of LZR_MODEL_FIXED_WIDTH (ignored by the other models). */
 int orc_lm_run(int model, const double *data, int n, double xoff, double yoff, double sigma,
                const double *theta0, const OrcOptions *opt, double *theta, double *f_out,
                double *J_out, int *iterations, int *lm_status, int *accepted, double *trace,
-               int max_trace);
+               int max_trace, double fixed_width);
 
 /* Eigen-decomposition of [[sx^2, rho sx sy], [rho sx sy, sy^2]] as used at
  * source/Localization.cpp:525-547: ascending eigenvalues, sd = sqrt(lambda),

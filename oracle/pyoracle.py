@@ -49,6 +49,12 @@ def lib():
         L.orc_model.restype = ctypes.c_int
         L.orc_jacobian.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_jacobian.restype = ctypes.c_int
+        L.orc_model_fixed_width.argtypes = [_dp, ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, _dp]
+        L.orc_model_fixed_width.restype = ctypes.c_int
+        L.orc_jacobian_fixed_width.argtypes = [_dp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, _dp]
+        L.orc_jacobian_fixed_width.restype = ctypes.c_int
         L.orc_nrm2.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
         L.orc_nrm2.restype = ctypes.c_double
         L.orc_test_delta.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double]
@@ -57,7 +63,7 @@ def lib():
         L.orc_covar.restype = None
         L.orc_lm_run.argtypes = [
             ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-            _dp, ctypes.POINTER(Options), _dp, _dp, _dp, _ip, _ip, _ip, _dp, ctypes.c_int,
+            _dp, ctypes.POINTER(Options), _dp, _dp, _dp, _ip, _ip, _ip, _dp, ctypes.c_int, ctypes.c_double,
         ]
         L.orc_lm_run.restype = ctypes.c_int
         L.orc_eigen2.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
@@ -131,10 +137,28 @@ def covar(J):
     return C
 
 
-def lm_run(model_code, data, n, xoff, yoff, theta0, sigma=1.0, options=None, max_trace=0):
+def model_fixed_width(theta, width, data, n, xoff, yoff, sigma=1.0):
+    """Fixed-width model residuals, theta = (A, x0, y0, b)."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    data = np.ascontiguousarray(data, np.float64).reshape(-1)
+    f = np.empty(n * n)
+    st = lib().orc_model_fixed_width(_ptr(theta), float(width), _ptr(data), n, float(xoff), float(yoff),
+                                     float(sigma), _ptr(f))
+    return st, f
+
+
+def jacobian_fixed_width(theta, width, n, xoff, yoff, sigma=1.0):
+    theta = np.ascontiguousarray(theta, np.float64)
+    J = np.empty((n * n, 4))
+    st = lib().orc_jacobian_fixed_width(_ptr(theta), float(width), n, float(xoff), float(yoff), float(sigma),
+                                        _ptr(J))
+    return st, J
+
+
+def lm_run(model_code, data, n, xoff, yoff, theta0, sigma=1.0, options=None, max_trace=0, fixed_width=0.0):
     """Runs the reference LM loop on one window.  Returns a dict with theta, f, J,
     iterations, lm_status, accepted, trials and (if max_trace) the accepted point after
-    every trial."""
+    every trial.  fixed_width: the width of the fixed-width model."""
     p = abi.n_params(model_code)
     opt = options if options is not None else default_options()
     data = np.ascontiguousarray(data, np.float64).reshape(-1)
@@ -147,7 +171,7 @@ def lm_run(model_code, data, n, xoff, yoff, theta0, sigma=1.0, options=None, max
     trials = lib().orc_lm_run(
         int(model_code), _ptr(data), n, float(xoff), float(yoff), float(sigma), _ptr(theta0),
         ctypes.byref(opt), _ptr(theta), _ptr(f), _ptr(J), ctypes.byref(it), ctypes.byref(st),
-        ctypes.byref(acc), _ptr(trace) if max_trace else None, int(max_trace),
+        ctypes.byref(acc), _ptr(trace) if max_trace else None, int(max_trace), float(fixed_width),
     )
     return dict(theta=theta, f=f, J=J, iterations=it.value, lm_status=st.value, accepted=acc.value,
                 trials=trials, trace=trace[: min(trials, max_trace)] if max_trace else None)

@@ -66,6 +66,8 @@ int main(int argc, char **argv) {
             fitter.reset(new FitPositions_SymmetricGaussianHIP(w, sigma));
         else if (model == LZR_MODEL_ELLIPSOIDAL)
             fitter.reset(new FitPositions_EllipsoidalGaussianHIP(w, sigma));
+        else if (model == LZR_MODEL_FIXED_WIDTH)
+            fitter.reset(new FitPositions_FixedWidthGaussianHIP(w, sigma));
         else
             fitter.reset(new FitPositions_EllipsoidalGaussian_SymmetricPSFHIP(w, sigma));
         if (mode == "single") {
@@ -82,6 +84,8 @@ int main(int argc, char **argv) {
                 res = static_cast<FitPositions_SymmetricGaussianHIP *>(fitter.get())->fit_positions_batch(images, lists);
             else if (model == LZR_MODEL_ELLIPSOIDAL)
                 res = static_cast<FitPositions_EllipsoidalGaussianHIP *>(fitter.get())->fit_positions_batch(images, lists);
+            else if (model == LZR_MODEL_FIXED_WIDTH)
+                res = static_cast<FitPositions_FixedWidthGaussianHIP *>(fitter.get())->fit_positions_batch(images, lists);
             else
                 res = static_cast<FitPositions_EllipsoidalGaussian_SymmetricPSFHIP *>(fitter.get())
                           ->fit_positions_batch(images, lists);

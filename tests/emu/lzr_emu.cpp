@@ -20,7 +20,7 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
         for (int i = 0; i < n; ++i) D[i + n * j] = (float)src[(size_t)j * nx + i];
     for (;;) {
         PassIn<P> q;
-        lane_pass_in<P>(s, cfg.sigma, q);
+        lane_pass_in<P>(s, cfg, q);
         PassAcc<P, Acc> a;
         a.zero();
         struct {
@@ -64,6 +64,9 @@ extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, i
         if (m == LZR_MODEL_SYMMETRIC) {
             if (dacc) { if (f64) LZR_EMU_CALL(5, double, true); else LZR_EMU_CALL(5, double, false); }
             else { if (f64) LZR_EMU_CALL(5, float, true); else LZR_EMU_CALL(5, float, false); }
+        } else if (m == LZR_MODEL_FIXED_WIDTH) {
+            if (dacc) { if (f64) LZR_EMU_CALL(4, double, true); else LZR_EMU_CALL(4, double, false); }
+            else { if (f64) LZR_EMU_CALL(4, float, true); else LZR_EMU_CALL(4, float, false); }
         } else {
             if (dacc) { if (f64) LZR_EMU_CALL(7, double, true); else LZR_EMU_CALL(7, double, false); }
             else { if (f64) LZR_EMU_CALL(7, float, true); else LZR_EMU_CALL(7, float, false); }

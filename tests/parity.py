@@ -3,13 +3,14 @@ import numpy as np
 
 from localizer_amd import abi
 
-NAMES = {5: ["A", "r", "x0", "y0", "b"], 7: ["A", "sx", "sy", "x0", "y0", "rho", "b"]}
+NAMES = {4: ["A", "x0", "y0", "b"], 5: ["A", "r", "x0", "y0", "b"], 7: ["A", "sx", "sy", "x0", "y0", "rho", "b"]}
+X_INDEX = {4: 1, 5: 2, 7: 3}
 
 
 def compare(ref, got, model):
     """Returns a dict of agreement statistics between reference and candidate fits."""
     p = abi.n_params(model)
-    xi = 2 if p == 5 else 3
+    xi = X_INDEX[p]
     acc_r = ref["status"] == 0
     acc_g = got["status"] == 0
     both = acc_r & acc_g

@@ -22,7 +22,8 @@ def test_dropin_compiles_against_reference_headers():
     src = ('#include "%s/localizer_amd/dropin/LocalizationHIP.h"\n'
            "int main() { FitPositions *f = new FitPositions_SymmetricGaussianHIP(1.3, 1.0); delete f;\n"
            "  f = new FitPositions_EllipsoidalGaussianHIP(1.3, 1.0); delete f;\n"
-           "  f = new FitPositions_EllipsoidalGaussian_SymmetricPSFHIP(1.3, 1.0); delete f; return 0; }\n" % ROOT)
+           "  f = new FitPositions_EllipsoidalGaussian_SymmetricPSFHIP(1.3, 1.0); delete f;\n"
+           "  f = new FitPositions_FixedWidthGaussianHIP(1.3, 1.0); delete f; return 0; }\n" % ROOT)
     cmd = ["g++", "-std=c++11", "-fsyntax-only", "-DLZR_WITH_LOCALIZER_HEADERS", "-include", "memory", "-include",
            "cstring", "-DHAVE_INLINE", f"-I{REF}/include", f"-I{REF}/include/eigen3", f"-I{REF}/source",
            f"-I{REF}/libsource/LNBTIFF", "-x", "c++", "-"]
@@ -61,7 +62,8 @@ def _read_output(path, nf):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("model,mode", [(0, "single"), (0, "threads"), (0, "batch"), (1, "batch"), (2, "single")])
+@pytest.mark.parametrize("model,mode", [(0, "single"), (0, "threads"), (0, "batch"), (1, "batch"), (2, "single"),
+                                        (3, "batch")])
 def test_dropin_matches_c_abi_and_oracle(tmp_path, model, mode):
     """The C++ layer returns exactly the C ABI's accepted fits (in particle order) and
     erases exactly the particles the reference erases for those statuses; the C ABI
@@ -79,8 +81,8 @@ def test_dropin_matches_c_abi_and_oracle(tmp_path, model, mode):
     got = _read_output(fout, frames.shape[0])
     with native.Context(model, w_psf) as ctx:
         capi = ctx.fit(frames, cands)
-    xi = 2 if model == 0 else 3
-    xcol = 3 if model == 0 else 4
+    xi = {0: 2, 3: 1}.get(model, 3)      # x0 in LzrFit.params
+    xcol = {0: 3, 3: 2}.get(model, 4)    # x column of getLocalizedPositionsAsMatrix
     for fi in range(frames.shape[0]):
         r = capi[cands["frame"] == fi]
         acc = r["status"] == 0

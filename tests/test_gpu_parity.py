@@ -97,16 +97,32 @@ def test_gpu_symmetric_psf_filter(plain_gpu_sample):
     assert set(np.unique(got["status"])) <= {0, 2, 3, 4, 5, 6, 7}
 
 
+def test_gpu_fixed_width_matches_oracle(plain_gpu_sample):
+    _say("case: fixed width 9x9, 200 frames 256x256, 10000 fits")
+    frames, _, cands = plain_gpu_sample
+    with native.Context(abi.MODEL_FIXED_WIDTH, 1.35, default_half_width=4) as ctx:
+        got = ctx.fit(frames, cands)
+    ref = po.fit_batch(frames, cands, abi.MODEL_FIXED_WIDTH, 1.35, default_half_width=4)
+    st = compare(ref, got, abi.MODEL_FIXED_WIDTH)
+    _say(format_stats("gpu vs oracle fixed width", st))
+    assert st["status_agreement"] >= 0.999, st
+    assert st["xy_frac_within_1e-3"] >= 0.999, st
+    for k in ("A", "b"):
+        assert st[f"{k}_frac_within_1e-4"] >= 0.995, (k, st)
+    assert set(np.unique(got["status"])) <= {0, 2, 3, 9}
+
+
 def test_gpu_mixed_models_and_windows():
-    _say("case: mixed models and windows h=2..7")
+    _say("case: mixed models (symmetric, ellipsoidal, fixed width) and windows h=2..7")
     w = synth.MIXED_2M.scaled(40)
     frames, _, cands = synth.generate(w)
+    cands["model"][::7] = abi.MODEL_FIXED_WIDTH
     with native.Context(abi.MODEL_SYMMETRIC, 1.5) as ctx:
         ctx.set_option(abi.OPT_MAX_HALF_WIDTH, 7)
         got = ctx.fit(frames, cands)
     ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.5)
     assert np.all(got["model"] == ref["model"])
-    for m in (abi.MODEL_SYMMETRIC, abi.MODEL_ELLIPSOIDAL):
+    for m in (abi.MODEL_SYMMETRIC, abi.MODEL_ELLIPSOIDAL, abi.MODEL_FIXED_WIDTH):
         sel = cands["model"] == m
         st = compare(ref[sel], got[sel], m)
         _say(format_stats(f"gpu vs oracle mixed model {m}", st))

@@ -44,6 +44,38 @@ def test_model_and_jacobian_match_reference_functions(gold):
     assert checked >= 50
 
 
+def test_fixed_width_model_is_reference_symmetric_model_at_fixed_width(gold):
+    """FitFunction_/Jacobian_FixedWidthGaussian (Localization.cpp:1050-1076, :1183-1225)
+    evaluate the symmetric functions at r = width and keep columns A, x0, y0, b: the
+    reference's own symmetric outputs pin them."""
+    checked = 0
+    for c, k, n, p, data, fref, Jref in _cases(gold):
+        if k != 0 or gold["status_f"][c] != 0:
+            continue
+        th = gold["theta"][c][:5]
+        t4 = th[[0, 2, 3, 4]]
+        st, f = po.model_fixed_width(t4, th[1], data, n, gold["xoff"][c], gold["yoff"][c], gold["sigma"][c])
+        stj, J = po.jacobian_fixed_width(t4, th[1], n, gold["xoff"][c], gold["yoff"][c], gold["sigma"][c])
+        assert st == 0 and stj == 0
+        np.testing.assert_allclose(f, fref, rtol=1e-12, atol=1e-12 * np.abs(fref).max())
+        Jsel = Jref[:, [0, 2, 3, 4]]
+        assert np.all(np.abs(J - Jsel) <= 1e-12 * np.abs(Jsel).max(axis=0) + 1e-300)
+        checked += 1
+    assert checked >= 20
+
+
+def test_fixed_width_fit_recovers_noiseless_spot():
+    """Known answer: a spot rendered exactly from the model at the fixed width is a zero
+    of the residual; LM from a displaced start must land on it."""
+    n, xoff, yoff, w = 9, 40.0, 70.0, 1.3
+    truth = np.array([250.0, 44.3, 73.6, 20.0])
+    data = po.model_fixed_width(truth, w, np.zeros(n * n), n, xoff, yoff)[1]  # model - 0
+    t0 = truth + np.array([-60.0, 0.4, -0.35, 5.0])
+    r = po.lm_run(3, data, n, xoff, yoff, t0, fixed_width=w)
+    assert r["lm_status"] == 0
+    np.testing.assert_allclose(r["theta"], truth, rtol=1e-7, atol=1e-6)
+
+
 def test_eigen_closed_form_matches_eigen(gold):
     for (sx, sy, rho), info, ref in zip(gold["eig_in"], gold["eig_info"], gold["eig_out"]):
         st, sd1, sd2, theta = po.eigen2(sx, sy, rho)
