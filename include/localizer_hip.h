@@ -157,4 +157,94 @@ LZR_EXPORT int lzr_fit_batch_device(LzrContext *ctx, const uint16_t *d_frames, i
 /* ABI version (LZR_ABI_VERSION). */
 LZR_EXPORT int lzr_abi_version(void);
 
+/* ---- LM start points --------------------------------------------------------------
+ * No reference counterpart: the reference always starts the LM at the particle
+ * (A0 = intensity, w, x, y, b0; source/Localization.cpp:138-142).  A start point
+ * replaces those values for one candidate; the window still follows the candidate's
+ * x, y, so the data fitted and every post-check are the reference's.
+ *   width >  0  start at (amplitude, width, x, y, background), absolute pixels
+ *               (ellipsoidal: sigma_x = sigma_y = width, rho = 0; fixed width: width unused)
+ *   width == 0  the reference start
+ *   width <  0  skip the candidate: its result record is left untouched */
+typedef struct LzrStart {
+    double amplitude;
+    double width;
+    double x;
+    double y;
+    double background;
+} LzrStart;
+
+/* lzr_fit_batch_device with one start point per candidate (d_starts may be NULL). */
+LZR_EXPORT int lzr_fit_batch_device_start(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny,
+                                          int nframes, const LzrCandidate *d_candidates,
+                                          const LzrStart *d_starts, int ncandidates, LzrFit *d_results,
+                                          void *stream);
+
+/* ---- learned initializer --------------------------------------------------------------
+ * No reference counterpart.  A small CNN predicts the LM start of the symmetric model
+ * from the 11 x 11 patch around each candidate (the window of half-width 5 placed by
+ * the reference rule x_offset = floor(x - 5)), normalized as (patch - median) / scale,
+ * scale = max(patch) - median (1 if smaller).  Network: conv 3x3 1->32 + ReLU (11->9),
+ * conv 3x3 32->32 + ReLU (9->7), flatten 1568 (index (y*7 + x)*32 + channel), FC 64 +
+ * ReLU, FC 5.  BF16 weights and activations, FP32 accumulation (MFMA), FP32 master
+ * weights and Adam state.  Outputs t = (A/scale, log(r/w), x0 - xc, y0 - yc,
+ * (b - median)/scale), (xc, yc) = window centre.
+ * Parameters (FP32, this order): conv1 W[32][3][3] b[32]; conv2 W[32 out][32 in][3][3]
+ * b[32]; fc1 W[64][1568] b[64]; fc2 W[5][64] b[5]. */
+typedef struct LzrNet LzrNet;
+#define LZR_NET_HALF_WIDTH 5
+#define LZR_NET_OUTPUTS 5
+#define LZR_NET_OFF_CONV1_W 0
+#define LZR_NET_OFF_CONV1_B 288
+#define LZR_NET_OFF_CONV2_W 320
+#define LZR_NET_OFF_CONV2_B 9536
+#define LZR_NET_OFF_FC1_W 9568
+#define LZR_NET_OFF_FC1_B 109920
+#define LZR_NET_OFF_FC2_W 109984
+#define LZR_NET_OFF_FC2_B 110304
+#define LZR_NET_NPARAMS 110309
+
+/* Creates a network on the current device, He-initialized from `seed`.  Returns 0 or -1. */
+LZR_EXPORT int lzr_net_create(LzrNet **net, double initial_psf_width, unsigned long long seed);
+LZR_EXPORT int lzr_net_destroy(LzrNet *net);
+LZR_EXPORT int lzr_net_last_error(const LzrNet *net, char *buf, int len);
+/* Host FP32 parameters in the order above; set_params also resets the Adam state. */
+LZR_EXPORT int lzr_net_set_params(LzrNet *net, const float *params, int count);
+LZR_EXPORT int lzr_net_get_params(LzrNet *net, float *params, int count);
+
+/* Raw network outputs t (ncandidates x 5 FP32, device) for the candidates' patches;
+ * rows of candidates whose patch leaves the frame are NaN. */
+LZR_EXPORT int lzr_net_forward_device(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                      const LzrCandidate *d_candidates, int ncandidates, float *d_outputs,
+                                      void *stream);
+
+/* Start points from the network (width = 0, the reference start, where the patch
+ * leaves the frame or the prediction is not finite). */
+LZR_EXPORT int lzr_net_predict_device(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                      const LzrCandidate *d_candidates, int ncandidates, LzrStart *d_starts,
+                                      void *stream);
+
+/* One training step on a batch: labels are LM fits of the same candidates (accepted
+ * symmetric fits are used, the others are ignored); loss = mean over used candidates of
+ * sum_k (t_k - target_k)^2.  apply = 1: Adam step with `learning_rate`; apply = 0: only
+ * gradients.  *loss (host, may be NULL) receives the batch loss and grads (host, may be
+ * NULL, LZR_NET_NPARAMS floats) the gradient; either makes the call synchronous. */
+LZR_EXPORT int lzr_net_train_step_device(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                         const LzrCandidate *d_candidates, const LzrFit *d_labels,
+                                         int ncandidates, float learning_rate, int apply, float *loss,
+                                         float *grads, void *stream);
+
+/* Net-initialized LM with fallback: starts from lzr_net_predict_device, then refits
+ * from the reference start every candidate whose net-start fit is a high-residual
+ * spot: normalized residual chi/sqrt(dof) above residual_factor * sqrt(max(b, 1))
+ * (b = fitted offset; sqrt(b) is the residual Poisson noise alone leaves).  With
+ * refit_rejected != 0, net-start fits that were rejected or did not converge
+ * (LZR_LM_SUCCESS) are refitted too.  d_work: ncandidates LzrStart of scratch;
+ * afterwards its width is 0 for refitted candidates and -1 for the others.
+ * Returns 0 or -1. */
+LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const uint16_t *d_frames, int nx,
+                                            int ny, int nframes, const LzrCandidate *d_candidates,
+                                            int ncandidates, LzrFit *d_results, LzrStart *d_work,
+                                            double residual_factor, int refit_rejected, void *stream);
+
 #endif /* LOCALIZER_HIP_H */

@@ -82,6 +82,27 @@ FIT_DTYPE = np.dtype(
 assert CANDIDATE_DTYPE.itemsize == 40
 assert FIT_DTYPE.itemsize == 192
 
+# LzrStart: an LM start point (width > 0), the reference start (0) or skip (< 0).
+START_DTYPE = np.dtype(
+    [("amplitude", "<f8"), ("width", "<f8"), ("x", "<f8"), ("y", "<f8"), ("background", "<f8")]
+)
+assert START_DTYPE.itemsize == 40
+
+# Learned initializer (include/localizer_hip.h, LZR_NET_*).
+NET_HALF_WIDTH = 5
+NET_OUTPUTS = 5
+NET_OFF = {
+    "conv1_w": (0, (32, 3, 3)),
+    "conv1_b": (288, (32,)),
+    "conv2_w": (320, (32, 32, 3, 3)),
+    "conv2_b": (9536, (32,)),
+    "fc1_w": (9568, (64, 1568)),
+    "fc1_b": (109920, (64,)),
+    "fc2_w": (109984, (5, 64)),
+    "fc2_b": (110304, (5,)),
+}
+NET_NPARAMS = 110309
+
 # Truth records written by the synthetic generator (localizer_amd/csrc/lzr_synth.h).
 EMITTER_DTYPE = np.dtype(
     [

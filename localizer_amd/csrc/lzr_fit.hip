@@ -42,6 +42,7 @@ struct KernelArgs {
     const uint16_t *frames;
     int nx, ny, nframes;
     const LzrCandidate *cands;
+    const LzrStart *starts; /* per-candidate LM start, or null     */
     const int *order;      /* queue position -> candidate index   */
     const unsigned *count; /* number of queue entries (device)    */
     unsigned *head;        /* queue head (device, zeroed per call) */
@@ -68,7 +69,9 @@ constexpr int kKeys = kClasses * kKeysPerClass;
 constexpr int kHist = 2 * kClasses;
 constexpr int kCounterWords = kHist + 2 * kKeys;
 
-__device__ __forceinline__ int sort_key(const LzrCandidate &c, int default_model) {
+/* -1: the candidate is skipped (start width < 0) and not queued. */
+__device__ __forceinline__ int sort_key(const LzrCandidate &c, const LzrStart *st, int default_model) {
+    if (st && st->width < 0.0) return -1;
     const int model = c.model < 0 ? default_model : c.model;
     const int cls = model_class(model);
     const float I = (float)c.intensity;
@@ -77,12 +80,16 @@ __device__ __forceinline__ int sort_key(const LzrCandidate &c, int default_model
     return cls * kKeysPerClass + b;
 }
 
-__global__ void histogram_kernel(const LzrCandidate *cands, int ncands, int default_model, unsigned *counters) {
+__global__ void histogram_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, int default_model,
+                                 unsigned *counters) {
     __shared__ unsigned h[kKeys];
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncands) atomicAdd(&h[sort_key(cands[i], default_model)], 1u);
+    if (i < ncands) {
+        const int key = sort_key(cands[i], starts ? starts + i : nullptr, default_model);
+        if (key >= 0) atomicAdd(&h[key], 1u);
+    }
     __syncthreads();
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
         if (h[t]) atomicAdd(&counters[kHist + t], h[t]);
@@ -108,23 +115,23 @@ __global__ void scan_kernel(unsigned *counters) {
     }
 }
 
-__global__ void scatter_kernel(const LzrCandidate *cands, int ncands, int default_model, unsigned *counters,
-                               int *order) {
+__global__ void scatter_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, int default_model,
+                               unsigned *counters, int *order) {
     __shared__ unsigned h[kKeys], base[kKeys];
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int key = 0;
+    int key = -1;
     unsigned local = 0;
     if (i < ncands) {
-        key = sort_key(cands[i], default_model);
-        local = atomicAdd(&h[key], 1u);
+        key = sort_key(cands[i], starts ? starts + i : nullptr, default_model);
+        if (key >= 0) local = atomicAdd(&h[key], 1u);
     }
     __syncthreads();
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
         if (h[t]) base[t] = atomicAdd(&counters[kHist + kKeys + t], h[t]);
     __syncthreads();
-    if (i < ncands) order[base[key] + local] = i;
+    if (key >= 0) order[base[key] + local] = i;
 }
 
 /* Turns the raw end states into the reference records (post-checks, covariance,
@@ -218,7 +225,10 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
                         const int idx = order[q];
                         const LzrCandidate c = args.cands[idx];
                         model = c.model >= 0 ? c.model : args.cfg.model;
-                        if (lane_start<P>(s, args.cfg, c, idx, args.nx, args.ny, args.nframes, model, args.out[idx]))
+                        LzrStart st;
+                        if (args.starts) st = args.starts[idx];
+                        if (lane_start<P>(s, args.cfg, c, idx, args.nx, args.ny, args.nframes, model, args.out[idx],
+                                          args.starts ? &st : nullptr))
                             fresh = true;
                         else
                             s.phase = kIdle;
@@ -399,15 +409,15 @@ typedef double AccT;           /* normal-equation accumulation type      */
 constexpr bool kF64Residual = true; /* current-point residual in FP64 */
 
 static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int nframes,
-                   const LzrCandidate *cands, int ncands, LzrFit *out, hipStream_t stream) {
+                   const LzrCandidate *cands, const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
     const int tpb = 256;
     const int nb = (ncands + tpb - 1) / tpb;
-    hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(tpb), 0, stream, cands, ncands, ctx->cfg.model,
+    hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, ctx->cfg.model,
                        ctx->d_counters);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kKeys), 0, stream, ctx->d_counters);
-    hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(tpb), 0, stream, cands, ncands, ctx->cfg.model,
+    hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, ctx->cfg.model,
                        ctx->d_counters, ctx->d_order);
     KernelArgs a;
     a.frames = frames;
@@ -415,6 +425,7 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.ny = ny;
     a.nframes = nframes;
     a.cands = cands;
+    a.starts = starts;
     a.order = ctx->d_order;
     a.count = ctx->d_counters;
     a.head = ctx->d_counters + kClasses;
@@ -576,7 +587,7 @@ LZR_EXPORT int lzr_fit_batch_device(LzrContext *ctx, const uint16_t *d_frames, i
     if (!d_frames || !d_cands || !d_out) return fail(ctx, "null device pointer");
     try {
         LZR_CHECK(hipSetDevice(ctx->device));
-        launch(ctx, d_frames, nx, ny, nframes, d_cands, ncands, d_out, (hipStream_t)stream);
+        launch(ctx, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_out, (hipStream_t)stream);
     } catch (const std::string &e) {
         return fail(ctx, e);
     }
@@ -600,7 +611,7 @@ LZR_EXPORT int lzr_fit_batch(LzrContext *ctx, const uint16_t *frames, int nx, in
         LZR_CHECK(hipMemcpyAsync(ctx->d_frames, frames, npx * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
         LZR_CHECK(hipMemcpyAsync(ctx->d_cands, cands, (size_t)ncands * sizeof(LzrCandidate), hipMemcpyHostToDevice,
                                  ctx->stream));
-        launch(ctx, ctx->d_frames, nx, ny, nframes, ctx->d_cands, ncands, ctx->d_out, ctx->stream);
+        launch(ctx, ctx->d_frames, nx, ny, nframes, ctx->d_cands, nullptr, ncands, ctx->d_out, ctx->stream);
         LZR_CHECK(hipMemcpyAsync(results, ctx->d_out, (size_t)ncands * sizeof(LzrFit), hipMemcpyDeviceToHost,
                                  ctx->stream));
         LZR_CHECK(hipStreamSynchronize(ctx->stream));
@@ -610,4 +621,81 @@ LZR_EXPORT int lzr_fit_batch(LzrContext *ctx, const uint16_t *frames, int nx, in
     int accepted = 0;
     for (int i = 0; i < ncands; ++i) accepted += results[i].status == LZR_ACCEPTED;
     return accepted;
+}
+
+LZR_EXPORT int lzr_fit_batch_device_start(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                          const LzrCandidate *d_cands, const LzrStart *d_starts, int ncands,
+                                          LzrFit *d_out, void *stream) {
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+    if (ncands == 0) return 0;
+    if (!d_frames || !d_cands || !d_out) return fail(ctx, "null device pointer");
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        launch(ctx, d_frames, nx, ny, nframes, d_cands, d_starts, ncands, d_out, (hipStream_t)stream);
+    } catch (const std::string &e) {
+        return fail(ctx, e);
+    }
+    return 0;
+}
+
+namespace lzr {
+
+/* Marks the candidates whose net-start fit is refitted from the reference start:
+ * width 0 (refit) when the net start was used (width > 0) and the fit left a
+ * normalized residual chi/sqrt(dof) above factor*sqrt(max(b,1)) (or, with
+ * refit_rejected, was rejected or did not converge); width -1 (keep) otherwise. */
+__global__ void mark_fallback_kernel(const LzrCandidate *cands, const LzrFit *fits, LzrStart *starts, int n,
+                                     LaneConfig cfg, double factor, int refit_rejected) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const LzrFit &f = fits[i];
+    const LzrCandidate c = cands[i];
+    bool refit = false;
+    if (starts[i].width > 0.0) {
+        const bool failed = f.status != LZR_ACCEPTED || f.lm_status != LZR_LM_SUCCESS;
+        refit = refit_rejected && failed;
+        if (!refit && f.status == LZR_ACCEPTED) {
+            const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
+            const int p = f.model == LZR_MODEL_SYMMETRIC ? 5 : (f.model == LZR_MODEL_FIXED_WIDTH ? 4 : 7);
+            const double dof = (double)((2 * h - 1) * (2 * h - 1) - p);
+            const double b = f.params[p - 1];
+            const double norm = dof > 0.0 ? f.chi / sqrt(dof) : 0.0;
+            refit = !(norm <= factor * sqrt(b > 1.0 ? b : 1.0));
+        }
+    }
+    starts[i].width = refit ? 0.0 : -1.0;
+}
+
+}  // namespace lzr
+
+LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const uint16_t *d_frames, int nx, int ny,
+                                            int nframes, const LzrCandidate *d_cands, int ncands, LzrFit *d_out,
+                                            LzrStart *d_work, double residual_factor, int refit_rejected,
+                                            void *stream) {
+    if (!ctx) return -1;
+    if (!net) return fail(ctx, "null network");
+    if (!d_work && ncands > 0) return fail(ctx, "null work buffer");
+    if (lzr_net_predict_device(net, d_frames, nx, ny, nframes, d_cands, ncands, d_work, stream) != 0) {
+        char buf[512];
+        lzr_net_last_error(net, buf, (int)sizeof buf);
+        return fail(ctx, std::string("network: ") + buf);
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+    if (ncands == 0) return 0;
+    if (!d_frames || !d_cands || !d_out) return fail(ctx, "null device pointer");
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        hipStream_t s = (hipStream_t)stream;
+        launch(ctx, d_frames, nx, ny, nframes, d_cands, d_work, ncands, d_out, s);
+        hipLaunchKernelGGL(mark_fallback_kernel, dim3((ncands + 255) / 256), dim3(256), 0, s, d_cands, d_out, d_work,
+                           ncands, ctx->cfg, residual_factor, refit_rejected);
+        launch(ctx, d_frames, nx, ny, nframes, d_cands, d_work, ncands, d_out, s);
+        LZR_CHECK(hipGetLastError());
+    } catch (const std::string &e) {
+        return fail(ctx, e);
+    }
+    return 0;
 }

@@ -739,9 +739,10 @@ LZR_HD LZR_INLINE void clear_record(LzrFit &o) {
 /* Prepares a lane for a new candidate.  Returns false when the candidate is rejected
  * before fitting (window outside the frame, invalid request); `out` then holds the
  * final record. */
+/* st (may be null): an LM start point replacing the reference start when st->width > 0. */
 template <int P>
 LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const LzrCandidate &c, int roi, int nx,
-                                  int ny, int nframes, int model, LzrFit &out) {
+                                  int ny, int nframes, int model, LzrFit &out, const LzrStart *st = nullptr) {
     s.roi = roi;
     const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
     if (h < 1 || h > cfg.max_half_width || c.frame < 0 || c.frame >= nframes) {
@@ -786,6 +787,29 @@ LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const 
         s.th[4] = c.y - yoff;
         s.th[5] = 0.0;
         s.th[6] = c.background;
+    }
+    if (st && st->width > 0.0) {
+        const double sx = st->x - xoff, sy = st->y - yoff;
+        if constexpr (P == 4) {
+            s.th[0] = st->amplitude;
+            s.th[1] = sx;
+            s.th[2] = sy;
+            s.th[3] = st->background;
+        } else if constexpr (P == 5) {
+            s.th[0] = st->amplitude;
+            s.th[1] = st->width;
+            s.th[2] = sx;
+            s.th[3] = sy;
+            s.th[4] = st->background;
+        } else {
+            s.th[0] = st->amplitude;
+            s.th[1] = st->width;
+            s.th[2] = st->width;
+            s.th[3] = sx;
+            s.th[4] = sy;
+            s.th[5] = 0.0;
+            s.th[6] = st->background;
+        }
     }
 #pragma unroll
     for (int i = 0; i < P; ++i) s.dx[i] = 0.0;

@@ -22,6 +22,16 @@ EXPORTED = (
     "lzr_fit_batch",
     "lzr_fit_batch_device",
     "lzr_abi_version",
+    "lzr_fit_batch_device_start",
+    "lzr_fit_batch_device_netinit",
+    "lzr_net_create",
+    "lzr_net_destroy",
+    "lzr_net_last_error",
+    "lzr_net_set_params",
+    "lzr_net_get_params",
+    "lzr_net_forward_device",
+    "lzr_net_predict_device",
+    "lzr_net_train_step_device",
 )
 
 
@@ -57,6 +67,20 @@ def lib():
         L.lzr_fit_batch_device.restype = ctypes.c_int
         L.lzr_abi_version.argtypes = []
         L.lzr_abi_version.restype = ctypes.c_int
+        i, d, f = ctypes.c_int, ctypes.c_double, ctypes.c_float
+        L.lzr_fit_batch_device_start.argtypes = [vp, vp, i, i, i, vp, vp, i, vp, vp]
+        L.lzr_fit_batch_device_netinit.argtypes = [vp, vp, vp, i, i, i, vp, i, vp, vp, d, i, vp]
+        L.lzr_net_create.argtypes = [ctypes.POINTER(vp), d, ctypes.c_ulonglong]
+        L.lzr_net_destroy.argtypes = [vp]
+        L.lzr_net_last_error.argtypes = [vp, ctypes.c_char_p, i]
+        L.lzr_net_set_params.argtypes = [vp, vp, i]
+        L.lzr_net_get_params.argtypes = [vp, vp, i]
+        L.lzr_net_forward_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
+        L.lzr_net_predict_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
+        L.lzr_net_train_step_device.argtypes = [vp, vp, i, i, i, vp, vp, i, f, i, ctypes.POINTER(f), vp, vp]
+        for name in EXPORTED:
+            if name.startswith(("lzr_net", "lzr_fit_batch_device_")):
+                getattr(L, name).restype = ctypes.c_int
         if L.lzr_abi_version() != 1:
             raise LocalizerError("ABI version mismatch")
         _LIB = L
@@ -78,13 +102,14 @@ class Context:
                             int(default_half_width)) != 0:
             raise LocalizerError(_error(None))
         self._h = h
+        self._destroy = lib().lzr_destroy  # held so that closing works during interpreter shutdown
         self.model = int(model)
         self.initial_psf_width = float(initial_psf_width)
         self.sigma = float(sigma)
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().lzr_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     __del__ = close
@@ -113,6 +138,27 @@ class Context:
             raise LocalizerError(_error(self._h))
         return out
 
+    def fit_device_start(self, d_frames, nx, ny, nframes, d_cands, d_starts, ncands, d_out, stream=0):
+        """lzr_fit_batch_device_start: per-candidate LM start points (d_starts may be 0)."""
+        rc = lib().lzr_fit_batch_device_start(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                              ctypes.c_void_p(d_cands), ctypes.c_void_p(d_starts), int(ncands),
+                                              ctypes.c_void_p(d_out), ctypes.c_void_p(stream))
+        if rc < 0:
+            raise LocalizerError(_error(self._h))
+        return rc
+
+    def fit_device_netinit(self, net, d_frames, nx, ny, nframes, d_cands, ncands, d_out, d_work,
+                           residual_factor=3.0, refit_rejected=False, stream=0):
+        """lzr_fit_batch_device_netinit: network start, fallback to the reference start."""
+        rc = lib().lzr_fit_batch_device_netinit(self._h, net._h, ctypes.c_void_p(d_frames), int(nx), int(ny),
+                                                int(nframes), ctypes.c_void_p(d_cands), int(ncands),
+                                                ctypes.c_void_p(d_out), ctypes.c_void_p(d_work),
+                                                float(residual_factor), int(bool(refit_rejected)),
+                                                ctypes.c_void_p(stream))
+        if rc < 0:
+            raise LocalizerError(_error(self._h))
+        return rc
+
     def fit_device(self, d_frames, nx, ny, nframes, d_cands, ncands, d_out, stream=0):
         """Device pointers in (ints), enqueued on `stream` (lzr_fit_batch_device)."""
         rc = lib().lzr_fit_batch_device(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
@@ -121,3 +167,70 @@ class Context:
         if rc < 0:
             raise LocalizerError(_error(self._h))
         return rc
+
+
+def _net_error(h):
+    buf = ctypes.create_string_buffer(1024)
+    lib().lzr_net_last_error(h, buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+class Net:
+    """The learned initializer (lzr_net_*): a CNN on 11 x 11 patches whose forward and
+    backward passes run on the GPU."""
+
+    def __init__(self, initial_psf_width, seed=0x10CA112E):
+        h = ctypes.c_void_p()
+        if lib().lzr_net_create(ctypes.byref(h), float(initial_psf_width), int(seed)) != 0:
+            raise LocalizerError(_net_error(None))
+        self._h = h
+        self._destroy = lib().lzr_net_destroy
+        self.initial_psf_width = float(initial_psf_width)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LocalizerError(_net_error(self._h))
+
+    def get_params(self):
+        p = np.zeros(abi.NET_NPARAMS, np.float32)
+        self._check(lib().lzr_net_get_params(self._h, p.ctypes.data, p.size))
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(lib().lzr_net_set_params(self._h, p.ctypes.data, p.size))
+
+    def forward_device(self, d_frames, nx, ny, nframes, d_cands, ncands, d_out, stream=0):
+        self._check(lib().lzr_net_forward_device(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                                 ctypes.c_void_p(d_cands), int(ncands), ctypes.c_void_p(d_out),
+                                                 ctypes.c_void_p(stream)))
+
+    def predict_device(self, d_frames, nx, ny, nframes, d_cands, ncands, d_starts, stream=0):
+        self._check(lib().lzr_net_predict_device(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                                 ctypes.c_void_p(d_cands), int(ncands), ctypes.c_void_p(d_starts),
+                                                 ctypes.c_void_p(stream)))
+
+    def train_step_device(self, d_frames, nx, ny, nframes, d_cands, d_labels, ncands, lr, apply=True,
+                          want_loss=True, want_grads=False, stream=0):
+        """One step; returns (loss or None, grads or None)."""
+        loss = ctypes.c_float(0.0)
+        grads = np.zeros(abi.NET_NPARAMS, np.float32) if want_grads else None
+        self._check(lib().lzr_net_train_step_device(
+            self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes), ctypes.c_void_p(d_cands),
+            ctypes.c_void_p(d_labels), int(ncands), float(lr), int(bool(apply)),
+            ctypes.byref(loss) if want_loss else None, ctypes.c_void_p(grads.ctypes.data) if want_grads else None,
+            ctypes.c_void_p(stream)))
+        return (loss.value if want_loss else None), grads

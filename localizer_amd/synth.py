@@ -149,7 +149,26 @@ DENSE_2048 = Workload(
     pair_distance_sigmas=2.0,
 )
 
-WORKLOADS = {w.name: w for w in (PLAIN_CPU_512, PLAIN_GPU_256, MIXED_2M, DENSE_2048)}
+# Learned initializer: independent 11x11 ROIs (h = 5), photons log-uniform 100-2000,
+# background uniform 50-200, sub-pixel offset uniform +-1 px from the window centre,
+# sigma uniform 1.1-1.8; 4M training ROIs and 4M fresh evaluation ROIs (generate_rois
+# with the training and evaluation streams).  nframes = ROI count.
+LEARNED_INIT_11 = Workload(
+    name="net-init-lm-11x11-4M-rois",
+    config_id=4,
+    nx=11,
+    ny=11,
+    nframes=4000000,
+    emitters_per_frame=1,
+    sigma=(1.1, 1.8),
+    photons=(100.0, 2000.0),
+    background=(50.0, 200.0),
+    half_width=(5, 5),
+    initial_psf_width=1.45,
+    extra={"max_offset": 1.0, "train_stream": 4, "eval_stream": 44},
+)
+
+WORKLOADS = {w.name: w for w in (PLAIN_CPU_512, PLAIN_GPU_256, MIXED_2M, LEARNED_INIT_11, DENSE_2048)}
 
 
 def _lib_path():
