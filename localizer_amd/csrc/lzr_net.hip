@@ -11,17 +11,20 @@
  *   t  = W_fc2 h + b_fc2                        5             FP32
  *
  * MFMA tiling.  v_mfma_f32_16x16x32_bf16: lane l holds A[m = l%16][k-slot (l/16, i)],
- * B[k-slot (l/16, i)][n = l%16] (i < 8) and C[m = 4(l/16) + i][n = l%16] (i < 4); the
- * k-slot (g, i) is k = 8g + i.  One K-chunk of 32 is one 3x3 tap over the 32 input
- * channels (conv2), or one 7x7 position of the flattened map (fc1), so every A operand
- * is a single 16-byte LDS read of 8 consecutive channels.  conv1 (K = 9 taps) uses
- * v_mfma_f32_16x16x16bf16_1k with the taps in k-slots 0..8.
+ * B[k-slot (l/16, i)][n = l%16] (i < 8) and C[m = 4(l/16) + i][n = l%16] (i < 4).  Any
+ * bijection of k-slots to K indices works as long as A and B use the same one.  The
+ * forward pass computes every layer transposed (features x patches): weights are the A
+ * operand, activations the B operand.  One K-chunk of 32 is one 3x3 tap over the 32
+ * input channels (conv2) or one 7x7 position of the flattened map (fc1).  conv1 (K = 9
+ * taps) uses v_mfma_f32_16x16x16bf16_1k with the taps in k-slots 0..8.
  *
- * Forward kernel: persistent, one 7-wave block per CU, 16 patches per group.  A
- * group's M-tiles are "one position x 16 patches", so a conv2 output tile at position
- * q is the fc1 A operand of K-chunk q after a 1 KB LDS transpose: the 7x7x32 map is
- * never stored (inference), and fc1 accumulates while conv2 runs.  Weights packed in
- * MFMA operand order (conv2: 72 VGPRs per lane) are loaded once per wave.
+ * Forward kernel: persistent, one 7-wave block per CU, 16 patches per group (the MFMA
+ * N dimension).  conv1 writes a1 to LDS (8-byte stores of 4 channels); conv2 at
+ * position q reads 8 channels of one patch per tap (16-byte LDS reads), and the 8
+ * outputs a lane holds are its fc1 B operand for K-chunk q, so the 7x7x32 map never
+ * leaves registers (inference) and fc1 accumulates while conv2 runs.  Weights packed
+ * in MFMA operand order (conv2: 72 VGPRs per lane) are loaded once per wave; fc1's are
+ * prefetched one position ahead.
  *
  * Backward (training): activations are stored transposed ([feature][patch], patch
  * fastest) so that every weight-gradient GEMM reads contiguous 16-byte K-slices
@@ -66,7 +69,6 @@ constexpr int kGroup = 16;             /* patches per forward group (one MFMA M-
 constexpr int kWaves = 7; /* 49 conv2 positions = 7 per wave */
 constexpr int kThreads = 64 * kWaves;
 constexpr int kA1Stride = kP1 * kC + 8; /* bf16 per patch in LDS (padded) */
-constexpr int kTbStride = kC + 8;
 constexpr int kHsStride = kHid + 8;
 
 constexpr int OFF_W1 = LZR_NET_OFF_CONV1_W, OFF_B1 = LZR_NET_OFF_CONV1_B;
@@ -79,12 +81,12 @@ static_assert(OFF_WF1 == OFF_B2 + 32 && OFF_BF1 == OFF_WF1 + 64 * kF && OFF_WF2 
 static_assert(OFF_BF2 == OFF_WF2 + 5 * 64 && NPARAMS == OFF_BF2 + 5, "layout");
 
 /* Packed BF16 operand buffers (element counts). */
-constexpr int PK_W1 = 2 * 64 * 4;         /* [ntile 2][lane][4]            conv1 B   */
-constexpr int PK_W2 = 9 * 2 * 64 * 8;     /* [tap][ntile 2][lane][8]       conv2 B   */
+constexpr int PK_W1 = 2 * 64 * 4;         /* [tile 2][lane][4]             conv1^T A */
+constexpr int PK_W2 = 9 * 2 * 64 * 8;     /* [tap][tile 2][lane][8]        conv2^T A */
 constexpr int PK_W2T = 9 * 2 * 64 * 8;    /* [tap][ntile 2][lane][8]       conv2^T B */
-constexpr int PK_WF1 = 49 * 4 * 64 * 8;   /* [q][ntile 4][lane][8]         fc1 B     */
+constexpr int PK_WF1 = 49 * 4 * 64 * 8;   /* [q][mtile 4][lane][8]         fc1^T A   */
 constexpr int PK_WF1T = 98 * 2 * 64 * 8;  /* [ntile 98][kchunk 2][lane][8] fc1^T B   */
-constexpr int PK_WF2 = 2 * 64 * 8;        /* [kchunk 2][lane][8]           fc2 B     */
+constexpr int PK_WF2 = 2 * 64 * 8;        /* [kchunk 2][lane][8]           fc2^T A   */
 constexpr int PK_TOTAL = PK_W1 + PK_W2 + PK_W2T + PK_WF1 + PK_WF1T + PK_WF2;
 
 /* ---- BF16 helpers ---------------------------------------------------------------- */
@@ -243,35 +245,63 @@ __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int n
 
 /* ---- fused forward ---------------------------------------------------------------- */
 
+/* Two FP32 -> packed BF16 (round to nearest even; v_cvt_pk_bf16_f32). */
+__device__ inline uint32_t cvt_pk(float a, float b) {
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    const f32x2_t v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+
+/* Every layer is computed transposed, C[feature][patch] = W * X^T: the weights are the
+ * A operand (registers / L2) and the activations the B operand (LDS), so a lane's four
+ * outputs are four consecutive features of one patch (one 8-byte LDS store), and the
+ * eight conv2 outputs a lane holds at position q are directly its fc1 B operand for
+ * K-chunk q, with the k-slot (g, i) of that chunk mapped to channel
+ * o(g, i) = (i < 4 ? 4g + i : 16 + 4g + i - 4) (the fc1 weights are packed the same way). */
 template <bool kTrain>
 __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, int n, Packed W, float *out,
-                                                      TrainStore T) {
+                                                           TrainStore T) {
     __shared__ __attribute__((aligned(16))) uint16_t xs[kGroup * kXRow];
     __shared__ __attribute__((aligned(16))) uint16_t a1s[kGroup * kA1Stride];
-    __shared__ __attribute__((aligned(16))) uint16_t tb[kWaves * kGroup * kTbStride];
     __shared__ __attribute__((aligned(16))) uint16_t hs[kGroup * kHsStride];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lg = lane >> 4;
 
-    /* weights in MFMA operand order, once per wave */
+    /* weights in MFMA A-operand order, once per wave */
     s16x4_t w1r[2];
 #pragma unroll
-    for (int nt = 0; nt < 2; ++nt) {
-        const uint16_t *p = W.w1 + (nt * 64 + lane) * 4;
+    for (int mt = 0; mt < 2; ++mt) {
+        const uint16_t *p = W.w1 + (mt * 64 + lane) * 4;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) w1r[nt][i] = (short)p[i];
+        for (int i = 0; i < 4; ++i) w1r[mt][i] = (short)p[i];
     }
     u32x4_t w2r[9][2];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) w2r[t][nt] = ld16(W.w2 + ((t * 2 + nt) * 64 + lane) * 8);
+        for (int mt = 0; mt < 2; ++mt) w2r[t][mt] = ld16(W.w2 + ((t * 2 + mt) * 64 + lane) * 8);
     u32x4_t wf2r[2];
 #pragma unroll
     for (int kc = 0; kc < 2; ++kc) wf2r[kc] = ld16(W.wf2 + (kc * 64 + lane) * 8);
-    const float b1v[2] = {W.params[OFF_B1 + lr], W.params[OFF_B1 + 16 + lr]};
-    const float b2v[2] = {W.params[OFF_B2 + lr], W.params[OFF_B2 + 16 + lr]};
-    const float bf2v = lr < kOut ? W.params[OFF_BF2 + lr] : 0.f;
+    /* biases of this lane's output rows (features mt*16 + 4*lg + i) */
+    f32x4_t b1v[2], b2v[2], bf2v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            b1v[mt][i] = W.params[OFF_B1 + mt * 16 + 4 * lg + i];
+            b2v[mt][i] = W.params[OFF_B2 + mt * 16 + 4 * lg + i];
+        }
+        bf2v[i] = 4 * lg + i < kOut ? W.params[OFF_BF2 + 4 * lg + i] : 0.f;
+    }
+    /* conv1 taps of this lane's k-slots 4*lg + i */
+    int toff[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int k = 4 * lg + i;
+        toff[i] = k < 9 ? (k / 3) * kN + k % 3 : 0;
+    }
 
     const int ngroups = (n + kGroup - 1) / kGroup;
     for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
@@ -292,105 +322,102 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
         }
         __syncthreads();
 
-        /* phase 1: conv1, one position (16 patches x 32 channels) per step */
+        /* phase 1: conv1 at position p: a1^T[32 ch][16 patches] */
         for (int p = wave; p < kP1; p += kWaves) {
             const int py = p / kS1, px = p - py * kS1;
-            s16x4_t a;
+            const uint16_t *xr = xs + lr * kXRow + py * kN + px;
+            s16x4_t bx;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int k = 4 * lg + i;
-                a[i] = k < 9 ? (short)xs[lr * kXRow + (py + k / 3) * kN + px + k % 3] : (short)0;
-            }
+            for (int i = 0; i < 4; ++i) bx[i] = 4 * lg + i < 9 ? (short)xr[toff[i]] : (short)0;
 #pragma unroll
-            for (int nt = 0; nt < 2; ++nt) {
-                const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
-                const f32x4_t c = mfma16(a, w1r[nt], z);
-                const int ch = nt * 16 + lr;
-                uint16_t o[4];
+            for (int mt = 0; mt < 2; ++mt) {
+                const f32x4_t c = mfma16(w1r[mt], bx, b1v[mt]);
+                const float v0 = fmaxf(c[0], 0.f), v1 = fmaxf(c[1], 0.f), v2 = fmaxf(c[2], 0.f), v3 = fmaxf(c[3], 0.f);
+                u32x2_t pk;
+                pk[0] = cvt_pk(v0, v1);
+                pk[1] = cvt_pk(v2, v3);
+                *reinterpret_cast<u32x2_t *>(a1s + lr * kA1Stride + p * kC + mt * 16 + 4 * lg) = pk;
+                if (kTrain) {
+                    const uint16_t o[4] = {(uint16_t)(pk[0] & 0xffffu), (uint16_t)(pk[0] >> 16),
+                                           (uint16_t)(pk[1] & 0xffffu), (uint16_t)(pk[1] >> 16)};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    o[i] = f2bf(fmaxf(c[i] + b1v[nt], 0.f));
-                    a1s[(4 * lg + i) * kA1Stride + p * kC + ch] = o[i];
+                    for (int i = 0; i < 4; ++i)
+                        T.a1T[(size_t)(p * kC + mt * 16 + 4 * lg + i) * T.ldb + roi0 + lr] = o[i];
                 }
-                if (kTrain) st8(T.a1T + (size_t)(p * kC + ch) * T.ldb + roi0 + 4 * lg, o[0], o[1], o[2], o[3]);
             }
         }
         __syncthreads();
 
-        /* phase 2: conv2 at position q, then fc1 K-chunk q */
+        /* phase 2: conv2 at position q (registers) -> fc1 K-chunk q */
         f32x4_t hacc[4];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) hacc[nt] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-        uint16_t *tbw = tb + wave * kGroup * kTbStride;
-        /* fc1 weights of this wave's next position, loaded one position ahead */
-        u32x4_t wf[4];
+        for (int mt = 0; mt < 4; ++mt) hacc[mt] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        u32x4_t wf[4]; /* fc1 weights of this wave's next position, one position ahead */
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) wf[nt] = ld16(W.wf1 + ((wave * 4 + nt) * 64 + lane) * 8);
+        for (int mt = 0; mt < 4; ++mt) wf[mt] = ld16(W.wf1 + ((wave * 4 + mt) * 64 + lane) * 8);
         for (int q = wave; q < kP2; q += kWaves) {
             const int qy = q / kS2, qx = q - qy * kS2;
-            f32x4_t c0 = {b2v[0], b2v[0], b2v[0], b2v[0]};
-            f32x4_t c1 = {b2v[1], b2v[1], b2v[1], b2v[1]};
+            f32x4_t c0 = b2v[0], c1 = b2v[1];
+            const uint16_t *ar = a1s + lr * kA1Stride + lg * 8;
 #pragma unroll
             for (int t = 0; t < 9; ++t) {
                 const int pos = (qy + t / 3) * kS1 + qx + t % 3;
-                const u32x4_t a = ld16(a1s + lr * kA1Stride + pos * kC + lg * 8);
-                c0 = mfma32(a, w2r[t][0], c0);
-                c1 = mfma32(a, w2r[t][1], c1);
+                const u32x4_t b = ld16(ar + pos * kC);
+                c0 = mfma32(w2r[t][0], b, c0);
+                c1 = mfma32(w2r[t][1], b, c1);
             }
-            uint16_t o0[4], o1[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                o0[i] = f2bf(fmaxf(c0[i], 0.f));
-                o1[i] = f2bf(fmaxf(c1[i], 0.f));
-                tbw[(4 * lg + i) * kTbStride + lr] = o0[i];
-                tbw[(4 * lg + i) * kTbStride + 16 + lr] = o1[i];
-            }
+            u32x4_t bq;
+            bq[0] = cvt_pk(fmaxf(c0[0], 0.f), fmaxf(c0[1], 0.f));
+            bq[1] = cvt_pk(fmaxf(c0[2], 0.f), fmaxf(c0[3], 0.f));
+            bq[2] = cvt_pk(fmaxf(c1[0], 0.f), fmaxf(c1[1], 0.f));
+            bq[3] = cvt_pk(fmaxf(c1[2], 0.f), fmaxf(c1[3], 0.f));
             if (kTrain) {
-                st8(T.a2T + (size_t)(q * kC + lr) * T.ldb + roi0 + 4 * lg, o0[0], o0[1], o0[2], o0[3]);
-                st8(T.a2T + (size_t)(q * kC + 16 + lr) * T.ldb + roi0 + 4 * lg, o1[0], o1[1], o1[2], o1[3]);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int o = i < 4 ? 4 * lg + i : 16 + 4 * lg + (i - 4);
+                    const uint16_t v = (uint16_t)((bq[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+                    T.a2T[(size_t)(q * kC + o) * T.ldb + roi0 + lr] = v;
+                }
             }
-            wave_sync();
-            const u32x4_t a2 = ld16(tbw + lr * kTbStride + lg * 8);
             u32x4_t cur[4];
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) cur[nt] = wf[nt];
+            for (int mt = 0; mt < 4; ++mt) cur[mt] = wf[mt];
             if (q + kWaves < kP2) {
 #pragma unroll
-                for (int nt = 0; nt < 4; ++nt) wf[nt] = ld16(W.wf1 + (((q + kWaves) * 4 + nt) * 64 + lane) * 8);
+                for (int mt = 0; mt < 4; ++mt) wf[mt] = ld16(W.wf1 + (((q + kWaves) * 4 + mt) * 64 + lane) * 8);
             }
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) hacc[nt] = mfma32(a2, cur[nt], hacc[nt]);
-            wave_sync();
+            for (int mt = 0; mt < 4; ++mt) hacc[mt] = mfma32(cur[mt], bq, hacc[mt]);
         }
         __syncthreads();
-        float *red = reinterpret_cast<float *>(a1s); /* [waves][16][64], a1 no longer needed */
+        float *red = reinterpret_cast<float *>(a1s); /* [waves][64 j][16 patches], a1 no longer needed */
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
+        for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) red[(wave * kGroup + 4 * lg + i) * kHid + nt * 16 + lr] = hacc[nt][i];
+            for (int i = 0; i < 4; ++i) red[(wave * kHid + mt * 16 + 4 * lg + i) * kGroup + lr] = hacc[mt][i];
         __syncthreads();
 
         /* phase 3: h = relu(sum over waves + b) */
         for (int e = tid; e < kGroup * kHid; e += kThreads) {
-            const int r = e >> 6, j = e & 63;
+            const int r = e & (kGroup - 1), j = e >> 4;
             float s = W.params[OFF_BF1 + j];
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) s += red[(w * kGroup + r) * kHid + j];
+            for (int w = 0; w < kWaves; ++w) s += red[(w * kHid + j) * kGroup + r];
             const uint16_t hb = f2bf(fmaxf(s, 0.f));
             hs[r * kHsStride + j] = hb;
             if (kTrain) T.h[(size_t)(roi0 + r) * kHid + j] = hb;
         }
         __syncthreads();
 
-        /* phase 4: fc2 */
+        /* phase 4: fc2, out^T[5][16 patches] */
         if (wave == 0) {
-            f32x4_t o = {bf2v, bf2v, bf2v, bf2v};
+            f32x4_t o = bf2v;
 #pragma unroll
-            for (int kc = 0; kc < 2; ++kc) o = mfma32(ld16(hs + lr * kHsStride + kc * 32 + lg * 8), wf2r[kc], o);
+            for (int kc = 0; kc < 2; ++kc) o = mfma32(wf2r[kc], ld16(hs + lr * kHsStride + kc * 32 + lg * 8), o);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int r = 4 * lg + i;
-                if (lr < kOut && roi0 + r < n) out[(size_t)(roi0 + r) * kOut + lr] = o[i];
+                const int t = 4 * lg + i;
+                if (t < kOut && roi0 + lr < n) out[(size_t)(roi0 + lr) * kOut + t] = o[i];
             }
         }
         __syncthreads();
@@ -714,8 +741,11 @@ __global__ void pack_kernel(const float *P, uint16_t *pk, float *wf2f) {
         const int o = 8 * (lane >> 4) + i, c = nt * 16 + (lane & 15);
         v = P[OFF_W2 + o * 288 + c * 9 + t];
     } else if ((e -= PK_W2T) < PK_WF1) {
-        const int q = e / 2048, nt = (e / 512) % 4, lane = (e / 8) % 64, i = e % 8;
-        const int k = q * 32 + 8 * (lane >> 4) + i, j = nt * 16 + (lane & 15);
+        /* A operand of the transposed fc1: row j, k-slot (g, i) -> channel o(g, i) of chunk q */
+        const int q = e / 2048, mt = (e / 512) % 4, lane = (e / 8) % 64, i = e % 8;
+        const int g = lane >> 4;
+        const int o = i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4);
+        const int k = q * 32 + o, j = mt * 16 + (lane & 15);
         v = P[OFF_WF1 + j * kF + k];
     } else if ((e -= PK_WF1) < PK_WF1T) {
         const int nt = e / 1024, kc = (e / 512) % 2, lane = (e / 8) % 64, i = e % 8;

@@ -112,13 +112,15 @@ def test_gpu_start_points_drive_the_lm(rois):
 
 
 def test_gpu_trained_network_cuts_lm_work():
-    """Train on the GPU, then compare net-start LM (with fallback) against the plain LM
-    on fresh ROIs: fewer trial evaluations, same fits to 1e-3 px."""
-    _say("case: train 131072 ROIs (64 steps x 2048), evaluate 20000 fresh ROIs")
+    """Train on the GPU, then compare net-start LM (residual fallback) against the plain
+    LM on fresh ROIs: fewer trial evaluations, the same fits to 1e-3 px where both
+    accept.  (The LM's convergence is linear -- the reference's width column is the
+    true derivative / sqrt(2) -- so a better start saves a logarithmic number of steps.)"""
+    _say("case: train 1048576 ROIs (512 steps x 2048), evaluate 20000 fresh ROIs")
     with native.Net(W.initial_psf_width, seed=3) as net:
-        stats = netinit.train(net, W, 131072, epochs=1.0, batch=2048, lr=2e-3, log=_say)
+        stats = netinit.train(net, W, 1 << 20, epochs=1.0, batch=2048, lr=2e-3, log=None)
         _say(f"train: {stats['steps']} steps in {stats['train_s']:.2f} s, losses {stats['losses']}")
-        assert stats["losses"][-1][1] < 0.5 * stats["losses"][0][1]
+        assert stats["losses"][-1][1] < 0.3 * stats["losses"][0][1]
         frames, _, cands = netinit.simulate(W, 20000, W.extra["eval_stream"])
         n = len(cands)
         d_frames, d_cands = device.upload(frames), device.upload(cands)
@@ -132,17 +134,23 @@ def test_gpu_trained_network_cuts_lm_work():
             ctx.fit_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_plain.ptr, s)
             net.predict_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_starts.ptr, s)
             ctx.fit_device_start(d_frames.ptr, 11, 11, n, d_cands.ptr, d_starts.ptr, n, d_first.ptr, s)
-            ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_net.ptr, d_work.ptr, 3.0, s)
+            ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_net.ptr, d_work.ptr, 3.0,
+                                   stream=s)
         plain, first, final, work = d_plain.numpy(), d_first.numpy(), d_net.numpy(), d_work.numpy()
     refit = work["width"] == 0.0
+    assert np.all((work["width"] == 0.0) | (work["width"] == -1.0))
+    # the net-init result is the net-start fit, refitted from the reference start where flagged
+    keep = ~refit
+    assert np.array_equal(final["params"][keep], first["params"][keep])
     t_plain = plain["trials"].mean()
     t_net = (first["trials"].sum() + final["trials"][refit].sum()) / n
-    both = (plain["status"] == 0) & (final["status"] == 0)
+    acc = plain["status"] == 0
+    both = acc & (final["status"] == 0)
     dxy = np.abs(final["params"][both][:, 2:4] - plain["params"][both][:, 2:4]).max(axis=1)
-    agree = np.mean(plain["status"] == final["status"])
-    _say(f"trials per ROI: plain {t_plain:.2f}, net-init {t_net:.2f} (first pass {first['trials'].mean():.2f}, "
-         f"fallback {refit.mean():.4f}); status agreement {agree:.5f}; xy within 1e-3: {np.mean(dxy <= 1e-3):.5f}, "
-         f"max {dxy.max():.3g}")
-    assert t_net < 0.8 * t_plain
-    assert agree >= 0.995
-    assert np.mean(dxy <= 1e-3) >= 0.99
+    _say(f"trials per ROI: plain {t_plain:.2f} (accepted {plain['trials'][acc].mean():.2f}), net-init {t_net:.2f} "
+         f"(accepted {first['trials'][acc].mean():.2f}), fallback {refit.mean():.4f}; accepted plain {acc.mean():.4f} "
+         f"net-init {np.mean(final['status'] == 0):.4f}; xy within 1e-3 where both accept: {np.mean(dxy <= 1e-3):.5f}")
+    assert first["trials"][acc].mean() < 0.8 * plain["trials"][acc].mean()
+    assert t_net < 0.85 * t_plain
+    assert np.sum(acc & (final["status"] != 0)) <= 0.01 * acc.sum()
+    assert np.mean(dxy <= 1e-3) >= 0.98
