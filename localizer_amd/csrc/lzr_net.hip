@@ -18,7 +18,7 @@
  * input channels (conv2) or one 7x7 position of the flattened map (fc1).  conv1 (K = 9
  * taps) uses v_mfma_f32_16x16x16bf16_1k with the taps in k-slots 0..8.
  *
- * Forward kernel: persistent, one 7-wave block per CU, 16 patches per group (the MFMA
+ * Forward kernel: persistent, one 8-wave block per CU, 16 patches per group (the MFMA
  * N dimension).  conv1 writes a1 to LDS (8-byte stores of 4 channels); conv2 at
  * position q reads 8 channels of one patch per tap (16-byte LDS reads), and the 8
  * outputs a lane holds are its fc1 B operand for K-chunk q, so the 7x7x32 map never
@@ -66,7 +66,8 @@ constexpr int kF = kP2 * kC;           /* 1568 */
 constexpr int kHid = 64;
 constexpr int kOut = LZR_NET_OUTPUTS;  /* 5 */
 constexpr int kGroup = 16;             /* patches per forward group (one MFMA M-tile) */
-constexpr int kWaves = 7; /* 49 conv2 positions = 7 per wave */
+constexpr int kWaves = 8; /* 2 per SIMD */
+constexpr int kXsStride = kXRow + 8; /* LDS row of a patch: 272 B, 16 patches hit 16 banks */
 constexpr int kThreads = 64 * kWaves;
 constexpr int kA1Stride = kP1 * kC + 8; /* bf16 per patch in LDS (padded) */
 constexpr int kHsStride = kHid + 8;
@@ -262,7 +263,7 @@ __device__ inline uint32_t cvt_pk(float a, float b) {
 template <bool kTrain>
 __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, int n, Packed W, float *out,
                                                            TrainStore T) {
-    __shared__ __attribute__((aligned(16))) uint16_t xs[kGroup * kXRow];
+    __shared__ __attribute__((aligned(16))) uint16_t xs[kGroup * kXsStride];
     __shared__ __attribute__((aligned(16))) uint16_t a1s[kGroup * kA1Stride];
     __shared__ __attribute__((aligned(16))) uint16_t hs[kGroup * kHsStride];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -304,14 +305,19 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
     }
 
     const int ngroups = (n + kGroup - 1) / kGroup;
+    /* thread tid < 256 moves 16 bytes of the group's input; the next group's are loaded
+     * while conv2 runs */
+    const int xr_ = tid >> 4, xc_ = tid & 15;
+    u32x4_t xnext = {0u, 0u, 0u, 0u};
+    if (tid < kGroup * 16 && (int)blockIdx.x * kGroup + xr_ < n)
+        xnext = ld16(xin + (size_t)(blockIdx.x * kGroup + xr_) * kXRow + xc_ * 8);
     for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
         const int roi0 = g * kGroup;
         /* phase 0: the group's normalized patches */
         if (tid < kGroup * 16) {
-            const int r = tid >> 4, c = tid & 15;
-            u32x4_t v = {0u, 0u, 0u, 0u};
-            if (roi0 + r < n) v = ld16(xin + (size_t)(roi0 + r) * kXRow + c * 8);
-            *reinterpret_cast<u32x4_t *>(xs + r * kXRow + c * 8) = v;
+            const int r = xr_, c = xc_;
+            const u32x4_t v = xnext;
+            *reinterpret_cast<u32x4_t *>(xs + r * kXsStride + c * 8) = v;
             if (kTrain) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -322,27 +328,41 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
         }
         __syncthreads();
 
-        /* phase 1: conv1 at position p: a1^T[32 ch][16 patches] */
-        for (int p = wave; p < kP1; p += kWaves) {
-            const int py = p / kS1, px = p - py * kS1;
-            const uint16_t *xr = xs + lr * kXRow + py * kN + px;
-            s16x4_t bx;
+        /* phase 1: conv1 at position p: a1^T[32 ch][16 patches]; two positions per step so
+         * that the LDS reads of the second overlap the first's MFMAs */
+        for (int p0 = wave; p0 < kP1; p0 += 2 * kWaves) {
+            s16x4_t bx[2];
+            int pp[2];
+            pp[0] = p0;
+            pp[1] = p0 + kWaves < kP1 ? p0 + kWaves : -1; /* uniform across the wave */
 #pragma unroll
-            for (int i = 0; i < 4; ++i) bx[i] = 4 * lg + i < 9 ? (short)xr[toff[i]] : (short)0;
+            for (int u = 0; u < 2; ++u) {
+                const int p = pp[u] < 0 ? p0 : pp[u];
+                const int py = p / kS1, px = p - py * kS1;
+                const uint16_t *xr = xs + lr * kXsStride + py * kN + px;
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const f32x4_t c = mfma16(w1r[mt], bx, b1v[mt]);
-                const float v0 = fmaxf(c[0], 0.f), v1 = fmaxf(c[1], 0.f), v2 = fmaxf(c[2], 0.f), v3 = fmaxf(c[3], 0.f);
-                u32x2_t pk;
-                pk[0] = cvt_pk(v0, v1);
-                pk[1] = cvt_pk(v2, v3);
-                *reinterpret_cast<u32x2_t *>(a1s + lr * kA1Stride + p * kC + mt * 16 + 4 * lg) = pk;
-                if (kTrain) {
-                    const uint16_t o[4] = {(uint16_t)(pk[0] & 0xffffu), (uint16_t)(pk[0] >> 16),
-                                           (uint16_t)(pk[1] & 0xffffu), (uint16_t)(pk[1] >> 16)};
+                for (int i = 0; i < 4; ++i) bx[u][i] = 4 * lg + i < 9 ? (short)xr[toff[i]] : (short)0;
+            }
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        T.a1T[(size_t)(p * kC + mt * 16 + 4 * lg + i) * T.ldb + roi0 + lr] = o[i];
+            for (int u = 0; u < 2; ++u) {
+                if (pp[u] < 0) continue;
+                const int p = pp[u];
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    const f32x4_t c = mfma16(w1r[mt], bx[u], b1v[mt]);
+                    const float v0 = fmaxf(c[0], 0.f), v1 = fmaxf(c[1], 0.f), v2 = fmaxf(c[2], 0.f),
+                                v3 = fmaxf(c[3], 0.f);
+                    u32x2_t pk;
+                    pk[0] = cvt_pk(v0, v1);
+                    pk[1] = cvt_pk(v2, v3);
+                    *reinterpret_cast<u32x2_t *>(a1s + lr * kA1Stride + p * kC + mt * 16 + 4 * lg) = pk;
+                    if (kTrain) {
+                        const uint16_t o[4] = {(uint16_t)(pk[0] & 0xffffu), (uint16_t)(pk[0] >> 16),
+                                               (uint16_t)(pk[1] & 0xffffu), (uint16_t)(pk[1] >> 16)};
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            T.a1T[(size_t)(p * kC + mt * 16 + 4 * lg + i) * T.ldb + roi0 + lr] = o[i];
+                    }
                 }
             }
         }
@@ -352,6 +372,11 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
         f32x4_t hacc[4];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) hacc[mt] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+        if (tid < kGroup * 16) {
+            const int gn = g + (int)gridDim.x;
+            xnext = (u32x4_t){0u, 0u, 0u, 0u};
+            if (gn < ngroups && gn * kGroup + xr_ < n) xnext = ld16(xin + (size_t)(gn * kGroup + xr_) * kXRow + xc_ * 8);
+        }
         u32x4_t wf[4]; /* fc1 weights of this wave's next position, one position ahead */
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) wf[mt] = ld16(W.wf1 + ((wave * 4 + mt) * 64 + lane) * 8);
