@@ -10,6 +10,12 @@ One step = one fit of all 500k candidates with frames and candidates resident in
 Prints one JSON line.  The CPU baseline is the FP64 oracle (frame-parallel thread pool,
 one frame per task, as the reference's PALMAnalysisController) on a bounded sample of
 the same workload.
+
+The line's "net_init" object is the second half of the metric, on the learned-initializer
+workload: a CNN trained on the GPU (4M simulated 11x11 ROIs labelled by the plain LM)
+predicts LM start points for 4M fresh ROIs; plain LM and net-initialized LM (with the
+residual fallback) are timed on the same ROIs, and net-initialized localizations are
+counted only where the plain LM also accepts and x, y agree within 1e-3 px.
 """
 import argparse
 import json
@@ -22,7 +28,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from localizer_amd import abi, native, synth  # noqa: E402
+from localizer_amd import abi, device, native, netinit, synth  # noqa: E402
 
 METRIC = "localizations/sec on 1 MI355X: plain LM vs net-initialized LM at 1e-3 px"
 PEAK_FP32_TFLOPS = 157.3
@@ -82,6 +88,94 @@ def cpu_baseline(w, frames, cands, target_seconds, nthreads):
     }
 
 
+NET_FLOPS_PER_ROI = 2.0 * (81 * 32 * 9 + 49 * 32 * 288 + 1568 * 64 + 64 * 5)
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log):
+    """Plain vs net-initialized LM on the learned-initializer workload (one GPU)."""
+    import torch
+
+    W = synth.LEARNED_INIT_11
+    net = native.Net(W.initial_psf_width, seed=3)
+    tr = netinit.train(net, W, train_rois, epochs=epochs, batch=2048, lr=2e-3)
+    log(f"net: trained on {train_rois} ROIs, {tr['steps']} steps in {tr['train_s']:.2f} s "
+        f"(simulation {tr['simulate_s']:.1f} s), last loss {tr['losses'][-1][1]:.4f}")
+    frames, _, cands = netinit.simulate(W, eval_rois, W.extra["eval_stream"])
+    n = len(cands)
+    d_frames, d_cands = device.upload(frames), device.upload(cands)
+    d_plain = device.empty(abi.FIT_DTYPE, n)
+    d_net = device.empty(abi.FIT_DTYPE, n)
+    d_work = device.empty(abi.START_DTYPE, n)
+    d_starts = device.empty(abi.START_DTYPE, n)
+    ctx = native.Context(abi.MODEL_SYMMETRIC, W.initial_psf_width, 1.0, abi.NET_HALF_WIDTH)
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+
+    def plain():
+        ctx.fit_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_plain.ptr, s)
+
+    def netinit_fit():
+        ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_net.ptr, d_work.ptr, 3.0, stream=s)
+
+    def predict():
+        net.predict_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_starts.ptr, s)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    ms_plain = timed(plain)
+    ms_net = timed(netinit_fit)
+    ms_pred = timed(predict)
+    P, F, wk = d_plain.numpy(), d_net.numpy(), d_work.numpy()
+    acc = P["status"] == 0
+    both = acc & (F["status"] == 0)
+    dxy = np.abs(F["params"][:, 2:4] - P["params"][:, 2:4]).max(axis=1)
+    good = both & (dxy <= 1e-3)
+    refit = wk["width"] == 0.0
+    out = {
+        "workload": W.name,
+        "rois": n,
+        "window": "11x11",
+        "initial_psf_width": W.initial_psf_width,
+        "plain_value": float(acc.sum() / (ms_plain * 1e-3)),
+        "value": float(good.sum() / (ms_net * 1e-3)),
+        "speedup": float(good.sum() / (ms_net * 1e-3)) / float(acc.sum() / (ms_plain * 1e-3)),
+        "ms_plain": ms_plain,
+        "ms_netinit": ms_net,
+        "ms_predict": ms_pred,
+        "accepted_plain": int(acc.sum()),
+        "accepted_netinit": int(np.sum(F["status"] == 0)),
+        "counted_netinit": int(good.sum()),
+        "xy_within_1e-3": float(np.mean(dxy[both] <= 1e-3)),
+        "status_agreement": float(np.mean(F["status"] == P["status"])),
+        "fallback_fraction": float(refit.mean()),
+        "trials_plain": float(P["trials"].mean()),
+        "trials_netinit_final": float(F["trials"].mean()),
+        "train": {"rois": train_rois, "steps": tr["steps"], "batch": tr["batch"], "seconds": tr["train_s"],
+                  "last_loss": tr["losses"][-1][1]},
+        "network": {"bound": "mfma", "achieved": NET_FLOPS_PER_ROI * n / (ms_pred * 1e-3) / 1e12,
+                    "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                    "frac": NET_FLOPS_PER_ROI * n / (ms_pred * 1e-3) / 1e12 / PEAK_BF16_TFLOPS,
+                    "note": "algorithmic FLOPs of the CNN forward / duration of lzr_net_predict_device "
+                            "(patch prep + forward + decode)"},
+    }
+    log(f"net-init: plain {ms_plain:.2f} ms ({out['plain_value']:.4g} loc/s), net-init {ms_net:.2f} ms "
+        f"({out['value']:.4g} loc/s counted), predict {ms_pred:.2f} ms, speedup {out['speedup']:.3f}")
+    ctx.close()
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +188,11 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent grid size per CU (0: resident max)")
     ap.add_argument("--rounds", type=int, default=0, help="fitting-kernel rounds (fits parked in between)")
     ap.add_argument("--park-threshold", type=int, default=-1, help="busy lanes at which a wave parks its fits")
+    ap.add_argument("--no-net", action="store_true", help="skip the net-initialized LM leg")
+    ap.add_argument("--only-net", action="store_true", help="run only the net-initialized LM leg (profiling)")
+    ap.add_argument("--net-train-rois", type=int, default=4 << 20)
+    ap.add_argument("--net-eval-rois", type=int, default=4000000)
+    ap.add_argument("--net-epochs", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -104,6 +203,11 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.only_net:
+        net = net_init_leg(args.net_train_rois, args.net_eval_rois, args.net_epochs, max(1, args.steps),
+                           args.warmup, lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps({"net_init": net}), flush=True)
+        return
     w = synth.PLAIN_GPU_256
     if args.frames:
         w = w.scaled(args.frames)
@@ -159,6 +263,13 @@ def main():
           f"{achieved:.2f} TFLOP/s algorithmic, mean trials {fits['trials'].mean():.2f}",
           file=sys.stderr, flush=True)
 
+    net = None
+    if not args.no_net:
+        del d_frames, d_cands, d_out
+        torch.cuda.empty_cache()
+        net = net_init_leg(args.net_train_rois, args.net_eval_rois, args.net_epochs, max(1, min(args.steps, 5)),
+                           1, lambda m: print(m, file=sys.stderr, flush=True))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, frames, cands, args.cpu_seconds, nthreads)
@@ -201,6 +312,7 @@ def main():
             "gather_gbs": nbytes / (kernel_ms * 1e-3) / 1e9,
         },
         "cpu_baseline": cpu,
+        "net_init": net,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
