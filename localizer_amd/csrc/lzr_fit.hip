@@ -484,7 +484,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 4 * kClasses * sizeof(unsigned)));
     } catch (const std::string &e) {
         g_create_error = "lzr_create: " + e;
-        delete ctx;
+        lzr_destroy(ctx); /* frees whatever was created */
         return -1;
     }
     ctx->cfg.initial_psf_width = initial_psf_width;
