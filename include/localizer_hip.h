@@ -157,6 +157,34 @@ LZR_EXPORT int lzr_fit_batch_device(LzrContext *ctx, const uint16_t *d_frames, i
 /* ABI version (LZR_ABI_VERSION). */
 LZR_EXPORT int lzr_abi_version(void);
 
+/* Reads an LZR_OPT_* option (e.g. the configuration lzr_autotune_device chose).
+ * Returns 0 or -1.  (No reference counterpart.) */
+LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value);
+
+/* ---- frame-block graphs ----------------------------------------------------------------
+ * No reference counterpart (the reference fits frame by frame on host threads,
+ * source/PALMAnalysis.cpp:196-280).  A graph is one lzr_fit_batch_device call captured
+ * once on fixed device buffers (frames, candidates, results) and replayed per frame
+ * block: the caller refills the buffers and *d_count (device int, the block's candidate
+ * count, at most max_candidates), then launches the graph.  Options set on the context
+ * after capture do not affect the graph.  A context's graphs and calls share its work
+ * buffers: enqueue them on one stream (or order them) -- overlapping replays race. */
+typedef struct LzrGraph LzrGraph;
+LZR_EXPORT int lzr_graph_create(LzrContext *ctx, LzrGraph **graph, const uint16_t *d_frames, int nx, int ny,
+                                int nframes, const LzrCandidate *d_candidates, const int *d_count,
+                                int max_candidates, LzrFit *d_results);
+LZR_EXPORT int lzr_graph_launch(LzrGraph *graph, void *stream);
+LZR_EXPORT int lzr_graph_destroy(LzrGraph *graph);
+
+/* Autotuner: times the launch configurations (register budget LZR_OPT_OCCUPANCY,
+ * persistent grid LZR_OPT_WAVES_PER_CU, continuation rounds LZR_OPT_ROUNDS) on this
+ * batch -- each fits the whole batch once into d_results -- and keeps the fastest in the
+ * context.  *best_ms (may be NULL) receives its time.  Synchronous.  Returns 0 or -1.
+ * (No reference counterpart.) */
+LZR_EXPORT int lzr_autotune_device(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                   const LzrCandidate *d_candidates, int ncandidates, LzrFit *d_results,
+                                   void *stream, double *best_ms);
+
 /* ---- LM start points --------------------------------------------------------------
  * No reference counterpart: the reference always starts the LM at the particle
  * (A0 = intensity, w, x, y, b0; source/Localization.cpp:138-142).  A start point

@@ -80,8 +80,17 @@ __device__ __forceinline__ int sort_key(const LzrCandidate &c, const LzrStart *s
     return cls * kKeysPerClass + b;
 }
 
-__global__ void histogram_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, int default_model,
-                                 unsigned *counters) {
+/* Candidate count of a launch: the host value, or (graph replays) a device integer
+ * clamped to the capacity the graph was built for. */
+__device__ __forceinline__ int live_count(const int *d_count, int ncands) {
+    if (!d_count) return ncands;
+    const int c = *d_count;
+    return c < 0 ? 0 : (c < ncands ? c : ncands);
+}
+
+__global__ void histogram_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, const int *d_count,
+                                 int default_model, unsigned *counters) {
+    ncands = live_count(d_count, ncands);
     __shared__ unsigned h[kKeys];
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
     __syncthreads();
@@ -115,8 +124,9 @@ __global__ void scan_kernel(unsigned *counters) {
     }
 }
 
-__global__ void scatter_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, int default_model,
-                               unsigned *counters, int *order) {
+__global__ void scatter_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, const int *d_count,
+                               int default_model, unsigned *counters, int *order) {
+    ncands = live_count(d_count, ncands);
     __shared__ unsigned h[kKeys], base[kKeys];
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
     __syncthreads();
@@ -136,9 +146,10 @@ __global__ void scatter_kernel(const LzrCandidate *cands, const LzrStart *starts
 
 /* Turns the raw end states into the reference records (post-checks, covariance,
  * 2x2 eigen-decomposition), one thread per candidate. */
-__global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, LzrFit *out, LaneConfig cfg) {
+__global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, const int *d_count, LzrFit *out,
+                                LaneConfig cfg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncands) return;
+    if (i >= live_count(d_count, ncands)) return;
     finish_any(out[i], cfg, cands[i]);
 }
 
@@ -376,10 +387,11 @@ static int grid_for(LzrContext *ctx, size_t lds) {
     return per_cu * ctx->num_cus;
 }
 
+/* Parking buffers of a class (used when rounds > 1). */
 template <int P, typename Acc, bool F64, int W>
-static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
-    const int grid = grid_for<P, Acc, F64, W>(ctx, lds);
-    const size_t cap = (size_t)grid * 64;
+static void ensure_park(LzrContext *ctx, int cls, size_t lds) {
+    if (ctx->rounds <= 1) return;
+    const size_t cap = (size_t)grid_for<P, Acc, F64, W>(ctx, lds) * 64;
     for (int b = 0; b < 2; ++b) {
         if (ctx->park_cap[cls][b] < cap) {
             if (ctx->d_park[cls][b]) LZR_CHECK(hipFree(ctx->d_park[cls][b]));
@@ -389,6 +401,12 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
             ctx->park_cap[cls][b] = cap;
         }
     }
+}
+
+template <int P, typename Acc, bool F64, int W>
+static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
+    const int grid = grid_for<P, Acc, F64, W>(ctx, lds);
+    ensure_park<P, Acc, F64, W>(ctx, cls, lds);
     unsigned *ctr = ctx->d_park_ctr + 4 * cls; /* buffer b: count ctr[2b], head ctr[2b+1] */
     for (int round = 0; round < ctx->rounds; ++round) {
         const int ob = round & 1, ib = ob ^ 1;
@@ -408,17 +426,38 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
 typedef double AccT;           /* normal-equation accumulation type      */
 constexpr bool kF64Residual = true; /* current-point residual in FP64 */
 
+static size_t launch_lds(const LzrContext *ctx) {
+    const int nmax = 2 * ctx->cfg.max_half_width + 1;
+    return (size_t)nmax * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * sizeof(uint16_t);
+}
+
+/* Allocates everything a launch of up to ncands candidates needs, so that the launch
+ * itself allocates nothing (stream capture). */
+static void prepare(LzrContext *ctx, int ncands) {
+    ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
+    const size_t lds = launch_lds(ctx);
+    if (ctx->occupancy5 == 1)
+        ensure_park<5, AccT, kF64Residual, 1>(ctx, 0, lds);
+    else
+        ensure_park<5, AccT, kF64Residual, 2>(ctx, 0, lds);
+    ensure_park<7, AccT, kF64Residual, 1>(ctx, 1, lds);
+    ensure_park<4, AccT, kF64Residual, 2>(ctx, 2, lds);
+}
+
+/* Enqueues one fit of ncands candidates (d_count, when not null, holds the live count
+ * on the device, at most ncands).  Allocates nothing once prepare() covered ncands. */
 static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int nframes,
-                   const LzrCandidate *cands, const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream) {
+                   const LzrCandidate *cands, const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream,
+                   const int *d_count = nullptr) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
     const int tpb = 256;
     const int nb = (ncands + tpb - 1) / tpb;
-    hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, ctx->cfg.model,
-                       ctx->d_counters);
+    hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, d_count,
+                       ctx->cfg.model, ctx->d_counters);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kKeys), 0, stream, ctx->d_counters);
-    hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, ctx->cfg.model,
-                       ctx->d_counters, ctx->d_order);
+    hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, d_count,
+                       ctx->cfg.model, ctx->d_counters, ctx->d_order);
     KernelArgs a;
     a.frames = frames;
     a.nx = nx;
@@ -431,16 +470,15 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.head = ctx->d_counters + kClasses;
     a.out = out;
     a.cfg = ctx->cfg;
-    const int nmax = 2 * ctx->cfg.max_half_width + 1;
-    const size_t lds = (size_t)nmax * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * sizeof(uint16_t);
+    const size_t lds = launch_lds(ctx);
     if (ctx->occupancy5 == 1)
         run_class<5, AccT, kF64Residual, 1>(ctx, a, 0, lds, stream);
     else
         run_class<5, AccT, kF64Residual, 2>(ctx, a, 0, lds, stream);
     run_class<7, AccT, kF64Residual, 1>(ctx, a, 1, lds, stream);
     run_class<4, AccT, kF64Residual, 2>(ctx, a, 2, lds, stream);
-    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, out,
-                       ctx->cfg);
+    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, d_count,
+                       out, ctx->cfg);
     LZR_CHECK(hipGetLastError());
 }
 
@@ -696,6 +734,155 @@ LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const 
         LZR_CHECK(hipGetLastError());
     } catch (const std::string &e) {
         return fail(ctx, e);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Options, frame-block graphs, autotuning                                               */
+/* ------------------------------------------------------------------------------------ */
+
+LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value) {
+    if (!ctx || !value) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    switch (option) {
+    case LZR_OPT_MAX_ITERATIONS: *value = ctx->cfg.max_iterations; return 0;
+    case LZR_OPT_EPSABS: *value = ctx->cfg.epsabs; return 0;
+    case LZR_OPT_EPSREL: *value = ctx->cfg.epsrel; return 0;
+    case LZR_OPT_DIAG_UPDATE_ROWS: *value = ctx->cfg.diag_update_rows; return 0;
+    case LZR_OPT_WAVES_PER_CU: *value = ctx->waves_per_cu; return 0;
+    case LZR_OPT_MAX_HALF_WIDTH: *value = ctx->cfg.max_half_width; return 0;
+    case LZR_OPT_OCCUPANCY: *value = ctx->occupancy5; return 0;
+    case LZR_OPT_ROUNDS: *value = ctx->rounds; return 0;
+    case LZR_OPT_PARK_THRESHOLD: *value = ctx->park_threshold; return 0;
+    default: return fail(ctx, "unknown option");
+    }
+}
+
+struct LzrGraph {
+    LzrContext *ctx = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+LZR_EXPORT int lzr_graph_destroy(LzrGraph *g) {
+    if (!g) return -1;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->capture) (void)hipStreamDestroy(g->capture);
+    delete g;
+    return 0;
+}
+
+LZR_EXPORT int lzr_graph_create(LzrContext *ctx, LzrGraph **out, const uint16_t *d_frames, int nx, int ny,
+                                int nframes, const LzrCandidate *d_cands, const int *d_count, int max_candidates,
+                                LzrFit *d_results) {
+    if (!ctx) return -1;
+    if (!out) return fail(ctx, "null output pointer");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, max_candidates)) return -1;
+    if (max_candidates < 1) return fail(ctx, "a graph needs a positive candidate capacity");
+    if (!d_frames || !d_cands || !d_count || !d_results) return fail(ctx, "null device pointer");
+    LzrGraph *g = new LzrGraph();
+    g->ctx = ctx;
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        prepare(ctx, max_candidates);
+        LZR_CHECK(hipDeviceSynchronize());
+        LZR_CHECK(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
+        LZR_CHECK(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
+        try {
+            launch(ctx, d_frames, nx, ny, nframes, d_cands, nullptr, max_candidates, d_results, g->capture, d_count);
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(g->capture, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
+        }
+        LZR_CHECK(hipStreamEndCapture(g->capture, &g->graph));
+        LZR_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (const std::string &e) {
+        lzr_graph_destroy(g);
+        return fail(ctx, "lzr_graph_create: " + e);
+    }
+    *out = g;
+    return 0;
+}
+
+LZR_EXPORT int lzr_graph_launch(LzrGraph *g, void *stream) {
+    if (!g) return -1;
+    LzrContext *ctx = g->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        LZR_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    } catch (const std::string &e) {
+        return fail(ctx, e);
+    }
+    return 0;
+}
+
+/* One candidate launch configuration of the autotuner. */
+struct TuneConfig {
+    int occupancy, waves_per_cu, rounds, park_threshold;
+};
+
+LZR_EXPORT int lzr_autotune_device(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                   const LzrCandidate *d_cands, int ncands, LzrFit *d_results, void *stream,
+                                   double *best_ms) {
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+    if (ncands == 0) return 0;
+    if (!d_frames || !d_cands || !d_results) return fail(ctx, "null device pointer");
+    const TuneConfig saved{ctx->occupancy5, ctx->waves_per_cu, ctx->rounds, ctx->park_threshold};
+    std::vector<TuneConfig> cand;
+    for (int occ : {1, 2})
+        for (int wpc : {0, 2})
+            for (int rounds : {1, 2}) cand.push_back({occ, wpc, rounds, rounds > 1 ? 16 : saved.park_threshold});
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        hipStream_t s = (hipStream_t)stream;
+        LZR_CHECK(hipEventCreate(&e0));
+        LZR_CHECK(hipEventCreate(&e1));
+        float best = 1e30f;
+        TuneConfig pick = saved;
+        for (size_t i = 0; i <= cand.size(); ++i) {
+            /* index 0 is a warm-up of the current configuration */
+            const TuneConfig c = i == 0 ? saved : cand[i - 1];
+            ctx->occupancy5 = c.occupancy;
+            ctx->waves_per_cu = c.waves_per_cu;
+            ctx->rounds = c.rounds;
+            ctx->park_threshold = c.park_threshold;
+            LZR_CHECK(hipEventRecord(e0, s));
+            launch(ctx, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_results, s);
+            LZR_CHECK(hipEventRecord(e1, s));
+            LZR_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            LZR_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (i > 0 && ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+        ctx->occupancy5 = pick.occupancy;
+        ctx->waves_per_cu = pick.waves_per_cu;
+        ctx->rounds = pick.rounds;
+        ctx->park_threshold = pick.park_threshold;
+        if (best_ms) *best_ms = best;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } catch (const std::string &e) {
+        ctx->occupancy5 = saved.occupancy;
+        ctx->waves_per_cu = saved.waves_per_cu;
+        ctx->rounds = saved.rounds;
+        ctx->park_threshold = saved.park_threshold;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        return fail(ctx, "lzr_autotune_device: " + e);
     }
     return 0;
 }

@@ -32,6 +32,11 @@ EXPORTED = (
     "lzr_net_forward_device",
     "lzr_net_predict_device",
     "lzr_net_train_step_device",
+    "lzr_get_option",
+    "lzr_graph_create",
+    "lzr_graph_launch",
+    "lzr_graph_destroy",
+    "lzr_autotune_device",
 )
 
 
@@ -78,8 +83,13 @@ def lib():
         L.lzr_net_forward_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
         L.lzr_net_predict_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
         L.lzr_net_train_step_device.argtypes = [vp, vp, i, i, i, vp, vp, i, f, i, ctypes.POINTER(f), vp, vp]
+        L.lzr_get_option.argtypes = [vp, i, ctypes.POINTER(d)]
+        L.lzr_graph_create.argtypes = [vp, ctypes.POINTER(vp), vp, i, i, i, vp, vp, i, vp]
+        L.lzr_graph_launch.argtypes = [vp, vp]
+        L.lzr_graph_destroy.argtypes = [vp]
+        L.lzr_autotune_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp, ctypes.POINTER(d)]
         for name in EXPORTED:
-            if name.startswith(("lzr_net", "lzr_fit_batch_device_")):
+            if name.startswith(("lzr_net", "lzr_fit_batch_device_", "lzr_graph", "lzr_get", "lzr_autotune")):
                 getattr(L, name).restype = ctypes.c_int
         if L.lzr_abi_version() != 1:
             raise LocalizerError("ABI version mismatch")
@@ -159,6 +169,26 @@ class Context:
             raise LocalizerError(_error(self._h))
         return rc
 
+    def get_option(self, option):
+        v = ctypes.c_double()
+        if lib().lzr_get_option(self._h, int(option), ctypes.byref(v)) != 0:
+            raise LocalizerError(_error(self._h))
+        return v.value
+
+    def autotune_device(self, d_frames, nx, ny, nframes, d_cands, ncands, d_out, stream=0):
+        """lzr_autotune_device: returns the best time (ms); the context keeps that configuration."""
+        best = ctypes.c_double()
+        rc = lib().lzr_autotune_device(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                       ctypes.c_void_p(d_cands), int(ncands), ctypes.c_void_p(d_out),
+                                       ctypes.c_void_p(stream), ctypes.byref(best))
+        if rc != 0:
+            raise LocalizerError(_error(self._h))
+        return best.value
+
+    def graph(self, d_frames, nx, ny, nframes, d_cands, d_count, max_candidates, d_out):
+        """A frame-block graph (lzr_graph_create) on fixed device buffers."""
+        return Graph(self, d_frames, nx, ny, nframes, d_cands, d_count, max_candidates, d_out)
+
     def fit_device(self, d_frames, nx, ny, nframes, d_cands, ncands, d_out, stream=0):
         """Device pointers in (ints), enqueued on `stream` (lzr_fit_batch_device)."""
         rc = lib().lzr_fit_batch_device(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
@@ -167,6 +197,31 @@ class Context:
         if rc < 0:
             raise LocalizerError(_error(self._h))
         return rc
+
+
+class Graph:
+    """lzr_graph_*: one captured fit replayed per frame block."""
+
+    def __init__(self, ctx, d_frames, nx, ny, nframes, d_cands, d_count, max_candidates, d_out):
+        h = ctypes.c_void_p()
+        if lib().lzr_graph_create(ctx._h, ctypes.byref(h), ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                  ctypes.c_void_p(d_cands), ctypes.c_void_p(d_count), int(max_candidates),
+                                  ctypes.c_void_p(d_out)) != 0:
+            raise LocalizerError(_error(ctx._h))
+        self._h = h
+        self._ctx = ctx
+        self._destroy = lib().lzr_graph_destroy
+
+    def launch(self, stream=0):
+        if lib().lzr_graph_launch(self._h, ctypes.c_void_p(stream)) != 0:
+            raise LocalizerError(_error(self._ctx._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._destroy(self._h)
+            self._h = None
+
+    __del__ = close
 
 
 def _net_error(h):
