@@ -176,6 +176,112 @@ def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log):
     return out
 
 
+def mixed_leg(steps, warmup, log, frames_per_block=1000):
+    """Batched uneven workload: 2M ROIs, windows 5x5..15x15, 60 % symmetric / 40 %
+    ellipsoidal; one call over all ROIs (default and autotuned launch configuration) and
+    frame blocks replayed through one captured HIP graph."""
+    import torch
+
+    w = synth.MIXED_2M
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    frames, _, cands = synth.generate(w, nthreads=nthreads)
+    log(f"mixed: generated {w.nframes} frames, {cands.size} ROIs in {time.perf_counter() - t0:.1f} s")
+    n = cands.size
+    d_frames, d_cands = device.upload(frames), device.upload(cands)
+    del frames
+    d_out = device.empty(abi.FIT_DTYPE, n)
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    ctx = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width)
+    ctx.set_option(abi.OPT_MAX_HALF_WIDTH, w.half_width[1])
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+    def one_call():
+        ctx.fit_device(d_frames.ptr, w.nx, w.ny, w.nframes, d_cands.ptr, n, d_out.ptr, s)
+
+    ms_default = timed(one_call)
+    fits = d_out.numpy()
+    accepted = int(np.sum(fits["status"] == 0))
+    # autotune on the first 10 % of the frames, then time the whole batch again
+    n_tune = int(np.searchsorted(cands["frame"], w.nframes // 10))
+    best = ctx.autotune_device(d_frames.ptr, w.nx, w.ny, w.nframes, d_cands.ptr, n_tune, d_out.ptr, s)
+    chosen = {"occupancy": ctx.get_option(abi.OPT_OCCUPANCY), "waves_per_cu": ctx.get_option(abi.OPT_WAVES_PER_CU),
+              "rounds": ctx.get_option(abi.OPT_ROUNDS)}
+    ms_tuned = timed(one_call)
+    # frame blocks through captured graphs: candidates of block b are staged (frames stay
+    # resident), a graph replays, results are copied out.  Blocks rotate over `pipes`
+    # pipelines (context + graph + stream), so one block's slowest fits overlap the next
+    # blocks' work.
+    per_block = w.emitters_per_frame * frames_per_block
+    nblocks = (n + per_block - 1) // per_block
+    counts = torch.tensor([min(per_block, n - b * per_block) for b in range(nblocks)], dtype=torch.int32,
+                          device="cuda")
+    cb, fb = abi.CANDIDATE_DTYPE.itemsize, abi.FIT_DTYPE.itemsize
+    cview, oview = d_cands.t, d_out.t
+    pipes = []
+    for _ in range(4):
+        pc = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width)
+        pc.set_option(abi.OPT_MAX_HALF_WIDTH, w.half_width[1])
+        st_c = device.empty(abi.CANDIDATE_DTYPE, per_block)
+        st_o = device.empty(abi.FIT_DTYPE, per_block)
+        st_n = torch.zeros(1, dtype=torch.int32, device="cuda")
+        g = pc.graph(d_frames.ptr, w.nx, w.ny, w.nframes, st_c.ptr, st_n.data_ptr(), per_block, st_o.ptr)
+        pipes.append((pc, st_c, st_o, st_n, g, torch.cuda.Stream()))
+
+    def blocks():
+        start_ev = torch.cuda.Event()
+        start_ev.record(stream)
+        for _, _, _, _, _, ps in pipes:
+            ps.wait_event(start_ev)
+        for b in range(nblocks):
+            pc, st_c, st_o, st_n, g, ps = pipes[b % len(pipes)]
+            k = int(min(per_block, n - b * per_block))
+            with torch.cuda.stream(ps):
+                st_c.t[: k * cb].copy_(cview[b * per_block * cb:(b * per_block + k) * cb])
+                st_n.copy_(counts[b:b + 1])
+                g.launch(ps.cuda_stream)
+                oview[b * per_block * fb:(b * per_block + k) * fb].copy_(st_o.t[: k * fb])
+        for *_, ps in pipes:
+            stream.wait_stream(ps)
+
+    ms_graph = timed(blocks)
+    gfits = d_out.numpy()
+    same = bool(np.array_equal(gfits["status"], fits["status"]))
+    for pc, *_rest, g, _ps in pipes:
+        g.close()
+        pc.close()
+    ctx.close()
+    out = {
+        "workload": w.name,
+        "rois": int(n),
+        "windows": "5x5..15x15",
+        "ellipsoidal_fraction": float(np.mean(cands["model"] == abi.MODEL_ELLIPSOIDAL)),
+        "accepted": accepted,
+        "value": accepted / (ms_tuned * 1e-3),
+        "ms_one_call_default": ms_default,
+        "ms_one_call_autotuned": ms_tuned,
+        "autotune": {"sample_rois": int(n_tune), "best_sample_ms": best, "chosen": chosen},
+        "graph_blocks": {"frames_per_block": frames_per_block, "blocks": int(nblocks), "pipelines": len(pipes),
+                         "ms": ms_graph,
+                         "value": accepted / (ms_graph * 1e-3), "status_equal_to_one_call": same},
+        "trials_mean": float(fits["trials"].mean()),
+    }
+    log(f"mixed: one call {ms_default:.2f} ms, autotuned {ms_tuned:.2f} ms ({chosen}), graph blocks {ms_graph:.2f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +296,8 @@ def main():
     ap.add_argument("--park-threshold", type=int, default=-1, help="busy lanes at which a wave parks its fits")
     ap.add_argument("--no-net", action="store_true", help="skip the net-initialized LM leg")
     ap.add_argument("--only-net", action="store_true", help="run only the net-initialized LM leg (profiling)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-workload leg")
+    ap.add_argument("--block-frames", type=int, default=5000, help="frames per graph-replayed block (mixed leg)")
     ap.add_argument("--net-train-rois", type=int, default=4 << 20)
     ap.add_argument("--net-eval-rois", type=int, default=4000000)
     ap.add_argument("--net-epochs", type=float, default=2.0)
@@ -263,10 +371,15 @@ def main():
           f"{achieved:.2f} TFLOP/s algorithmic, mean trials {fits['trials'].mean():.2f}",
           file=sys.stderr, flush=True)
 
+    del d_frames, d_cands, d_out
+    torch.cuda.empty_cache()
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_leg(max(1, min(args.steps, 5)), 1, lambda m: print(m, file=sys.stderr, flush=True),
+                          frames_per_block=args.block_frames)
+        torch.cuda.empty_cache()
     net = None
     if not args.no_net:
-        del d_frames, d_cands, d_out
-        torch.cuda.empty_cache()
         net = net_init_leg(args.net_train_rois, args.net_eval_rois, args.net_epochs, max(1, min(args.steps, 5)),
                            1, lambda m: print(m, file=sys.stderr, flush=True))
 
@@ -313,6 +426,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "net_init": net,
+        "mixed": mixed,
     }
     print(json.dumps(line), flush=True)
     ctx.close()
