@@ -107,6 +107,18 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
 /* Fits all candidates with a frame-parallel thread pool (nthreads <= 0: hardware
  * concurrency).  Candidates of one frame are fitted by one task, in list order, as
  * PALMAnalysisController's workers do.  Returns the number of accepted fits. */
+/* ---- candidate generation (lzr_find_oracle.c) ---- */
+/* GLRT window side for a PSF width: ceil(4 w), made odd. */
+int orc_glrt_window(double width);
+/* mask[y*nx + x] = 1 where the GLRT segments the pixel; test_out (may be NULL) receives
+ * the test statistic of every tested pixel (0 elsewhere).  Returns 0 or -1. */
+int orc_glrt_mask(const uint16_t *frame, int nx, int ny, double width, double pfa, uint8_t *mask,
+                  double *test_out);
+/* 8-connected components of the mask in the reference's order: out[4k..4k+3] =
+ * (x, y, intensity, background) of particle k (at most max_out written).  Returns the
+ * number of particles or -1. */
+int orc_find_particles(const uint16_t *frame, const uint8_t *mask, int nx, int ny, double *out, int max_out);
+
 int orc_fit_batch(const OrcSettings *s, const uint16_t *frames, int nx, int ny, int nframes,
                   const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads);
 

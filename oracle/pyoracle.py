@@ -75,6 +75,14 @@ def lib():
             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
         ]
         L.orc_fit_batch.restype = ctypes.c_int
+        L.orc_glrt_window.argtypes = [ctypes.c_double]
+        L.orc_glrt_window.restype = ctypes.c_int
+        L.orc_glrt_mask.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_glrt_mask.restype = ctypes.c_int
+        L.orc_find_particles.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int]
+        L.orc_find_particles.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -196,3 +204,35 @@ def fit_batch(frames, candidates, model_code, initial_psf_width, sigma=1.0, defa
     lib().orc_fit_batch(ctypes.byref(s), frames.ctypes.data, nx, ny, nframes, cands.ctypes.data,
                         cands.size, out.ctypes.data, int(nthreads))
     return out
+
+
+def glrt_window(width):
+    return lib().orc_glrt_window(float(width))
+
+
+def glrt_mask(frame, width, pfa):
+    """GLRT segmentation of one uint16 frame [ny, nx]: (mask uint8, test statistic)."""
+    frame = np.ascontiguousarray(frame, np.uint16)
+    ny, nx = frame.shape
+    mask = np.zeros((ny, nx), np.uint8)
+    test = np.zeros((ny, nx))
+    if lib().orc_glrt_mask(frame.ctypes.data, nx, ny, float(width), float(pfa), mask.ctypes.data,
+                           test.ctypes.data) != 0:
+        raise ValueError("window too large")
+    return mask, test
+
+
+def find_particles(frame, mask):
+    """8-connected components: array [n, 4] of (x, y, intensity, background)."""
+    frame = np.ascontiguousarray(frame, np.uint16)
+    mask = np.ascontiguousarray(mask, np.uint8)
+    ny, nx = frame.shape
+    cap = 4096
+    while True:
+        out = np.zeros((cap, 4))
+        n = lib().orc_find_particles(frame.ctypes.data, mask.ctypes.data, nx, ny, out.ctypes.data, cap)
+        if n < 0:
+            raise MemoryError
+        if n <= cap:
+            return out[:n]
+        cap = n
