@@ -185,6 +185,29 @@ LZR_EXPORT int lzr_autotune_device(LzrContext *ctx, const uint16_t *d_frames, in
                                    const LzrCandidate *d_candidates, int ncandidates, LzrFit *d_results,
                                    void *stream, double *best_ms);
 
+/* ---- candidate generation ---------------------------------------------------------------
+ * The reference's segmentation and particle finding on the device:
+ *   ThresholdImage_GLRT_FFT(PFA, width)  (source/Segmentation.cpp:241-430)
+ *   ParticleFinder_adjacent8             (source/ParticleFinding.cpp:283-427)
+ * A finder holds the GLRT parameters (window side ceil(4 w), made odd, at most 15) and
+ * work buffers; max_segmented_per_frame (<= 16384; 0: 16384) bounds the segmented pixels
+ * of one frame. */
+typedef struct LzrFinder LzrFinder;
+LZR_EXPORT int lzr_finder_create(LzrFinder **finder, double psf_width, double pfa, int max_segmented_per_frame);
+LZR_EXPORT int lzr_finder_destroy(LzrFinder *finder);
+LZR_EXPORT int lzr_finder_last_error(const LzrFinder *finder, char *buf, int len);
+
+/* Segments nframes device frames and writes their particles as candidates (frame-major,
+ * each frame in the reference's particle order; x, y = mean pixel coordinates of the
+ * 8-connected component, intensity = max - background, background = mean of the
+ * unsegmented pixels; half_width 0, model -1) to d_candidates; *d_count (device int)
+ * receives their number, or -1 when a frame had more segmented pixels than the finder
+ * holds or the particles exceed max_candidates (then nothing is written).  Enqueued on
+ * `stream`; the count can feed a frame-block graph directly.  Returns 0 or -1. */
+LZR_EXPORT int lzr_find_candidates_device(LzrFinder *finder, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                          LzrCandidate *d_candidates, int max_candidates, int *d_count,
+                                          void *stream);
+
 /* ---- LM start points --------------------------------------------------------------
  * No reference counterpart: the reference always starts the LM at the particle
  * (A0 = intensity, w, x, y, b0; source/Localization.cpp:138-142).  A start point

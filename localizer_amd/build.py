@@ -67,7 +67,8 @@ def build_hip():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "liblocalizer_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "lzr_fit.hip"), os.path.join(CSRC, "lzr_net.hip")])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "lzr_fit.hip"), os.path.join(CSRC, "lzr_net.hip"),
+          os.path.join(CSRC, "lzr_find.hip")])
     os.replace(tmp, out)
 
 

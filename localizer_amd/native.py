@@ -37,6 +37,10 @@ EXPORTED = (
     "lzr_graph_launch",
     "lzr_graph_destroy",
     "lzr_autotune_device",
+    "lzr_finder_create",
+    "lzr_finder_destroy",
+    "lzr_finder_last_error",
+    "lzr_find_candidates_device",
 )
 
 
@@ -88,8 +92,12 @@ def lib():
         L.lzr_graph_launch.argtypes = [vp, vp]
         L.lzr_graph_destroy.argtypes = [vp]
         L.lzr_autotune_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp, ctypes.POINTER(d)]
+        L.lzr_finder_create.argtypes = [ctypes.POINTER(vp), d, d, i]
+        L.lzr_finder_destroy.argtypes = [vp]
+        L.lzr_finder_last_error.argtypes = [vp, ctypes.c_char_p, i]
+        L.lzr_find_candidates_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
         for name in EXPORTED:
-            if name.startswith(("lzr_net", "lzr_fit_batch_device_", "lzr_graph", "lzr_get", "lzr_autotune")):
+            if name.startswith(("lzr_net", "lzr_fit_batch_device_", "lzr_graph", "lzr_get", "lzr_autotune", "lzr_find")):
                 getattr(L, name).restype = ctypes.c_int
         if L.lzr_abi_version() != 1:
             raise LocalizerError("ABI version mismatch")
@@ -289,3 +297,39 @@ class Net:
             ctypes.byref(loss) if want_loss else None, ctypes.c_void_p(grads.ctypes.data) if want_grads else None,
             ctypes.c_void_p(stream)))
         return (loss.value if want_loss else None), grads
+
+
+def _finder_error(h):
+    buf = ctypes.create_string_buffer(1024)
+    lib().lzr_finder_last_error(h, buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+class Finder:
+    """GLRT segmentation + 8-way particle finding on the device (lzr_finder_*)."""
+
+    def __init__(self, psf_width, pfa, max_segmented_per_frame=0):
+        h = ctypes.c_void_p()
+        if lib().lzr_finder_create(ctypes.byref(h), float(psf_width), float(pfa), int(max_segmented_per_frame)) != 0:
+            raise LocalizerError(_finder_error(None))
+        self._h = h
+        self._destroy = lib().lzr_finder_destroy
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def find_device(self, d_frames, nx, ny, nframes, d_cands, max_candidates, d_count, stream=0):
+        if lib().lzr_find_candidates_device(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                            ctypes.c_void_p(d_cands), int(max_candidates), ctypes.c_void_p(d_count),
+                                            ctypes.c_void_p(stream)) != 0:
+            raise LocalizerError(_finder_error(self._h))
