@@ -208,6 +208,26 @@ LZR_EXPORT int lzr_find_candidates_device(LzrFinder *finder, const uint16_t *d_f
                                           LzrCandidate *d_candidates, int max_candidates, int *d_count,
                                           void *stream);
 
+/* ---- whole pipeline ---------------------------------------------------------------------
+ * Segmentation + particle finding + fit of a frame stack in host memory, results as the
+ * reference's positions matrix (column-major, n_emitters rows, sorted by frame; within a
+ * frame in the particle finder's order), allocated by the callee and released with
+ * lzr_free.  Return 0 or -1 (lzr_pipeline_last_error explains). */
+
+/* LocalizerFitEmitters (source/LocalizerDLL.h:51-52, source/LocalizerDLL.cpp:89-129):
+ * ellipsoidal fit, 16 columns [frame, integral, sd1, sd2, x, y, theta, background,
+ * integralDev, sd1Dev, sd2Dev, xDev, yDev, thetaDev, backgroundDev, nFramesPresent]
+ * (source/Storage.cpp:614-640).  n_rows = x extent, n_cols = y extent. */
+LZR_EXPORT int lzr_fit_emitters(const uint16_t *image_data, int n_rows, int n_cols, int n_images, double psf_width,
+                                double glrt_insensitivity, int *n_emitters, double **emitter_coordinates);
+/* The same with any LZR_MODEL_*: 12 columns (symmetric, source/Storage.cpp:268-290),
+ * 10 (fixed width, :405-425) or 16 (ellipsoidal, symmetric-PSF filter). */
+LZR_EXPORT int lzr_localize(const uint16_t *image_data, int n_rows, int n_cols, int n_images, double psf_width,
+                            double glrt_insensitivity, int model, int *n_emitters, int *n_columns, double **matrix);
+/* LocalizerFree (source/LocalizerDLL.cpp:131-133). */
+LZR_EXPORT void lzr_free(double *ptr);
+LZR_EXPORT int lzr_pipeline_last_error(char *buf, int len);
+
 /* ---- LM start points --------------------------------------------------------------
  * No reference counterpart: the reference always starts the LM at the particle
  * (A0 = intensity, w, x, y, b0; source/Localization.cpp:138-142).  A start point

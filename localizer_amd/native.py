@@ -41,6 +41,10 @@ EXPORTED = (
     "lzr_finder_destroy",
     "lzr_finder_last_error",
     "lzr_find_candidates_device",
+    "lzr_fit_emitters",
+    "lzr_localize",
+    "lzr_free",
+    "lzr_pipeline_last_error",
 )
 
 
@@ -96,6 +100,15 @@ def lib():
         L.lzr_finder_destroy.argtypes = [vp]
         L.lzr_finder_last_error.argtypes = [vp, ctypes.c_char_p, i]
         L.lzr_find_candidates_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
+        L.lzr_fit_emitters.argtypes = [vp, i, i, i, d, d, ctypes.POINTER(i), ctypes.POINTER(ctypes.POINTER(d))]
+        L.lzr_fit_emitters.restype = ctypes.c_int
+        L.lzr_localize.argtypes = [vp, i, i, i, d, d, i, ctypes.POINTER(i), ctypes.POINTER(i),
+                                   ctypes.POINTER(ctypes.POINTER(d))]
+        L.lzr_localize.restype = ctypes.c_int
+        L.lzr_free.argtypes = [ctypes.POINTER(d)]
+        L.lzr_free.restype = None
+        L.lzr_pipeline_last_error.argtypes = [ctypes.c_char_p, i]
+        L.lzr_pipeline_last_error.restype = ctypes.c_int
         for name in EXPORTED:
             if name.startswith(("lzr_net", "lzr_fit_batch_device_", "lzr_graph", "lzr_get", "lzr_autotune", "lzr_find")):
                 getattr(L, name).restype = ctypes.c_int
@@ -333,3 +346,24 @@ class Finder:
                                             ctypes.c_void_p(d_cands), int(max_candidates), ctypes.c_void_p(d_count),
                                             ctypes.c_void_p(stream)) != 0:
             raise LocalizerError(_finder_error(self._h))
+
+
+def localize(frames, psf_width, glrt_insensitivity, model=abi.MODEL_ELLIPSOIDAL):
+    """Whole pipeline on host frames [n, ny, nx] (lzr_localize): the reference's positions
+    matrix (rows sorted by frame)."""
+    frames = np.ascontiguousarray(frames, np.uint16)
+    if frames.ndim == 2:
+        frames = frames[None]
+    nf, ny, nx = frames.shape
+    n, cols = ctypes.c_int(), ctypes.c_int()
+    ptr = ctypes.POINTER(ctypes.c_double)()
+    if lib().lzr_localize(frames.ctypes.data, nx, ny, nf, float(psf_width), float(glrt_insensitivity), int(model),
+                          ctypes.byref(n), ctypes.byref(cols), ctypes.byref(ptr)) != 0:
+        buf = ctypes.create_string_buffer(1024)
+        lib().lzr_pipeline_last_error(buf, len(buf))
+        raise LocalizerError(buf.value.decode(errors="replace"))
+    try:
+        flat = np.ctypeslib.as_array(ptr, shape=(max(n.value, 1) * cols.value,)).copy()
+    finally:
+        lib().lzr_free(ptr)
+    return flat[: n.value * cols.value].reshape(cols.value, n.value).T.copy()
