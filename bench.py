@@ -282,6 +282,97 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
     return out
 
 
+DENSE_PFA = 20.0
+
+
+def dense_leg(nframes, block_frames, latency_frames, log):
+    """Dense end-to-end stream: frames -> GLRT segmentation -> 8-way particle finding ->
+    11x11 patch gather -> CNN start -> LM with residual fallback, on the device; frames
+    resident in HBM, one host read of the candidate count per block."""
+    import torch
+
+    w = synth.DENSE_2048.scaled(nframes)
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    frames, _, _ = synth.generate(w, nthreads=nthreads)
+    log(f"dense: generated {w.nframes} frames of {w.nx}x{w.ny} in {time.perf_counter() - t0:.1f} s")
+    d_frames = device.upload(frames)
+    del frames
+    npx = w.nx * w.ny
+    net = native.Net(w.initial_psf_width, seed=5)
+    tr = netinit.train(net, synth.DENSE_ROIS_11, 1 << 20, epochs=1.0, batch=2048, lr=2e-3)
+    log(f"dense: network trained on {tr['rois']} ROIs in {tr['train_s']:.2f} s")
+    finder = native.Finder(w.initial_psf_width, DENSE_PFA)
+    ctx = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, 1.0, w.half_width[0])
+    cap = block_frames * 2048
+    d_cands = device.empty(abi.CANDIDATE_DTYPE, cap)
+    d_fits = device.empty(abi.FIT_DTYPE, cap)
+    d_work = device.empty(abi.START_DTYPE, cap)
+    d_count = torch.zeros(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+
+    def run_block(f0, nb, netinit_fit=True):
+        ptr = d_frames.ptr + f0 * npx * 2
+        finder.find_device(ptr, w.nx, w.ny, nb, d_cands.ptr, cap, d_count.data_ptr(), s)
+        n = int(d_count.item())  # synchronizes
+        if n < 0:
+            raise RuntimeError("dense stream: candidate capacity exceeded")
+        if n:
+            if netinit_fit:
+                ctx.fit_device_netinit(net, ptr, w.nx, w.ny, nb, d_cands.ptr, n, d_fits.ptr, d_work.ptr, 3.0,
+                                       stream=s)
+            else:
+                ctx.fit_device(ptr, w.nx, w.ny, nb, d_cands.ptr, n, d_fits.ptr, s)
+        return n
+
+    def stream_all(netinit_fit):
+        run_block(0, min(block_frames, nframes), netinit_fit)  # warm-up
+        torch.cuda.synchronize()
+        cands = acc = 0
+        t = time.perf_counter()
+        for f0 in range(0, nframes, block_frames):
+            nb = min(block_frames, nframes - f0)
+            n = run_block(f0, nb, netinit_fit)
+            torch.cuda.synchronize()
+            cands += n
+            acc += int(np.sum(d_fits.t[: n * abi.FIT_DTYPE.itemsize].cpu().numpy().view(abi.FIT_DTYPE)["status"] == 0))
+        return time.perf_counter() - t, cands, acc
+
+    sec_net, cands, acc_net = stream_all(True)
+    sec_plain, _, acc_plain = stream_all(False)
+    lat = []
+    for f in range(min(latency_frames, nframes)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        run_block(f, 1, True)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+    out = {
+        "workload": w.name,
+        "frames": nframes,
+        "frame_px": f"{w.nx}x{w.ny}",
+        "glrt_pfa": DENSE_PFA,
+        "window": f"{2 * w.half_width[0] + 1}x{2 * w.half_width[0] + 1}",
+        "block_frames": block_frames,
+        "candidates_per_frame": cands / nframes,
+        "netinit": {"frames_per_s": nframes / sec_net, "localizations_per_s": acc_net / sec_net,
+                    "accepted": acc_net, "seconds": sec_net},
+        "plain": {"frames_per_s": nframes / sec_plain, "localizations_per_s": acc_plain / sec_plain,
+                  "accepted": acc_plain, "seconds": sec_plain},
+        "latency_ms_per_frame": {"frames": int(lat.size), "p50": float(np.percentile(lat, 50)),
+                                 "p99": float(np.percentile(lat, 99)), "mode": "one frame per call, netinit"},
+    }
+    log(f"dense: {cands / nframes:.1f} candidates/frame; net-init {nframes / sec_net:.1f} frames/s "
+        f"({acc_net / sec_net:.4g} loc/s), plain {nframes / sec_plain:.1f} frames/s; latency p50 "
+        f"{out['latency_ms_per_frame']['p50']:.2f} ms p99 {out['latency_ms_per_frame']['p99']:.2f} ms")
+    finder.close()
+    ctx.close()
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,7 +388,10 @@ def main():
     ap.add_argument("--no-net", action="store_true", help="skip the net-initialized LM leg")
     ap.add_argument("--only-net", action="store_true", help="run only the net-initialized LM leg (profiling)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-workload leg")
-    ap.add_argument("--block-frames", type=int, default=5000, help="frames per graph-replayed block (mixed leg)")
+    ap.add_argument("--block-frames", type=int, default=10000, help="frames per graph-replayed block (mixed leg)")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense end-to-end stream leg")
+    ap.add_argument("--dense-frames", type=int, default=1000)
+    ap.add_argument("--dense-block", type=int, default=50, help="frames per block of the dense stream")
     ap.add_argument("--net-train-rois", type=int, default=4 << 20)
     ap.add_argument("--net-eval-rois", type=int, default=4000000)
     ap.add_argument("--net-epochs", type=float, default=2.0)
@@ -378,6 +472,10 @@ def main():
         mixed = mixed_leg(max(1, min(args.steps, 5)), 1, lambda m: print(m, file=sys.stderr, flush=True),
                           frames_per_block=args.block_frames)
         torch.cuda.empty_cache()
+    dense = None
+    if not args.no_dense:
+        dense = dense_leg(args.dense_frames, args.dense_block, 100, lambda m: print(m, file=sys.stderr, flush=True))
+        torch.cuda.empty_cache()
     net = None
     if not args.no_net:
         net = net_init_leg(args.net_train_rois, args.net_eval_rois, args.net_epochs, max(1, min(args.steps, 5)),
@@ -427,6 +525,7 @@ def main():
         "cpu_baseline": cpu,
         "net_init": net,
         "mixed": mixed,
+        "dense": dense,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -168,6 +168,24 @@ LEARNED_INIT_11 = Workload(
     extra={"max_offset": 1.0, "train_stream": 4, "eval_stream": 44},
 )
 
+# Training ROIs for the dense stream's initializer: the dense workload's PSF, photon and
+# background ranges on independent 11x11 ROIs, the spot within +-0.5 px of the centre
+# (candidates come from the particle finder's blob centre).
+DENSE_ROIS_11 = Workload(
+    name="dense-stream-training-rois-11x11",
+    config_id=55,
+    nx=11,
+    ny=11,
+    nframes=1 << 20,
+    emitters_per_frame=1,
+    sigma=(1.3, 1.3),
+    photons=(150.0, 3000.0),
+    background=(100.0, 150.0),
+    half_width=(5, 5),
+    initial_psf_width=1.3,
+    extra={"max_offset": 0.5, "train_stream": 55, "eval_stream": 56},
+)
+
 WORKLOADS = {w.name: w for w in (PLAIN_CPU_512, PLAIN_GPU_256, MIXED_2M, LEARNED_INIT_11, DENSE_2048)}
 
 
