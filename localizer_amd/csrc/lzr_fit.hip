@@ -251,19 +251,48 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         }
         if (!open && s.phase == kIdle) s.phase = kFinished;
 
-        /* ---- gather the windows of newly loaded lanes ---- */
+        /* ---- gather the windows of newly loaded lanes ----
+         * kGatherLanes windows at a time, every load of the batch issued before the first
+         * LDS store, so the batch costs one global-memory latency. */
         unsigned long long gmask = __ballot(fresh);
         while (gmask) {
-            const int L = __ffsll((long long)gmask) - 1;
-            gmask &= gmask - 1ull;
-            const int fL = __shfl(s.frame, L, 64);
-            const int xL = __shfl(s.xoff, L, 64);
-            const int yL = __shfl(s.yoff, L, 64);
-            const int nL = __shfl(s.n, L, 64);
-            const uint16_t *src = args.frames + ((size_t)fL * args.ny + yL) * args.nx + xL;
-            for (int k = lane; k < nL * nL; k += 64) {
-                const int j = k / nL, i = k - j * nL;
-                tile[k * 64 + L] = src[(size_t)j * args.nx + i];
+            constexpr int kGatherLanes = 4;
+            int Lb[kGatherLanes], nb[kGatherLanes], inv[kGatherLanes];
+            const uint16_t *src[kGatherLanes];
+            int most = 0;
+#pragma unroll
+            for (int b = 0; b < kGatherLanes; ++b) {
+                const int L = gmask ? __ffsll((long long)gmask) - 1 : -1;
+                if (gmask) gmask &= gmask - 1ull;
+                const int Ls = L < 0 ? 0 : L;
+                const int fL = __shfl(s.frame, Ls, 64);
+                const int xL = __shfl(s.xoff, Ls, 64);
+                const int yL = __shfl(s.yoff, Ls, 64);
+                const int nL = L < 0 ? 0 : __shfl(s.n, Ls, 64);
+                Lb[b] = Ls;
+                nb[b] = nL;
+                /* j = k / n as (k * inv) >> 16: exact for k < n^2 when n <= 31 (half-width <= 15) */
+                inv[b] = nL > 0 ? 65536 / nL + 1 : 0;
+                src[b] = args.frames + ((size_t)fL * args.ny + yL) * args.nx + xL;
+                most = max(most, nL * nL);
+            }
+            for (int k0 = 0; k0 < most; k0 += 4 * 64) {
+                uint16_t v[kGatherLanes][4];
+#pragma unroll
+                for (int b = 0; b < kGatherLanes; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = k0 + lane + 64 * r;
+                        const int j = (k * inv[b]) >> 16, i = k - j * nb[b];
+                        v[b][r] = k < nb[b] * nb[b] ? src[b][(size_t)j * args.nx + i] : (uint16_t)0;
+                    }
+#pragma unroll
+                for (int b = 0; b < kGatherLanes; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = k0 + lane + 64 * r;
+                        if (k < nb[b] * nb[b]) tile[k * 64 + Lb[b]] = v[b][r];
+                    }
             }
         }
         fresh = false;
@@ -301,12 +330,13 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             for (int k = 0; k < P; ++k) z[k] = 1.0;
             make_pass_in(z, z, 1.0, 1.0, q);
         }
-        const int nw = wave_max(n);
+        const int nw = __builtin_amdgcn_readfirstlane(wave_max(n)); /* wave-uniform loop bound */
         const bool mixed = __any(active && n != nw);
         struct {
             const uint16_t *t;
             int n;
             __device__ float operator()(int i, int j) const { return (float)t[(i + j * n) * 64]; }
+            __device__ uint16_t raw(int i, int j) const { return t[(i + j * n) * 64]; }
         } pix{tile + lane, active ? n : nw};
         struct {
             double *c;

@@ -162,6 +162,35 @@ struct PassAcc {
     }
 };
 
+/* Per-pixel arithmetic is written with explicit fused multiply-adds and compiled with
+ * contraction off: every inlined copy of a pixel function (the loop variants of a pass,
+ * the three kernels, the host emulation) then evaluates exactly the same operations, so a
+ * fit's result never depends on which copy ran it. */
+#if defined(__clang__)
+#define LZR_NO_CONTRACT _Pragma("clang fp contract(off)")
+#else
+#define LZR_NO_CONTRACT
+#endif
+
+LZR_HD LZR_INLINE float ffma(float a, float b, float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fmaf(a, b, c);
+#else
+    return std::fma(a, b, c);
+#endif
+}
+
+LZR_HD LZR_INLINE double dfma(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_fma(a, b, c);
+#else
+    return std::fma(a, b, c);
+#endif
+}
+
+LZR_HD LZR_INLINE float acc_fma(float a, float b, float c) { return ffma(a, b, c); }
+LZR_HD LZR_INLINE double acc_fma(double a, double b, double c) { return dfma(a, b, c); }
+
 LZR_HD LZR_INLINE float fexp(float a) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __expf(a);
@@ -183,8 +212,18 @@ LZR_HD LZR_INLINE double dexp(double a) {
  * accuracy); for a large one e1 is evaluated directly, where the subtraction e1 - e0
  * loses at most ~1e-6 relative and e0 may have underflowed. */
 LZR_HD LZR_INLINE void exp_change(float e0, float arg1, float darg, float &e1, float &e0m) {
+    LZR_NO_CONTRACT
     const float a = -darg;
-    const float em = a * (1.0f + a * (0.5f + a * (1.0f / 6.0f + a * (1.0f / 24.0f + a * (1.0f / 120.0f)))));
+    float p = ffma(a, 1.0f / 120.0f, 1.0f / 24.0f);
+    p = ffma(a, p, 1.0f / 6.0f);
+    p = ffma(a, p, 0.5f);
+    p = ffma(a, p, 1.0f);
+    float em = a * p;
+#if defined(__HIP_DEVICE_COMPILE__)
+    /* both branches are a handful of FMAs: keep them unconditional (two v_cndmask) rather
+     * than a divergent branch per pixel */
+    asm volatile("" : "+v"(em));
+#endif
     const float e1d = fexp(-arg1);
     const bool small = darg > -0.0625f && darg < 0.0625f;
     e0m = small ? e0 * em : e1d - e0;
@@ -200,13 +239,14 @@ LZR_HD LZR_INLINE void exp_change(float e0, float arg1, float darg, float &e1, f
 template <bool F64>
 LZR_HD LZR_INLINE void residual0(double qd_arg0, float arg0_f, double A0d, double b0d, double isd, float A0,
                                  float b0, float is, float D, float &e0, float &f0) {
+    LZR_NO_CONTRACT
     if constexpr (F64) {
         const double e = dexp(-qd_arg0);
         e0 = (float)e;
-        f0 = (float)((A0d * e + (b0d - (double)D)) * isd);
+        f0 = (float)(dfma(A0d, e, b0d - (double)D) * isd);
     } else {
         e0 = fexp(-arg0_f);
-        f0 = (A0 * e0 + b0 - D) * is;
+        f0 = (ffma(A0, e0, b0) - D) * is;
     }
 }
 
@@ -215,35 +255,37 @@ LZR_HD LZR_INLINE void residual0(double qd_arg0, float arg0_f, double A0d, doubl
 template <typename Acc>
 LZR_HD LZR_INLINE void pixel_sym(const PassIn<5> &q, float ux0, float uy0, float uy1, float ddy, float D, double e0d,
                                  PassAcc<5, Acc> &a) {
+    LZR_NO_CONTRACT
     const float ux1 = ux0 - q.dxc;
-    const float d0 = ux0 * ux0 + uy0 * uy0;
-    const float d1 = ux1 * ux1 + uy1 * uy1;
+    const float d0 = ffma(ux0, ux0, uy0 * uy0);
+    const float d1 = ffma(ux1, ux1, uy1 * uy1);
     const float arg0 = d0 * q.c0;
     const float e0 = (float)e0d;
-    const float f0 = (float)((q.A0d * e0d + (q.b0d - (double)D)) * q.inv_sigma_d);
+    const float f0 = (float)(dfma(q.A0d, e0d, q.b0d - (double)D) * q.inv_sigma_d);
     /* exponent change, free of cancellation: (d1-d0) c1 + d0 (c1-c0) */
-    const float dd = ddy - q.dxc * (ux1 + ux0);
-    const float darg = dd * (q.c0 + q.dc) + d0 * q.dc;
+    const float dd = ffma(-q.dxc, ux1 + ux0, ddy);
+    const float darg = ffma(dd, q.c0 + q.dc, d0 * q.dc);
     float e1, e0m;
     exp_change(e0, arg0 + darg, darg, e1, e0m);
-    const float df = (q.dA * e1 + q.A0 * e0m + q.db) * q.inv_sigma;
+    const float df = ffma(q.dA, e1, ffma(q.A0, e0m, q.db)) * q.inv_sigma;
     const float f1 = f0 + df;
-    a.sse0 += Acc(f0) * Acc(f0);
-    a.dsse += Acc(df) * Acc(f0 + f1);
+    a.sse0 = acc_fma(Acc(f0), Acc(f0), a.sse0);
+    a.dsse = acc_fma(Acc(df), Acc(f0 + f1), a.dsse);
     /* Jacobian at the trial point, reference columns (A, r, x0, y0) */
     const float jA = e1 * q.inv_sigma;
     const float ae = q.A1 * e1;
-    const float jr = q.kr * ae * d1;
-    const float jx = q.kx * ae * ux1;
-    const float jy = q.kx * ae * uy1;
+    const float kae = q.kx * ae;
+    const float jr = (q.kr * ae) * d1;
+    const float jx = kae * ux1;
+    const float jy = kae * uy1;
     const Acc J[4] = {Acc(jA), Acc(jr), Acc(jx), Acc(jy)};
     const Acc F = Acc(f1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
-        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] += J[i] * J[j];
+        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] = acc_fma(J[i], J[j], a.hq[tri(i, j)]);
         a.sj[i] += J[i];
-        a.gq[i] += J[i] * F;
+        a.gq[i] = acc_fma(J[i], F, a.gq[i]);
     }
     a.sf += F;
 }
@@ -253,72 +295,76 @@ LZR_HD LZR_INLINE void pixel_sym(const PassIn<5> &q, float ux0, float uy0, float
 template <typename Acc>
 LZR_HD LZR_INLINE void pixel_sym(const PassIn<4> &q, float ux0, float uy0, float uy1, float ddy, float D, double e0d,
                                  PassAcc<4, Acc> &a) {
+    LZR_NO_CONTRACT
     const float ux1 = ux0 - q.dxc;
-    const float d0 = ux0 * ux0 + uy0 * uy0;
+    const float d0 = ffma(ux0, ux0, uy0 * uy0);
     const float arg0 = d0 * q.c0;
     const float e0 = (float)e0d;
-    const float f0 = (float)((q.A0d * e0d + (q.b0d - (double)D)) * q.inv_sigma_d);
-    const float darg = (ddy - q.dxc * (ux1 + ux0)) * q.c0;
+    const float f0 = (float)(dfma(q.A0d, e0d, q.b0d - (double)D) * q.inv_sigma_d);
+    const float darg = ffma(-q.dxc, ux1 + ux0, ddy) * q.c0;
     float e1, e0m;
     exp_change(e0, arg0 + darg, darg, e1, e0m);
-    const float df = (q.dA * e1 + q.A0 * e0m + q.db) * q.inv_sigma;
+    const float df = ffma(q.dA, e1, ffma(q.A0, e0m, q.db)) * q.inv_sigma;
     const float f1 = f0 + df;
-    a.sse0 += Acc(f0) * Acc(f0);
-    a.dsse += Acc(df) * Acc(f0 + f1);
+    a.sse0 = acc_fma(Acc(f0), Acc(f0), a.sse0);
+    a.dsse = acc_fma(Acc(df), Acc(f0 + f1), a.dsse);
     const float jA = e1 * q.inv_sigma;
     const float ae = q.A1 * e1;
-    const float jx = q.kx * ae * ux1;
-    const float jy = q.kx * ae * uy1;
+    const float kae = q.kx * ae;
+    const float jx = kae * ux1;
+    const float jy = kae * uy1;
     const Acc J[3] = {Acc(jA), Acc(jx), Acc(jy)};
     const Acc F = Acc(f1);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
-        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] += J[i] * J[j];
+        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] = acc_fma(J[i], J[j], a.hq[tri(i, j)]);
         a.sj[i] += J[i];
-        a.gq[i] += J[i] * F;
+        a.gq[i] = acc_fma(J[i], F, a.gq[i]);
     }
     a.sf += F;
 }
 
 template <typename Acc>
 LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D, PassAcc<7, Acc> &a) {
+    LZR_NO_CONTRACT
     const float ux0 = fi - q.x0, uy0 = fj - q.y0;
     const float ux1 = ux0 - q.dxc, uy1 = uy0 - q.dyc;
     const float xx0 = ux0 * ux0, yy0 = uy0 * uy0, xy0 = ux0 * uy0;
     const float xx1 = ux1 * ux1, yy1 = uy1 * uy1, xy1 = ux1 * uy1;
-    const float arg0 = q.qa0 * xx0 + 2.0f * q.qc0 * xy0 + q.qb0 * yy0;
+    const float arg0 = ffma(q.qa0, xx0, ffma(2.0f * q.qc0, xy0, q.qb0 * yy0));
     float e0, f0;
     {
         const double dx = ux0, dy = uy0;
-        residual0<true>(q.qa0d * dx * dx + 2.0 * q.qc0d * dx * dy + q.qb0d * dy * dy, arg0, q.A0d, q.b0d,
-                        q.inv_sigma_d, q.A0, q.b0, q.inv_sigma, D, e0, f0);
+        const double qd = dfma(q.qa0d * dx, dx, dfma(2.0 * q.qc0d * dx, dy, (q.qb0d * dy) * dy));
+        residual0<true>(qd, arg0, q.A0d, q.b0d, q.inv_sigma_d, q.A0, q.b0, q.inv_sigma, D, e0, f0);
     }
     const float dxx = -q.dxc * (ux1 + ux0);
     const float dyy = -q.dyc * (uy1 + uy0);
-    const float dxy = -(ux1 * q.dyc + uy0 * q.dxc);
-    const float darg = q.dqa * xx1 + q.qa0 * dxx + 2.0f * (q.dqc * xy1 + q.qc0 * dxy) + q.dqb * yy1 + q.qb0 * dyy;
+    const float dxy = -ffma(ux1, q.dyc, uy0 * q.dxc);
+    const float dqc = ffma(q.dqc, xy1, q.qc0 * dxy);
+    const float darg = ffma(q.dqa, xx1, ffma(q.qa0, dxx, ffma(2.0f, dqc, ffma(q.dqb, yy1, q.qb0 * dyy))));
     float e1, e0m;
     exp_change(e0, arg0 + darg, darg, e1, e0m);
-    const float df = (q.dA * e1 + q.A0 * e0m + q.db) * q.inv_sigma;
+    const float df = ffma(q.dA, e1, ffma(q.A0, e0m, q.db)) * q.inv_sigma;
     const float f1 = f0 + df;
-    a.sse0 += Acc(f0) * Acc(f0);
-    a.dsse += Acc(df) * Acc(f0 + f1);
+    a.sse0 = acc_fma(Acc(f0), Acc(f0), a.sse0);
+    a.dsse = acc_fma(Acc(df), Acc(f0 + f1), a.dsse);
     const float ae = q.A1 * e1;
     const float jA = e1 * q.inv_sigma;
-    const float jsx = ae * (q.k1 * xx1 - q.k2 * xy1);
-    const float jsy = ae * (q.k3 * yy1 - q.k4 * xy1);
-    const float jx = ae * (q.k5 * ux1 - q.k6 * uy1);
-    const float jy = ae * (q.k7 * uy1 - q.k6 * ux1);
-    const float jr = ae * (q.k8 * xy1 - q.k9 * (xx1 + yy1 - q.two_rho * xy1));
+    const float jsx = ae * ffma(q.k1, xx1, -(q.k2 * xy1));
+    const float jsy = ae * ffma(q.k3, yy1, -(q.k4 * xy1));
+    const float jx = ae * ffma(q.k5, ux1, -(q.k6 * uy1));
+    const float jy = ae * ffma(q.k7, uy1, -(q.k6 * ux1));
+    const float jr = ae * ffma(q.k8, xy1, -(q.k9 * ffma(-q.two_rho, xy1, xx1 + yy1)));
     const Acc J[6] = {Acc(jA), Acc(jsx), Acc(jsy), Acc(jx), Acc(jy), Acc(jr)};
     const Acc F = Acc(f1);
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
 #pragma unroll
-        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] += J[i] * J[j];
+        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] = acc_fma(J[i], J[j], a.hq[tri(i, j)]);
         a.sj[i] += J[i];
-        a.gq[i] += J[i] * F;
+        a.gq[i] = acc_fma(J[i], F, a.gq[i]);
     }
     a.sf += F;
 }
@@ -326,7 +372,8 @@ LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D
 /* One pass over a lane's window: residuals at the current point, residual change and
  * Jacobian at the trial point, normal equations.  nloop >= n is the loop bound shared
  * by the wave; `guard` masks pixels beyond the lane's own n (mixed window sizes).
- * pix(i, j) returns pixel (i, j); xb holds one FP64 value per column.
+ * pix(i, j) returns pixel (i, j) as float, pix.raw(i, j) as stored; xb holds one FP64
+ * value per column.
  * Symmetric and fixed-width models: exp(-c0 (ux^2 + uy^2)) = Ex[i] Ey[j], 2n FP64
  * exponentials per pass. */
 template <int P, typename Acc, typename Pix, typename XBuf>
@@ -341,11 +388,27 @@ LZR_HD LZR_INLINE void window_pass_separable(const PassIn<P> &q, int n, int nloo
         const float uy1 = uy0 - q.dyc;
         const float ddy = -q.dyc * (uy1 + uy0);
         const double Ey = dexp(-q.c0d * (double)uy0 * (double)uy0);
+        /* the next pixel's LDS reads are issued before this pixel's arithmetic */
+        auto Dn = pix.raw(0, j); /* converted at use, a full pixel later */
+        double xn = xb.get(0);
         if (guard) {
-            for (int i = 0; i < nloop; ++i)
-                if (i < n && j < n) pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, pix(i, j), xb.get(i) * Ey, a);
+            for (int i = 0; i < nloop; ++i) {
+                const float D = (float)Dn;
+                const double xv = xn;
+                const int inext = i + 1 < nloop ? i + 1 : i; /* unconditional, in-bounds read */
+                Dn = pix.raw(inext, j);
+                xn = xb.get(inext);
+                if (i < n && j < n) pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, D, xv * Ey, a);
+            }
         } else {
-            for (int i = 0; i < nloop; ++i) pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, pix(i, j), xb.get(i) * Ey, a);
+            for (int i = 0; i < nloop; ++i) {
+                const float D = (float)Dn;
+                const double xv = xn;
+                const int inext = i + 1 < nloop ? i + 1 : i;
+                Dn = pix.raw(inext, j);
+                xn = xb.get(inext);
+                pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, D, xv * Ey, a);
+            }
         }
     }
 }

@@ -27,6 +27,7 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             const float *D;
             int n;
             float operator()(int i, int j) const { return D[i + n * j]; }
+            float raw(int i, int j) const { return D[i + n * j]; }
         } pix{D, n};
         struct {
             double v[2 * LZR_MAX_HALF_WIDTH + 1];
