@@ -444,6 +444,23 @@ LZR_HD LZR_INLINE double dsqrt(double v) {
     return std::sqrt(v);
 #endif
 }
+/* 1/sqrt(v) for v > 0: the hardware estimate refined by two Newton steps
+ * (r <- r + r (1/2 - v r^2 / 2)), within ~1 ulp; one quarter-rate instruction and six
+ * FMAs instead of a correctly rounded square root followed by a division. */
+LZR_HD LZR_INLINE double drsqrt(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rsq(v);
+    const double h = 0.5 * v;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = __builtin_fma(-(h * r), r, 0.5);
+        r = __builtin_fma(r, e, r);
+    }
+    return r;
+#else
+    return 1.0 / std::sqrt(v);
+#endif
+}
 LZR_HD LZR_INLINE double dabs(double v) { return v < 0 ? -v : v; }
 LZR_HD LZR_INLINE double dmax(double a, double b) { return a > b ? a : b; }
 LZR_HD LZR_INLINE double dmin(double a, double b) { return a < b ? a : b; }
@@ -457,9 +474,8 @@ LZR_HD LZR_INLINE bool cholesky(const double (&M)[NP<P>::S], double (&L)[NP<P>::
 #pragma unroll
         for (int k = 0; k < j; ++k) s -= L[tri(j, k)] * L[tri(j, k)];
         if (!(s > 0.0)) return false;
-        const double d = dsqrt(s);
-        const double inv = 1.0 / d;
-        L[tri(j, j)] = d;
+        const double inv = drsqrt(s);
+        L[tri(j, j)] = s * inv;
         inv_diag[j] = inv;
 #pragma unroll
         for (int i = j + 1; i < P; ++i) {
@@ -507,11 +523,16 @@ LZR_HD LZR_INLINE double scaled_norm(const double (&d)[P], const double (&v)[P])
 }
 
 template <int P>
-LZR_HD LZR_INLINE double plain_norm(const double (&v)[P]) {
+LZR_HD LZR_INLINE double plain_norm2(const double (&v)[P]) {
     double s = 0.0;
 #pragma unroll
     for (int i = 0; i < P; ++i) s += v[i] * v[i];
-    return dsqrt(s);
+    return s;
+}
+
+template <int P>
+LZR_HD LZR_INLINE double plain_norm(const double (&v)[P]) {
+    return dsqrt(plain_norm2<P>(v));
 }
 
 /* Levenberg-Marquardt parameter (MINPACK lmpar on the normal equations).
@@ -543,11 +564,11 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
     }
     double parl = 0.0;
     if (full_rank) {
+        const double inv_dxnorm = 1.0 / dxnorm;
 #pragma unroll
-        for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] / dxnorm);
+        for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] * inv_dxnorm);
         forward<P>(L, id, u, y);
-        const double wn = plain_norm<P>(y);
-        const double phider = wn * wn;
+        const double phider = plain_norm2<P>(y);
         parl = phider > 0.0 ? fp / (delta * phider) : 0.0;
     }
 #pragma unroll
@@ -580,11 +601,11 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
         if (dabs(fp) <= 0.1 * delta) break;
         if (parl == 0.0 && fp <= fp_old && fp_old < 0.0) break;
         if (iter == 10) break;
+        const double inv_dxnorm = 1.0 / dxnorm;
 #pragma unroll
-        for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] / dxnorm);
+        for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] * inv_dxnorm);
         forward<P>(L, id, u, y);
-        const double wn = plain_norm<P>(y);
-        const double parc = fp / (delta * wn * wn);
+        const double parc = fp / (delta * plain_norm2<P>(y));
         if (fp > 0.0) {
             if (par > parl) parl = par;
         } else if (fp < 0.0) {
@@ -717,40 +738,60 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
                                             double (&cn)[P]) {
 #pragma unroll
     for (int c = 0; c < P; ++c) cn[c] = 0.0;
+    /* reciprocals once per call; the rows then take products only */
+    const double is = 1.0 / sigma;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    if constexpr (P == 4) {
+        r0 = 1.0 / (w * w); /* 1/w^2 */
+    } else if constexpr (P == 5) {
+        r0 = 1.0 / t[1];    /* 1/r */
+        r1 = r0 * r0;       /* 1/r^2 */
+    } else {
+        r0 = 1.0 / t[1];                    /* 1/sx */
+        r1 = 1.0 / t[2];                    /* 1/sy */
+        r2 = 1.0 / (1.0 - t[5] * t[5]);     /* 1/(1 - rho^2) */
+        r3 = 0.5 * r2 * is;                 /* 1/den */
+    }
     for (int k = 0; k < rows; ++k) {
         const int i = k % n, j = k / n;
         double row[P];
         if constexpr (P == 4) {
             const double A = t[0];
             const double ux = i - t[1], uy = j - t[2];
-            const double e = (double)fexp((float)(-(ux * ux + uy * uy) / (2.0 * w * w)));
-            row[0] = e / sigma;
-            row[1] = A * e * ux / (w * w * sigma);
-            row[2] = A * e * uy / (w * w * sigma);
-            row[3] = 1.0 / sigma;
+            const double e = (double)fexp((float)(-(ux * ux + uy * uy) * (0.5 * r0)));
+            const double ae = A * e * r0 * is;
+            row[0] = e * is;
+            row[1] = ae * ux;
+            row[2] = ae * uy;
+            row[3] = is;
         } else if constexpr (P == 5) {
-            const double A = t[0], r = t[1];
+            const double A = t[0];
             const double ux = i - t[2], uy = j - t[3];
             const double d2 = ux * ux + uy * uy;
-            const double e = (double)fexp((float)(-d2 / (2.0 * r * r)));
-            row[0] = e / sigma;
-            row[1] = A * e * d2 / (kSqrt2 * r * r * r * sigma);
-            row[2] = A * e * ux / (r * r * sigma);
-            row[3] = A * e * uy / (r * r * sigma);
-            row[4] = 1.0 / sigma;
+            const double e = (double)fexp((float)(-d2 * (0.5 * r1)));
+            const double ae = A * e * r1 * is;
+            row[0] = e * is;
+            row[1] = ae * d2 * r0 * (1.0 / kSqrt2);
+            row[2] = ae * ux;
+            row[3] = ae * uy;
+            row[4] = is;
         } else {
-            const double A = t[0], sx = t[1], sy = t[2], rho = t[5];
+            const double A = t[0], rho = t[5];
             const double ux = i - t[3], uy = j - t[4];
-            const double omr = 1.0 - rho * rho, den = 2.0 * omr * sigma;
-            const double q = ux * ux / (sx * sx) + uy * uy / (sy * sy) - 2.0 * rho * ux * uy / (sx * sy);
-            const double e = (double)fexp((float)(-q / (2.0 * omr)));
-            row[0] = e / sigma;
-            row[1] = -A * (2.0 * rho * ux * uy / (sx * sx * sy) - 2.0 * ux * ux / (sx * sx * sx)) * e / den;
-            row[2] = -A * (2.0 * rho * ux * uy / (sx * sy * sy) - 2.0 * uy * uy / (sy * sy * sy)) * e / den;
-            row[3] = -A * (2.0 * rho * uy / (sx * sy) - 2.0 * ux / (sx * sx)) * e / den;
-            row[4] = -A * (2.0 * rho * ux / (sx * sy) - 2.0 * uy / (sy * sy)) * e / den;
-            row[5] = A * (ux * uy / omr / (sx * sy) - rho * (ux * ux + uy * uy - 2.0 * rho * ux * uy) / (sx * sy) / omr / omr) * e / sigma;
-            row[6] = 1.0 / sigma;
+            const double vx = ux * r0, vy = uy * r1; /* ux/sx, uy/sy */
+            const double q = vx * vx + vy * vy - 2.0 * rho * vx * vy;
+            const double e = (double)fexp((float)(-q * (0.5 * r2)));
+            const double aed = A * e * r3;
+            row[0] = e * is;
+            row[1] = -aed * (2.0 * rho * vx * vy - 2.0 * vx * vx) * r0;
+            row[2] = -aed * (2.0 * rho * vx * vy - 2.0 * vy * vy) * r1;
+            row[3] = -aed * (2.0 * rho * vy - 2.0 * vx) * r0;
+            row[4] = -aed * (2.0 * rho * vx - 2.0 * vy) * r1;
+            /* the reference rho column divides the unscaled quadratic form by sx sy
+             * (exact only for sx = sy, source/Localization.cpp:1280) */
+            const double qr = (ux * ux + uy * uy) * r0 * r1 - 2.0 * rho * vx * vy;
+            row[5] = A * e * is * r2 * (vx * vy - rho * qr * r2);
+            row[6] = is;
         }
 #pragma unroll
         for (int c = 0; c < P; ++c) cn[c] += row[c] * row[c];
@@ -920,7 +961,7 @@ LZR_HD LZR_INLINE void propose(LaneState<P> &s) {
 }
 
 /* Start of one gsl_multifit_fdfsolver_iterate call of the reference loop
- * (source/Localization.cpp:148-155).  Returns true when a trial evaluation is needed;
+ * (source/Localization.cpp:148-155).  Returns true when a trial is to be proposed;
  * false when the loop ended (status in *st). */
 template <int P>
 LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int *st) {
@@ -939,7 +980,6 @@ LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int
     for (int i = 0; i < P; ++i) gmax = dmax(gmax, dabs((double)s.g[i] / s.diag[i]));
     s.gnorm = gmax / s.fnorm;
     s.inner = 0;
-    propose<P>(s);
     return true;
 }
 
@@ -963,11 +1003,10 @@ LZR_HD LZR_INLINE void store_final(LaneState<P> &s, int lm_status, int model, Lz
     s.phase = kIdle;
 }
 
-/* Consumes one pass.  Returns true if the lane needs another pass (a new trial), false
- * when the fit is complete (out written). */
+/* The bookkeeping of one pass; true when a new trial is to be proposed, otherwise false
+ * with the fit's final status in *st_out. */
 template <int P, typename Acc>
-LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int model,
-                                       LzrRawFit &out) {
+LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int *st_out) {
     const int npix = s.n * s.n;
     if (s.phase == kInit) {
         /* lmsder set(): f, J at the start point */
@@ -986,7 +1025,7 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
         s.iter_gsl = 1;
         int st = 0;
         if (begin_iterate<P>(s, cfg, &st)) return true;
-        store_final<P>(s, st, model, out);
+        *st_out = st;
         return false;
     }
     /* trial evaluated at th + dx */
@@ -1059,7 +1098,7 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
         if (st == LZR_LM_CONTINUE && s.iterations < cfg.max_iterations) {
             if (begin_iterate<P>(s, cfg, &st)) return true;
         }
-        store_final<P>(s, st, model, out);
+        *st_out = st;
         return false;
     }
     if (dabs(actred) <= kDblEpsilon && prered <= kDblEpsilon && 0.5 * ratio <= 1.0) {
@@ -1071,6 +1110,21 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
     } else if (s.inner >= 10) {
         st = LZR_LM_ENOPROG;
     } else {
+        return true;
+    }
+    *st_out = st;
+    return false;
+}
+
+/* Consumes one pass.  Returns true if the lane needs another pass (a new trial), false
+ * when the fit is complete (out written).  Every path that continues ends in the one
+ * call of propose (lmpar) at the bottom, so the lanes of a wave run lmpar together
+ * whichever branch each of them took. */
+template <int P, typename Acc>
+LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int model,
+                                       LzrRawFit &out) {
+    int st = LZR_LM_CONTINUE;
+    if (lane_step<P, Acc>(s, cfg, a, &st)) {
         propose<P>(s);
         return true;
     }
