@@ -385,6 +385,8 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent grid size per CU (0: resident max)")
     ap.add_argument("--rounds", type=int, default=0, help="fitting-kernel rounds (fits parked in between)")
     ap.add_argument("--park-threshold", type=int, default=-1, help="busy lanes at which a wave parks its fits")
+    ap.add_argument("--straggler-trials", type=int, default=-1,
+                    help="trials after which a fit moves to the 8-lane straggler kernel (0: off)")
     ap.add_argument("--no-net", action="store_true", help="skip the net-initialized LM leg")
     ap.add_argument("--only-net", action="store_true", help="run only the net-initialized LM leg (profiling)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-workload leg")
@@ -433,6 +435,8 @@ def main():
         ctx.set_option(abi.OPT_ROUNDS, args.rounds)
     if args.park_threshold >= 0:
         ctx.set_option(abi.OPT_PARK_THRESHOLD, args.park_threshold)
+    if args.straggler_trials >= 0:
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, args.straggler_trials)
     stream = torch.cuda.current_stream()
 
     def step():

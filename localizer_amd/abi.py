@@ -43,6 +43,7 @@ OPT_MAX_HALF_WIDTH = 6
 OPT_OCCUPANCY = 7
 OPT_ROUNDS = 8
 OPT_PARK_THRESHOLD = 9
+OPT_STRAGGLER_TRIALS = 10
 
 CANDIDATE_DTYPE = np.dtype(
     [

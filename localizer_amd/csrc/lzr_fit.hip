@@ -48,6 +48,11 @@ struct KernelArgs {
     unsigned *head;        /* queue head (device, zeroed per call) */
     LzrFit *out;
     LaneConfig cfg;
+    /* long fits: a fit still running after strag_trials trials is handed to the
+     * straggler kernel (0: never) through strag[] and strag_ctr = (count, head) */
+    int strag_trials;
+    void *strag;
+    unsigned *strag_ctr;
 };
 
 /* Queue class of a model: 0 symmetric (5 parameters), 1 ellipsoidal and the
@@ -346,9 +351,200 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         window_pass(q, n, nw, mixed, pix, xb, acc);
 
         /* ---- LM logic ---- */
+        bool hand_off = false;
         if (active) {
-            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, *reinterpret_cast<LzrRawFit *>(args.out + s.roi)))
+            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, reinterpret_cast<LzrRawFit *>(args.out + s.roi)))
                 s.phase = open ? kIdle : kFinished;
+            else
+                hand_off = args.strag_trials > 0 && s.trials >= args.strag_trials;
+        }
+        /* ---- hand long fits (a proposed trial pending) to the straggler kernel ---- */
+        const unsigned long long hmask = __ballot(hand_off);
+        if (hmask) {
+            const int leader = __ffsll((long long)hmask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(args.strag_ctr, (unsigned)__popcll(hmask));
+            base = __shfl(base, leader, 64);
+            if (hand_off) {
+                Parked<P> &pk = static_cast<Parked<P> *>(args.strag)[base + (unsigned)__popcll(hmask & below)];
+                pk.s = s;
+                pk.model = model;
+                s.phase = open ? kIdle : kFinished;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+/* ---- straggler kernel ------------------------------------------------------------
+ * Fits handed off after strag_trials trials are the long tail of a batch: one lane per
+ * fit would leave the end of the launch to a few lanes running ~200 more passes each.
+ * Here a fit is held by kGroup lanes: lane r of the group sums pixels p = r + kGroup m
+ * of the window, a butterfly over the group adds the partial sums (every lane of the
+ * group ends with the same bits), and all lanes of the group run the same LM logic on
+ * the same inputs.  The split is fixed, so a fit's result does not depend on which
+ * wave or group runs it. */
+constexpr int kGroup = 8;
+
+/* Sum over the kGroup = 8 lanes of a group (lanes 8g .. 8g+7) by DPP: quad swaps
+ * (1 0 3 2), (2 3 0 1), then the half-row mirror; FP add is commutative, so all eight
+ * lanes hold identical sums. */
+__device__ __forceinline__ double group_sum(double v) {
+    auto step = [](double x, int ctrl) {
+        const long long b = __double_as_longlong(x);
+        int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+        switch (ctrl) {
+        case 0:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+            break;
+        case 1:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+            break;
+        default:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false);
+            break;
+        }
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    };
+    v += step(v, 0);
+    v += step(v, 1);
+    v += step(v, 2);
+    return v;
+}
+
+template <int P, typename Acc>
+__device__ __forceinline__ void group_reduce(PassAcc<P, Acc> &a) {
+#pragma unroll
+    for (int i = 0; i < NP<P>::SQ; ++i) a.hq[i] = group_sum(a.hq[i]);
+#pragma unroll
+    for (int i = 0; i < P - 1; ++i) {
+        a.sj[i] = group_sum(a.sj[i]);
+        a.gq[i] = group_sum(a.gq[i]);
+    }
+    a.sf = group_sum(a.sf);
+    a.sse0 = group_sum(a.sse0);
+    a.dsse = group_sum(a.dsse);
+}
+
+template <int P, typename Acc, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
+lm_straggler(KernelArgs args) {
+    extern __shared__ double lds_words[];
+    /* LDS: per-group FP64 factors E_x(0..n-1), E_y(0..n-1) (slot t at row t/8, column
+     * lane), then the windows: pixel p of a group at row p/8, column 8g + p%8 */
+    const int nmax = 2 * args.cfg.max_half_width + 1;
+    double *xcol = lds_words;
+    uint16_t *tile = reinterpret_cast<uint16_t *>(lds_words + nmax * 64);
+    const int lane = threadIdx.x, grp = lane / kGroup, r = lane % kGroup;
+    const unsigned count = args.strag_ctr[0];
+    unsigned *head = args.strag_ctr + 1;
+    const Parked<P> *park = static_cast<const Parked<P> *>(args.strag);
+
+    LaneState<P> s;
+    s.phase = kIdle;
+    s.n = 1;
+    int model = 0;
+    bool open = true;
+    for (;;) {
+        /* ---- refill idle groups (one queue entry per group) ---- */
+        bool fresh = false;
+        if (open) {
+            const unsigned long long want = __ballot(r == 0 && s.phase == kIdle);
+            if (want) {
+                const int cnt = __popcll(want);
+                const int leader = __ffsll((long long)want) - 1;
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(head, (unsigned)cnt);
+                base = __shfl(base, leader, 64);
+                if (base + (unsigned)cnt >= count) open = false;
+                if ((want >> (grp * kGroup)) & 1ull) {
+                    const unsigned q = base + (unsigned)__popcll(want & ((1ull << (grp * kGroup)) - 1ull));
+                    if (q < count) {
+                        s = park[q].s;
+                        model = park[q].model;
+                        fresh = true;
+                    } else {
+                        s.phase = kFinished;
+                    }
+                }
+            }
+        }
+        if (!open && s.phase == kIdle) s.phase = kFinished;
+
+        /* ---- gather the windows of new fits: lane r loads pixels r, r + 8, ... ---- */
+        if (fresh) {
+            const int n = s.n, n2 = n * n, inv = 65536 / n + 1;
+            const uint16_t *src = args.frames + ((size_t)s.frame * args.ny + s.yoff) * args.nx + s.xoff;
+            for (int m0 = 0; m0 * kGroup < n2; m0 += 8) {
+                uint16_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int pp = r + kGroup * (m0 + u);
+                    const int j = (pp * inv) >> 16, i = pp - j * n;
+                    v[u] = pp < n2 ? src[(size_t)j * args.nx + i] : (uint16_t)0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int pp = r + kGroup * (m0 + u);
+                    if (pp < n2) tile[(m0 + u) * 64 + lane] = v[u];
+                }
+            }
+        }
+        __syncthreads();
+
+        const bool active = s.phase == kTrial;
+        if (!__ballot(active)) break;
+
+        PassIn<P> q;
+        PassAcc<P, Acc> acc;
+        acc.zero();
+        const int n = active ? s.n : 1;
+        if (active) {
+            lane_pass_in<P>(s, args.cfg, q);
+        } else {
+            double z[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) z[k] = 1.0;
+            make_pass_in(z, z, 1.0, 1.0, q);
+        }
+        const int n2 = n * n, inv = 65536 / n + 1;
+        if constexpr (P != 7) {
+            /* separable factors, slots t < n: exp(-c (t - x0)^2), n <= t < 2n: y */
+            for (int t = r; t < 2 * n; t += kGroup) {
+                const bool isx = t < n;
+                const double u = (double)((float)(isx ? t : t - n) - (isx ? q.x0 : q.y0));
+                xcol[(t / kGroup) * 64 + lane] = dexp(-q.c0d * u * u);
+            }
+        }
+        __syncthreads();
+        const int mmax = __builtin_amdgcn_readfirstlane(wave_max((n2 + kGroup - 1) / kGroup));
+        const double *gx = xcol + grp * kGroup;
+        for (int m = 0; m < mmax; ++m) {
+            const int pp = r + kGroup * m;
+            if (pp < n2) {
+                const int j = (pp * inv) >> 16, i = pp - j * n;
+                const float D = (float)tile[m * 64 + lane];
+                if constexpr (P == 7) {
+                    pixel_ell(q, (float)i, (float)j, D, acc);
+                } else {
+                    const int ty = n + j;
+                    const double ex = gx[(i / kGroup) * 64 + i % kGroup];
+                    const double ey = gx[(ty / kGroup) * 64 + ty % kGroup];
+                    const float uy0 = (float)j - q.y0;
+                    const float uy1 = uy0 - q.dyc;
+                    const float ddy = -q.dyc * (uy1 + uy0);
+                    pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, D, ex * ey, acc);
+                }
+            }
+        }
+        group_reduce<P, Acc>(acc);
+
+        if (active) {
+            LzrRawFit *o = r == 0 ? reinterpret_cast<LzrRawFit *>(args.out + s.roi) : nullptr;
+            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, o)) s.phase = open ? kIdle : kFinished;
         }
         __syncthreads();
     }
@@ -376,6 +572,10 @@ struct LzrContext {
     void *d_park[kClasses][2] = {}; /* [class][buffer] */
     size_t park_cap[kClasses][2] = {};
     unsigned *d_park_ctr = nullptr; /* [class][buffer] x (count, head) */
+    int strag_trials = 32;             /* hand a fit to the straggler kernel after this many trials */
+    uint8_t *d_strag = nullptr;        /* handed-off fits of the running class */
+    size_t strag_cap = 0;              /* bytes */
+    unsigned *d_strag_ctr = nullptr;   /* (count, head) */
     int num_cus = 0;
     std::string err;
     std::mutex mu;
@@ -449,7 +649,15 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
         r.out = ctx->d_park[cls][ob];
         r.out_count = ctr + 2 * ob;
         r.threshold = round == ctx->rounds - 1 ? 0 : ctx->park_threshold;
+        if (round == 0 && a.strag_trials > 0)
+            hipLaunchKernelGGL(reset_park, dim3(1), dim3(64), 0, stream, a.strag_ctr);
         hipLaunchKernelGGL((lm_persistent<P, Acc, F64, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
+    }
+    if (a.strag_trials > 0) {
+        int per_cu = 0;
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1>, 64, lds));
+        if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1>), dim3(per_cu * ctx->num_cus), dim3(64), lds, stream, a);
     }
 }
 
@@ -463,8 +671,18 @@ static size_t launch_lds(const LzrContext *ctx) {
 
 /* Allocates everything a launch of up to ncands candidates needs, so that the launch
  * itself allocates nothing (stream capture). */
+constexpr size_t kParkedMax = std::max(sizeof(Parked<4>), std::max(sizeof(Parked<5>), sizeof(Parked<7>)));
+
+static void ensure_strag(LzrContext *ctx, int ncands) {
+    if (ctx->strag_trials <= 0) return;
+    size_t cap = ctx->strag_cap;
+    ensure(ctx->d_strag, cap, (size_t)ncands * kParkedMax);
+    ctx->strag_cap = cap;
+}
+
 static void prepare(LzrContext *ctx, int ncands) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
+    ensure_strag(ctx, ncands);
     const size_t lds = launch_lds(ctx);
     if (ctx->occupancy5 == 1)
         ensure_park<5, AccT, kF64Residual, 1>(ctx, 0, lds);
@@ -480,6 +698,7 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
                    const LzrCandidate *cands, const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream,
                    const int *d_count = nullptr) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
+    ensure_strag(ctx, ncands);
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
     const int tpb = 256;
     const int nb = (ncands + tpb - 1) / tpb;
@@ -500,6 +719,9 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.head = ctx->d_counters + kClasses;
     a.out = out;
     a.cfg = ctx->cfg;
+    a.strag_trials = ctx->strag_trials;
+    a.strag = ctx->d_strag;
+    a.strag_ctr = ctx->d_strag_ctr;
     const size_t lds = launch_lds(ctx);
     if (ctx->occupancy5 == 1)
         run_class<5, AccT, kF64Residual, 1>(ctx, a, 0, lds, stream);
@@ -550,6 +772,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         ctx->num_cus = prop.multiProcessorCount;
         LZR_CHECK(hipMalloc((void **)&ctx->d_counters, kCounterWords * sizeof(unsigned)));
         LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 4 * kClasses * sizeof(unsigned)));
+        LZR_CHECK(hipMalloc((void **)&ctx->d_strag_ctr, 2 * sizeof(unsigned)));
     } catch (const std::string &e) {
         g_create_error = "lzr_create: " + e;
         lzr_destroy(ctx); /* frees whatever was created */
@@ -576,6 +799,8 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_order);
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_park_ctr);
+    (void)hipFree(ctx->d_strag_ctr);
+    (void)hipFree(ctx->d_strag);
     for (int c = 0; c < kClasses; ++c)
         for (int b = 0; b < 2; ++b) (void)hipFree(ctx->d_park[c][b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -628,6 +853,10 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
     case LZR_OPT_PARK_THRESHOLD:
         if (!(value >= 0 && value <= 64)) return fail(ctx, "park threshold must be in 0..64");
         ctx->park_threshold = (int)value;
+        return 0;
+    case LZR_OPT_STRAGGLER_TRIALS:
+        if (!(value >= 0 && value <= 100000)) return fail(ctx, "straggler trials must be >= 0");
+        ctx->strag_trials = (int)value;
         return 0;
     case LZR_OPT_MAX_HALF_WIDTH:
         if (!(value >= 1 && value <= LZR_MAX_HALF_WIDTH)) return fail(ctx, "max half-width out of range");
@@ -785,6 +1014,7 @@ LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value) {
     case LZR_OPT_OCCUPANCY: *value = ctx->occupancy5; return 0;
     case LZR_OPT_ROUNDS: *value = ctx->rounds; return 0;
     case LZR_OPT_PARK_THRESHOLD: *value = ctx->park_threshold; return 0;
+    case LZR_OPT_STRAGGLER_TRIALS: *value = ctx->strag_trials; return 0;
     default: return fail(ctx, "unknown option");
     }
 }

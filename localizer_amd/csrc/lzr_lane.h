@@ -986,20 +986,22 @@ LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int
 /* Stores the raw end state of the lane's fit (the reference's x, J^T J, |f| and loop
  * counters) for the epilogue. */
 template <int P>
-LZR_HD LZR_INLINE void store_final(LaneState<P> &s, int lm_status, int model, LzrRawFit &o) {
-    double x[P];
-    absolute<P>(s, s.th, x);
+LZR_HD LZR_INLINE void store_final(LaneState<P> &s, int lm_status, int model, LzrRawFit *o) {
+    if (o) { /* null: another lane holding the same fit writes the record */
+        double x[P];
+        absolute<P>(s, s.th, x);
 #pragma unroll
-    for (int i = 0; i < P; ++i) o.th[i] = x[i];
+        for (int i = 0; i < P; ++i) o->th[i] = x[i];
 #pragma unroll
-    for (int i = 0; i < NP<P>::S; ++i) o.H[i] = s.H[i];
-    o.chi = s.fnorm;
-    o.status = kPending;
-    o.lm_status = lm_status;
-    o.iterations = (int16_t)s.iterations;
-    o.trials = (int16_t)s.trials;
-    o.accepted_steps = (int16_t)s.accepted;
-    o.model = (int16_t)model;
+        for (int i = 0; i < NP<P>::S; ++i) o->H[i] = s.H[i];
+        o->chi = s.fnorm;
+        o->status = kPending;
+        o->lm_status = lm_status;
+        o->iterations = (int16_t)s.iterations;
+        o->trials = (int16_t)s.trials;
+        o->accepted_steps = (int16_t)s.accepted;
+        o->model = (int16_t)model;
+    }
     s.phase = kIdle;
 }
 
@@ -1117,12 +1119,12 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
 }
 
 /* Consumes one pass.  Returns true if the lane needs another pass (a new trial), false
- * when the fit is complete (out written).  Every path that continues ends in the one
+ * when the fit is complete (*out written unless out is null).  Every path that continues ends in the one
  * call of propose (lmpar) at the bottom, so the lanes of a wave run lmpar together
  * whichever branch each of them took. */
 template <int P, typename Acc>
 LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int model,
-                                       LzrRawFit &out) {
+                                       LzrRawFit *out) {
     int st = LZR_LM_CONTINUE;
     if (lane_step<P, Acc>(s, cfg, a, &st)) {
         propose<P>(s);

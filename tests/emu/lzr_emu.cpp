@@ -35,7 +35,7 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             double get(int i) const { return v[i]; }
         } xb;
         window_pass(q, n, n, false, pix, xb, a);
-        if (!lane_after_pass<P, Acc>(s, cfg, a, model, *reinterpret_cast<LzrRawFit *>(&out))) {
+        if (!lane_after_pass<P, Acc>(s, cfg, a, model, reinterpret_cast<LzrRawFit *>(&out))) {
             finish_any(out, cfg, c);
             return;
         }
