@@ -114,7 +114,7 @@ typedef struct LzrContext LzrContext;
 #define LZR_OPT_OCCUPANCY 7        /* register budget of the fitting kernel: 1 or 2 waves per SIMD (default 1) */
 #define LZR_OPT_ROUNDS 8           /* fitting-kernel launches per model class; unfinished fits are parked between them (default 1) */
 #define LZR_OPT_PARK_THRESHOLD 9   /* a wave parks its fits once the queue is drained and at most this many lanes are busy (default 16) */
-#define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues on 8 lanes in the straggler kernel (default 32, 0 = off) */
+#define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues on 16 lanes in the straggler kernel (default 48, 0 = off) */
 
 /* Creates a fitting context on the current HIP device.
  * Replaces the constructors FitPositions_SymmetricGaussian(initialPSFWidth, sigma)

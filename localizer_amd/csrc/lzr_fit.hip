@@ -30,6 +30,10 @@
 
 #include "lzr_lane.h"
 
+#ifndef LZR_STRAGGLER_GROUP
+#define LZR_STRAGGLER_GROUP 16 /* lanes per fit in the straggler kernel */
+#endif
+
 #define LZR_CHECK(call)                                                                      \
     do {                                                                                     \
         hipError_t e_ = (call);                                                              \
@@ -383,12 +387,14 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
  * of the window, a butterfly over the group adds the partial sums (every lane of the
  * group ends with the same bits), and all lanes of the group run the same LM logic on
  * the same inputs.  The split is fixed, so a fit's result does not depend on which
- * wave or group runs it. */
-constexpr int kGroup = 8;
+ * wave or group runs it.  (500k 9x9 symmetric fits: 7.90 ms without hand-off, 6.72 ms
+ * with 8-lane groups after 32 trials, 6.48 ms with 16-lane groups after 48.) */
+constexpr int kGroup = LZR_STRAGGLER_GROUP;
+static_assert(kGroup == 8 || kGroup == 16, "group sums are written for 8 or 16 lanes");
 
-/* Sum over the kGroup = 8 lanes of a group (lanes 8g .. 8g+7) by DPP: quad swaps
- * (1 0 3 2), (2 3 0 1), then the half-row mirror; FP add is commutative, so all eight
- * lanes hold identical sums. */
+/* Sum over the kGroup lanes of a group by DPP: quad swaps (1 0 3 2), (2 3 0 1), the
+ * half-row mirror (8 lanes) and, for 16 lanes, the row mirror; FP add is commutative,
+ * so every lane of the group holds identical sums. */
 __device__ __forceinline__ double group_sum(double v) {
     auto step = [](double x, int ctrl) {
         const long long b = __double_as_longlong(x);
@@ -402,9 +408,13 @@ __device__ __forceinline__ double group_sum(double v) {
             lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
             hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
             break;
-        default:
+        case 2:
             lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);
             hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false);
+            break;
+        default:
+            lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
+            hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false);
             break;
         }
         return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
@@ -412,6 +422,7 @@ __device__ __forceinline__ double group_sum(double v) {
     v += step(v, 0);
     v += step(v, 1);
     v += step(v, 2);
+    if constexpr (kGroup == 16) v += step(v, 3); /* row mirror: lanes i and 15 - i */
     return v;
 }
 
@@ -572,10 +583,13 @@ struct LzrContext {
     void *d_park[kClasses][2] = {}; /* [class][buffer] */
     size_t park_cap[kClasses][2] = {};
     unsigned *d_park_ctr = nullptr; /* [class][buffer] x (count, head) */
-    int strag_trials = 32;             /* hand a fit to the straggler kernel after this many trials */
+    int strag_trials = 48;             /* hand a fit to the straggler kernel after this many trials */
     uint8_t *d_strag = nullptr;        /* handed-off fits of the running class */
     size_t strag_cap = 0;              /* bytes */
     unsigned *d_strag_ctr = nullptr;   /* (count, head) */
+    hipEvent_t done = nullptr;         /* end of the last call's kernels */
+    bool done_recorded = false;
+    hipStream_t done_stream = nullptr;
     int num_cus = 0;
     std::string err;
     std::mutex mu;
@@ -699,6 +713,16 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
                    const int *d_count = nullptr) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
     ensure_strag(ctx, ncands);
+    /* The context's queue, counters and hand-off buffers are reused by every call: a call
+     * on another stream waits for the previous call's kernels (not inside a graph
+     * capture, whose replays the caller orders). */
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    LZR_CHECK(hipStreamIsCapturing(stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    if (!capturing) {
+        if (!ctx->done) LZR_CHECK(hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming));
+        if (ctx->done_recorded && ctx->done_stream != stream) LZR_CHECK(hipStreamWaitEvent(stream, ctx->done, 0));
+    }
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
     const int tpb = 256;
     const int nb = (ncands + tpb - 1) / tpb;
@@ -732,6 +756,11 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, d_count,
                        out, ctx->cfg);
     LZR_CHECK(hipGetLastError());
+    if (!capturing) {
+        LZR_CHECK(hipEventRecord(ctx->done, stream));
+        ctx->done_recorded = true;
+        ctx->done_stream = stream;
+    }
 }
 
 static int fail(LzrContext *ctx, const std::string &msg) {
@@ -800,6 +829,7 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_park_ctr);
     (void)hipFree(ctx->d_strag_ctr);
+    if (ctx->done) (void)hipEventDestroy(ctx->done);
     (void)hipFree(ctx->d_strag);
     for (int c = 0; c < kClasses; ++c)
         for (int b = 0; b < 2; ++b) (void)hipFree(ctx->d_park[c][b]);
