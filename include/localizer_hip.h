@@ -319,4 +319,56 @@ LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const 
                                             int ncandidates, LzrFit *d_results, LzrStart *d_work,
                                             double residual_factor, int refit_rejected, void *stream);
 
+/* ---- Localization bitmap ------------------------------------------------------
+ * PALMBitmapImageCalculator::CalculateImage(positions, xSize, ySize, outputScaleFactor)
+ * (source/PALMImages.cpp:68-157): every localization adds a Gaussian, integrated over
+ * each output pixel with the reference's erf lookup table (source/PALMImages.cpp:34-66),
+ * to a (int)(xSize*scale) x (int)(ySize*scale) FP64 image (x fast, the Eigen column-major
+ * Image of the reference), pixels accumulated in position order (the reference's sums,
+ * bit for bit).  The positions are the container getters the calculator reads
+ * (getXPosition, getYPosition, getIntegral, getXPositionDeviation, getBackground). */
+typedef struct LzrPosition {
+    double x, y;         /* px                                            */
+    double integral;     /* fitted integral                               */
+    double x_deviation;  /* fit uncertainty of x                          */
+    double background;   /* fitted offset                                 */
+} LzrPosition;
+
+/* Deviation of each Gaussian (PALMBITMAP_DEVIATION_*, source/Defines.h:113-115). */
+#define LZR_DEVIATION_CONSTANT 0        /* _Constant(p[0] = deviation), PALMImages.h:122-133          */
+#define LZR_DEVIATION_FIT_UNCERTAINTY 1 /* _FitUncertainty(p[0] = scale, p[1] = upperLimit), :106-120  */
+#define LZR_DEVIATION_GAUSSIAN_MASK 2   /* _GaussianMask(p[0] = PSFWidth, p[1] = cameraOffset,
+                                           p[2] = cameraMultiplicationFactor), :138-157, PALMImages.cpp:159-177 */
+/* Amplitude of each Gaussian (PALMBITMAP_EMITTERWEIGHING_*, source/Defines.h:110-111). */
+#define LZR_WEIGHING_SAME 0
+#define LZR_WEIGHING_INTEGRAL 1 /* the fitted integral; positions with integral 0 are skipped */
+
+typedef struct LzrRenderer LzrRenderer;
+
+/* Renderer with the erf table on the current device (PALMBitmapImageCalculator
+ * constructor, source/PALMImages.h:70-74).  Returns 0 or -1. */
+LZR_EXPORT int lzr_renderer_create(LzrRenderer **renderer);
+LZR_EXPORT int lzr_renderer_destroy(LzrRenderer *renderer);
+/* Error text of the last failing call (renderer NULL: of lzr_renderer_create). */
+LZR_EXPORT int lzr_renderer_last_error(const LzrRenderer *renderer, char *buf, int len);
+
+/* Output size (int)(x_size*output_scale) x (int)(y_size*output_scale); -1 when
+ * output_scale <= 0 (the reference throws) or the image would be smaller than 2x2. */
+LZR_EXPORT int lzr_render_image_size(int x_size, int y_size, double output_scale, int *width, int *height);
+
+/* CalculateImage on the device: d_image receives width*height doubles (x fast), every
+ * pixel written.  deviation_params holds 3 doubles (unused ones ignored).  Returns 0, or
+ * -1 with the reference's error text (unknown weighing, positions without integral /
+ * background for the Gaussian mask, output_scale <= 0).  Synchronizes the stream once
+ * (the number of (tile, position) pairs sizes the sort). */
+LZR_EXPORT int lzr_render_device(LzrRenderer *renderer, const LzrPosition *d_positions, int n_positions,
+                                 int x_size, int y_size, double output_scale, int deviation_method,
+                                 const double *deviation_params, int weighing, double *d_image, void *stream);
+
+/* The accepted fits of a result array as positions, in order (the container the
+ * reference fitters return holds exactly these, source/Localization.cpp:190-212).
+ * *d_count (device) receives the number written. */
+LZR_EXPORT int lzr_positions_from_fits_device(LzrRenderer *renderer, const LzrFit *d_fits, int n_fits,
+                                              LzrPosition *d_positions, int *d_count, void *stream);
+
 #endif /* LOCALIZER_HIP_H */

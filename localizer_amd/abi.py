@@ -132,3 +132,13 @@ def default_half_width(initial_psf_width):
     """The reference window: cutoff_radius = ceil(4 * initialPSFWidth)
     (source/Localization.h:68)."""
     return int(np.ceil(initial_psf_width * 4.0))
+
+# Localization bitmap (include/localizer_hip.h, LzrPosition)
+POSITION_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("integral", "<f8"), ("x_deviation", "<f8"),
+                           ("background", "<f8")])
+assert POSITION_DTYPE.itemsize == 40
+DEVIATION_CONSTANT = 0
+DEVIATION_FIT_UNCERTAINTY = 1
+DEVIATION_GAUSSIAN_MASK = 2
+WEIGHING_SAME = 0
+WEIGHING_INTEGRAL = 1

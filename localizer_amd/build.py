@@ -68,7 +68,8 @@ def build_hip():
     out = os.path.join(LIB, "liblocalizer_hip.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "lzr_fit.hip"), os.path.join(CSRC, "lzr_net.hip"),
-          os.path.join(CSRC, "lzr_find.hip"), os.path.join(CSRC, "lzr_pipeline.hip")])
+          os.path.join(CSRC, "lzr_find.hip"), os.path.join(CSRC, "lzr_pipeline.hip"),
+          os.path.join(CSRC, "lzr_render.hip")])
     os.replace(tmp, out)
 
 

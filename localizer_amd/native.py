@@ -45,6 +45,12 @@ EXPORTED = (
     "lzr_localize",
     "lzr_free",
     "lzr_pipeline_last_error",
+    "lzr_renderer_create",
+    "lzr_renderer_destroy",
+    "lzr_renderer_last_error",
+    "lzr_render_image_size",
+    "lzr_render_device",
+    "lzr_positions_from_fits_device",
 )
 
 
@@ -110,7 +116,8 @@ def lib():
         L.lzr_pipeline_last_error.argtypes = [ctypes.c_char_p, i]
         L.lzr_pipeline_last_error.restype = ctypes.c_int
         for name in EXPORTED:
-            if name.startswith(("lzr_net", "lzr_fit_batch_device_", "lzr_graph", "lzr_get", "lzr_autotune", "lzr_find")):
+            if name.startswith(("lzr_net", "lzr_fit_batch_device_", "lzr_graph", "lzr_get", "lzr_autotune", "lzr_find",
+                                "lzr_render", "lzr_positions")):
                 getattr(L, name).restype = ctypes.c_int
         if L.lzr_abi_version() != 1:
             raise LocalizerError("ABI version mismatch")
@@ -367,3 +374,54 @@ def localize(frames, psf_width, glrt_insensitivity, model=abi.MODEL_ELLIPSOIDAL)
     finally:
         lib().lzr_free(ptr)
     return flat[: n.value * cols.value].reshape(cols.value, n.value).T.copy()
+
+
+def _render_error(h):
+    buf = ctypes.create_string_buffer(1024)
+    lib().lzr_renderer_last_error(h, buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def render_image_size(x_size, y_size, output_scale):
+    """(width, height) of the localization bitmap (lzr_render_image_size)."""
+    w, h = ctypes.c_int(), ctypes.c_int()
+    if lib().lzr_render_image_size(int(x_size), int(y_size), ctypes.c_double(output_scale), ctypes.byref(w),
+                                   ctypes.byref(h)) != 0:
+        raise LocalizerError("output scale must be > 0 and the image at least 2x2 pixels")
+    return w.value, h.value
+
+
+class Renderer:
+    """Localization bitmap on the device (lzr_renderer_*, PALMBitmapImageCalculator)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        if lib().lzr_renderer_create(ctypes.byref(h)) != 0:
+            raise LocalizerError(_render_error(None))
+        self._h = h
+        self._destroy = lib().lzr_renderer_destroy
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def render_device(self, d_positions, n, x_size, y_size, output_scale, method, params, weighing, d_image, stream=0):
+        p = (ctypes.c_double * 3)(*(list(params) + [0.0, 0.0, 0.0])[:3])
+        if lib().lzr_render_device(self._h, ctypes.c_void_p(d_positions), int(n), int(x_size), int(y_size),
+                                   ctypes.c_double(output_scale), int(method), p, int(weighing),
+                                   ctypes.c_void_p(d_image), ctypes.c_void_p(stream)) != 0:
+            raise LocalizerError(_render_error(self._h))
+
+    def positions_from_fits_device(self, d_fits, n, d_positions, d_count, stream=0):
+        if lib().lzr_positions_from_fits_device(self._h, ctypes.c_void_p(d_fits), int(n), ctypes.c_void_p(d_positions),
+                                                ctypes.c_void_p(d_count), ctypes.c_void_p(stream)) != 0:
+            raise LocalizerError(_render_error(self._h))

@@ -119,6 +119,14 @@ int orc_glrt_mask(const uint16_t *frame, int nx, int ny, double width, double pf
  * number of particles or -1. */
 int orc_find_particles(const uint16_t *frame, const uint8_t *mask, int nx, int ny, double *out, int max_out);
 
+/* ---- localization bitmap (lzr_render_oracle.c) ---- */
+int orc_erf_table_size(void);
+void orc_erf_table(double *table);
+double orc_erf_lookup(const double *table, double x);
+int orc_render_size(int x_size, int y_size, double scale, int *width, int *height);
+int orc_render(const double *pos, int n, int x_size, int y_size, double scale, int method, const double *p,
+               int weighing, double *image);
+
 int orc_fit_batch(const OrcSettings *s, const uint16_t *frames, int nx, int ny, int nframes,
                   const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads);
 

@@ -83,6 +83,15 @@ def lib():
         L.orc_find_particles.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_int]
         L.orc_find_particles.restype = ctypes.c_int
+        L.orc_erf_table_size.restype = ctypes.c_int
+        L.orc_erf_table.argtypes = [ctypes.c_void_p]
+        L.orc_erf_lookup.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_erf_lookup.restype = ctypes.c_double
+        L.orc_render_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_render_size.restype = ctypes.c_int
+        L.orc_render.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_render.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -236,3 +245,38 @@ def find_particles(frame, mask):
         if n <= cap:
             return out[:n]
         cap = n
+
+
+# ---- localization bitmap (lzr_render_oracle.c) ----
+
+def erf_table():
+    """The reference's ERFLookupTable values (plus the weight-0 guard entry)."""
+    n = lib().orc_erf_table_size()
+    t = np.zeros(n + 1)
+    lib().orc_erf_table(t.ctypes.data)
+    return t
+
+
+def erf_lookup(table, x):
+    return lib().orc_erf_lookup(table.ctypes.data, float(x))
+
+
+def render_size(x_size, y_size, scale):
+    w, h = ctypes.c_int(), ctypes.c_int()
+    if lib().orc_render_size(int(x_size), int(y_size), float(scale), ctypes.byref(w), ctypes.byref(h)) != 0:
+        raise ValueError("bad render size")
+    return w.value, h.value
+
+
+def render(positions, x_size, y_size, scale, method, params, weighing):
+    """PALMBitmapImageCalculator::CalculateImage.  positions: [n, 5] (x, y, integral,
+    x_deviation, background).  Returns the image [height, width] (x fast)."""
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 5)
+    w, h = render_size(x_size, y_size, scale)
+    img = np.zeros((h, w))
+    p = np.zeros(3)
+    p[: len(params)] = params
+    if lib().orc_render(pos.ctypes.data, len(pos), int(x_size), int(y_size), float(scale), int(method),
+                        p.ctypes.data, int(weighing), img.ctypes.data) != 0:
+        raise ValueError("render failed")
+    return img
