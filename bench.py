@@ -285,6 +285,43 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
 DENSE_PFA = 20.0
 
 
+RENDER_SCALE = 10.0
+RENDER_DEVIATION = 0.15
+
+
+def render_leg(d_fits, nfits, w, log, reps=5):
+    """Localization bitmap of the plain leg's accepted fits: positions_from_fits on the
+    device, then the bitmap at 10x (constant deviation 0.15 px, integral weighing)."""
+    import torch
+
+    d_pos = device.empty(abi.POSITION_DTYPE, nfits)
+    d_count = device.empty(np.int32, 1)
+    width, height = native.render_image_size(w.nx, w.ny, RENDER_SCALE)
+    d_img = device.empty(np.float64, width * height)
+    s = torch.cuda.current_stream()
+    with native.Renderer() as r:
+        r.positions_from_fits_device(d_fits.data_ptr(), nfits, d_pos.ptr, d_count.ptr, s.cuda_stream)
+        n = int(d_count.numpy()[0])
+
+        def once():
+            r.render_device(d_pos.ptr, n, w.nx, w.ny, RENDER_SCALE, abi.DEVIATION_CONSTANT, [RENDER_DEVIATION],
+                            abi.WEIGHING_INTEGRAL, d_img.ptr, s.cuda_stream)
+
+        once()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            once()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+    box = 2 * int(np.ceil(4 * RENDER_DEVIATION * (width - 1) / (w.nx - 1))) + 1
+    pairs = n * box * box
+    log(f"render: {n} positions -> {width}x{height} image in {ms:.3f} ms ({n / ms * 1e3:.3e} positions/s, "
+        f"~{pairs / ms * 1e3:.3e} pixel-position terms/s)")
+    return {"positions": n, "image": f"{width}x{height}", "scale": RENDER_SCALE, "deviation": RENDER_DEVIATION,
+            "ms": round(ms, 3), "positions_per_s": n / ms * 1e3, "pixel_terms_per_s_approx": pairs / ms * 1e3}
+
+
 def dense_leg(nframes, block_frames, latency_frames, log):
     """Dense end-to-end stream: frames -> GLRT segmentation -> 8-way particle finding ->
     11x11 patch gather -> CNN start -> LM with residual fallback, on the device; frames
@@ -392,6 +429,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-workload leg")
     ap.add_argument("--block-frames", type=int, default=10000, help="frames per graph-replayed block (mixed leg)")
     ap.add_argument("--no-dense", action="store_true", help="skip the dense end-to-end stream leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the localization-bitmap leg")
     ap.add_argument("--dense-frames", type=int, default=1000)
     ap.add_argument("--dense-block", type=int, default=50, help="frames per block of the dense stream")
     ap.add_argument("--net-train-rois", type=int, default=4 << 20)
@@ -469,6 +507,9 @@ def main():
           f"{achieved:.2f} TFLOP/s algorithmic, mean trials {fits['trials'].mean():.2f}",
           file=sys.stderr, flush=True)
 
+    render = None
+    if not args.no_render:
+        render = render_leg(d_out, cands.size, w, lambda m: print(m, file=sys.stderr, flush=True))
     del d_frames, d_cands, d_out
     torch.cuda.empty_cache()
     mixed = None
@@ -529,6 +570,7 @@ def main():
         "cpu_baseline": cpu,
         "net_init": net,
         "mixed": mixed,
+        "render": render,
         "dense": dense,
     }
     print(json.dumps(line), flush=True)

@@ -365,6 +365,12 @@ LZR_EXPORT int lzr_render_device(LzrRenderer *renderer, const LzrPosition *d_pos
                                  int x_size, int y_size, double output_scale, int deviation_method,
                                  const double *deviation_params, int weighing, double *d_image, void *stream);
 
+/* The same on host buffers: positions in host memory, image (width*height doubles)
+ * written in host memory; blocks until done. */
+LZR_EXPORT int lzr_render(LzrRenderer *renderer, const LzrPosition *positions, int n_positions, int x_size,
+                          int y_size, double output_scale, int deviation_method, const double *deviation_params,
+                          int weighing, double *image);
+
 /* The accepted fits of a result array as positions, in order (the container the
  * reference fitters return holds exactly these, source/Localization.cpp:190-212).
  * *d_count (device) receives the number written. */

@@ -36,7 +36,7 @@
         if (e_ != hipSuccess) throw std::string(#call ": ") + hipGetErrorString(e_);        \
     } while (0)
 
-namespace lzr_render {
+namespace lzr_rnd {
 
 constexpr double kErfLower = -4.0; /* source/PALMImages.cpp:35-37 */
 constexpr double kErfUpper = 4.0;
@@ -216,9 +216,9 @@ void grow(T *&p, size_t &cap, size_t need) {
     cap = need;
 }
 
-}  // namespace lzr_render
+}  // namespace lzr_rnd
 
-using namespace lzr_render;
+using namespace lzr_rnd;
 
 struct LzrRenderer {
     std::string err;
@@ -363,9 +363,9 @@ LZR_EXPORT int lzr_render_device(LzrRenderer *r, const LzrPosition *d_positions,
         RND_CHECK(hipMemcpyAsync(&total, r->d_offsets + n, sizeof(long long), hipMemcpyDeviceToHost, s));
         RND_CHECK(hipMemcpyAsync(&error, r->d_error, sizeof(int), hipMemcpyDeviceToHost, s));
         RND_CHECK(hipStreamSynchronize(s));
-        if (error)
-            throw std::string("the used positions do not provide a PSF width and/or background estimate "
-                              "(use a different localization algorithm)");
+        if (error) /* the reference's text (source/PALMImages.cpp:164) */
+            return rfail(r, "the used positions do not provide a PSF width and/or background estimate "
+                            "(use a different localization algorithm)");
         if (total < 0 || total >= (long long)INT_MAX) throw std::string("too many (tile, position) pairs");
         const int M = (int)total;
         if (M > 0) {
@@ -391,6 +391,40 @@ LZR_EXPORT int lzr_render_device(LzrRenderer *r, const LzrPosition *d_positions,
         return rfail(r, "lzr_render_device: " + e);
     }
     return 0;
+}
+
+LZR_EXPORT int lzr_render(LzrRenderer *r, const LzrPosition *positions, int n_positions, int x_size, int y_size,
+                          double output_scale, int deviation_method, const double *deviation_params, int weighing,
+                          double *image) {
+    if (!r) return -1;
+    int W = 0, H = 0;
+    if (lzr_render_image_size(x_size, y_size, output_scale, &W, &H) != 0)
+        return rfail(r, "lzr_render: output_scale must be > 0 and the image at least 2x2 pixels");
+    if (n_positions < 0 || (n_positions > 0 && !positions) || !image)
+        return rfail(r, "lzr_render: null pointer or negative count");
+    LzrPosition *d_pos = nullptr;
+    double *d_img = nullptr;
+    hipStream_t s = nullptr;
+    int rc = -1;
+    try {
+        RND_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        RND_CHECK(hipMalloc((void **)&d_pos, std::max(n_positions, 1) * sizeof(LzrPosition)));
+        RND_CHECK(hipMalloc((void **)&d_img, (size_t)W * H * sizeof(double)));
+        if (n_positions > 0)
+            RND_CHECK(hipMemcpyAsync(d_pos, positions, n_positions * sizeof(LzrPosition), hipMemcpyHostToDevice, s));
+        if (lzr_render_device(r, d_pos, n_positions, x_size, y_size, output_scale, deviation_method, deviation_params,
+                              weighing, d_img, s) == 0) {
+            RND_CHECK(hipMemcpyAsync(image, d_img, (size_t)W * H * sizeof(double), hipMemcpyDeviceToHost, s));
+            RND_CHECK(hipStreamSynchronize(s));
+            rc = 0;
+        }
+    } catch (const std::string &e) {
+        rfail(r, "lzr_render: " + e);
+    }
+    if (d_pos) (void)hipFree(d_pos);
+    if (d_img) (void)hipFree(d_img);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
 }
 
 LZR_EXPORT int lzr_positions_from_fits_device(LzrRenderer *r, const LzrFit *d_fits, int n_fits,

@@ -131,8 +131,11 @@ public:
     virtual ~LocalizedPositionsContainer() {}
     virtual size_t getNPositions() const = 0;
     virtual size_t getFrameNumber(size_t index) const = 0;
+    virtual double getIntegral(size_t index) const { return 0; }
     virtual double getXPosition(size_t index) const { return 0; }
     virtual double getYPosition(size_t index) const { return 0; }
+    virtual double getBackground(size_t index) const { return 0; }
+    virtual double getXPositionDeviation(size_t index) const { return 0; }
     virtual void addPosition(std::shared_ptr<LocalizedPosition> newPosition) = 0;
     virtual void addPositions(std::shared_ptr<LocalizedPositionsContainer> other) = 0;
     virtual void setFrameNumbers(size_t frameNumber) = 0;
@@ -152,6 +155,7 @@ public:
     double getYPosition(size_t index) const { return positionsVector.at(index).yPosition; }
     double getBackground(size_t index) const { return positionsVector.at(index).background; }
     double getIntegral(size_t index) const { return positionsVector.at(index).integral; }
+    double getXPositionDeviation(size_t index) const { return positionsVector.at(index).xPositionDeviation; }
     const Pos &at(size_t index) const { return positionsVector.at(index); }
 
     void addPosition(std::shared_ptr<LocalizedPosition> p) {

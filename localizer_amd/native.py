@@ -50,6 +50,7 @@ EXPORTED = (
     "lzr_renderer_last_error",
     "lzr_render_image_size",
     "lzr_render_device",
+    "lzr_render",
     "lzr_positions_from_fits_device",
 )
 
@@ -420,6 +421,17 @@ class Renderer:
                                    ctypes.c_double(output_scale), int(method), p, int(weighing),
                                    ctypes.c_void_p(d_image), ctypes.c_void_p(stream)) != 0:
             raise LocalizerError(_render_error(self._h))
+
+    def render(self, positions, x_size, y_size, output_scale, method, params, weighing):
+        """Host positions (abi.POSITION_DTYPE) -> image [height, width] (lzr_render)."""
+        pos = np.ascontiguousarray(positions, abi.POSITION_DTYPE)
+        w, h = render_image_size(x_size, y_size, output_scale)
+        img = np.zeros((h, w))
+        p = (ctypes.c_double * 3)(*(list(params) + [0.0, 0.0, 0.0])[:3])
+        if lib().lzr_render(self._h, pos.ctypes.data if pos.size else None, int(pos.size), int(x_size), int(y_size),
+                            ctypes.c_double(output_scale), int(method), p, int(weighing), img.ctypes.data) != 0:
+            raise LocalizerError(_render_error(self._h))
+        return img
 
     def positions_from_fits_device(self, d_fits, n, d_positions, d_count, stream=0):
         if lib().lzr_positions_from_fits_device(self._h, ctypes.c_void_p(d_fits), int(n), ctypes.c_void_p(d_positions),

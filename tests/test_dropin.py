@@ -14,6 +14,7 @@ from oracle import pyoracle as po
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "cpp", "dropin_main")
+RENDER_DRIVER = os.path.join(ROOT, "tests", "cpp", "render_main")
 REF = "/root/reference"
 
 
@@ -30,8 +31,26 @@ def test_dropin_compiles_against_reference_headers():
     subprocess.run(cmd, input=src.encode(), check=True)
 
 
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "source")), reason="reference sources not present")
+def test_render_dropin_compiles_against_reference_headers():
+    """PALMBitmapImageCalculatorHIP reads the reference's LocalizedPositionsContainer
+    through its virtual getters and builds the reference's Image."""
+    src = ('#include "%s/localizer_amd/dropin/PALMImagesHIP.h"\n'
+           "int main() { std::shared_ptr<PALMBitmapImageDeviationCalculatorHIP> d(\n"
+           "    new PALMBitmapImageDeviationCalculator_GaussianMaskHIP(1.3, 100.0, 2.0));\n"
+           "  std::shared_ptr<ProgressReporter> pr(new ProgressReporter_Silent());\n"
+           "  PALMBitmapImageCalculatorHIP c(d, PALMBITMAP_EMITTERWEIGHING_INTEGRAL, pr);\n"
+           "  std::shared_ptr<LocalizedPositionsContainer> p(new LocalizedPositionsContainer_2DGauss());\n"
+           "  ImagePtr im = c.CalculateImage(p, 64, 64, 10.0); return (int)im->rows(); }\n" % ROOT)
+    cmd = ["g++", "-std=c++11", "-fsyntax-only", "-DLZR_WITH_LOCALIZER_HEADERS", "-include", "memory", "-include",
+           "cstring", "-DHAVE_INLINE", f"-I{REF}/include", f"-I{REF}/include/eigen3", f"-I{REF}/source",
+           f"-I{REF}/libsource/LNBTIFF", "-x", "c++", "-"]
+    subprocess.run(cmd, input=src.encode(), check=True)
+
+
 def test_dropin_driver_is_built():
     assert os.path.exists(DRIVER), "build with `python -m localizer_amd.build`"
+    assert os.path.exists(RENDER_DRIVER), "build with `python -m localizer_amd.build`"
 
 
 def _write_input(path, frames, cands):
@@ -102,3 +121,35 @@ def test_dropin_matches_c_abi_and_oracle(tmp_path, model, mode):
     print(f"drop-in model={model}: status agreement {agree:.4f}, xy within 1e-3: {np.mean(dxy <= 1e-3):.4f}")
     assert agree >= 0.99
     assert np.mean(dxy <= 1e-3) >= 0.98
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,params,weighing", [(0, (0.2, 0, 0), 1), (1, (4.0, 0.08, 0), 0), (2, (1.3, 100.0, 2.0), 1)])
+def test_render_dropin_matches_oracle(tmp_path, method, params, weighing):
+    """PALMBitmapImageCalculatorHIP::CalculateImage on a 2DGauss container equals the
+    oracle's image bit for bit; a Gaussian-mask run on positions without integral gives
+    the reference's error text."""
+    print(f"case: render drop-in method {method} weighing {weighing}", flush=True)
+    rng = np.random.default_rng(20 + method)
+    n = 700
+    rows = np.c_[rng.uniform(0, 47, n), rng.uniform(0, 35, n), rng.uniform(50, 3000, n), rng.uniform(0.005, 0.1, n),
+                 rng.uniform(90, 150, n)]
+    inp, out = str(tmp_path / "pos.bin"), str(tmp_path / "img.bin")
+    with open(inp, "wb") as f:
+        f.write(struct.pack("<i", n))
+        f.write(rows.astype("<f8").tobytes())
+    args = [RENDER_DRIVER, str(method), *map(str, params), str(weighing), "48", "36", "6.5", inp, out]
+    subprocess.run(args, check=True, timeout=120)
+    raw = open(out, "rb").read()
+    r, c = struct.unpack_from("<ii", raw, 0)
+    img = np.frombuffer(raw, "<f8", r * c, 8).reshape(c, r)  # column-major rows=x: [y][x]
+    want = po.render(rows, 48, 36, 6.5, method, list(params), weighing)
+    assert img.shape == want.shape and np.array_equal(img, want)
+    if method == 2:
+        rows[3, 2] = -1.0
+        with open(inp, "wb") as f:
+            f.write(struct.pack("<i", n))
+            f.write(rows.astype("<f8").tobytes())
+        res = subprocess.run(args, capture_output=True, timeout=120)
+        assert res.returncode == 3
+        assert b"do not provide a PSF width and/or background estimate" in res.stderr
