@@ -383,19 +383,24 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
 /* ---- straggler kernel ------------------------------------------------------------
  * Fits handed off after strag_trials trials are the long tail of a batch: one lane per
  * fit would leave the end of the launch to a few lanes running ~200 more passes each.
- * Here a fit is held by kGroup lanes: lane r of the group sums pixels p = r + kGroup m
+ * Here a fit is held by a group of k lanes: lane r of the group sums pixels p = r + k m
  * of the window, a butterfly over the group adds the partial sums (every lane of the
  * group ends with the same bits), and all lanes of the group run the same LM logic on
- * the same inputs.  The split is fixed, so a fit's result does not depend on which
- * wave or group runs it.  (500k 9x9 symmetric fits: 7.90 ms without hand-off, 6.72 ms
- * with 8-lane groups after 32 trials, 6.48 ms with 16-lane groups after 48.) */
-constexpr int kGroup = LZR_STRAGGLER_GROUP;
-static_assert(kGroup == 8 || kGroup == 16, "group sums are written for 8 or 16 lanes");
+ * the same inputs.  k is the largest power of two <= kMaxGroup for which the handed-off
+ * fits need at most two groups' turns on the launch's lanes: a function of the hand-off
+ * count, which the
+ * fits alone determine, so a fit's result does not depend on which wave or group runs
+ * it or on timing.  When many fits are long (dim data), k falls to 1: one lane per fit,
+ * the same pixel order and arithmetic as the first kernel.  (500k 9x9 symmetric fits:
+ * 7.90 ms without hand-off, 6.72 ms with 8-lane groups after 32 trials, 6.48 ms with
+ * 16-lane groups after 48.) */
+constexpr int kMaxGroup = LZR_STRAGGLER_GROUP;
+static_assert(kMaxGroup == 8 || kMaxGroup == 16, "group sums are written for up to 16 lanes");
 
-/* Sum over the kGroup lanes of a group by DPP: quad swaps (1 0 3 2), (2 3 0 1), the
- * half-row mirror (8 lanes) and, for 16 lanes, the row mirror; FP add is commutative,
- * so every lane of the group holds identical sums. */
-__device__ __forceinline__ double group_sum(double v) {
+/* Sum over the k lanes of a group by DPP (k a power of two <= 16): quad swaps
+ * (1 0 3 2) for 2 lanes, (2 3 0 1) for 4, the half-row mirror for 8, the row mirror
+ * for 16; FP add is commutative, so every lane of the group holds identical sums. */
+__device__ __forceinline__ double group_sum(double v, int k) {
     auto step = [](double x, int ctrl) {
         const long long b = __double_as_longlong(x);
         int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
@@ -419,38 +424,39 @@ __device__ __forceinline__ double group_sum(double v) {
         }
         return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
     };
-    v += step(v, 0);
-    v += step(v, 1);
-    v += step(v, 2);
-    if constexpr (kGroup == 16) v += step(v, 3); /* row mirror: lanes i and 15 - i */
+    if (k >= 2) v += step(v, 0);
+    if (k >= 4) v += step(v, 1);
+    if (k >= 8) v += step(v, 2);
+    if (k >= 16) v += step(v, 3);
     return v;
 }
 
 template <int P, typename Acc>
-__device__ __forceinline__ void group_reduce(PassAcc<P, Acc> &a) {
+__device__ __forceinline__ void group_reduce(PassAcc<P, Acc> &a, int k) {
+    if (k == 1) return;
 #pragma unroll
-    for (int i = 0; i < NP<P>::SQ; ++i) a.hq[i] = group_sum(a.hq[i]);
+    for (int i = 0; i < NP<P>::SQ; ++i) a.hq[i] = group_sum(a.hq[i], k);
 #pragma unroll
     for (int i = 0; i < P - 1; ++i) {
-        a.sj[i] = group_sum(a.sj[i]);
-        a.gq[i] = group_sum(a.gq[i]);
+        a.sj[i] = group_sum(a.sj[i], k);
+        a.gq[i] = group_sum(a.gq[i], k);
     }
-    a.sf = group_sum(a.sf);
-    a.sse0 = group_sum(a.sse0);
-    a.dsse = group_sum(a.dsse);
+    a.sf = group_sum(a.sf, k);
+    a.sse0 = group_sum(a.sse0, k);
+    a.dsse = group_sum(a.dsse, k);
 }
 
-template <int P, typename Acc, int W>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
-lm_straggler(KernelArgs args) {
-    extern __shared__ double lds_words[];
-    /* LDS: per-group FP64 factors E_x(0..n-1), E_y(0..n-1) (slot t at row t/8, column
-     * lane), then the windows: pixel p of a group at row p/8, column 8g + p%8 */
+/* The straggler loop for groups of 2^LK lanes. */
+template <int P, typename Acc, int LK>
+__device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, double *lds_words) {
+    /* LDS: per-group FP64 factors E_x(0..n-1), E_y(0..n-1) (slot t at row t/k, column
+     * k g + t%k; 2 nmax rows), then the windows: pixel p of a group at row p/k, column
+     * k g + p%k */
     const int nmax = 2 * args.cfg.max_half_width + 1;
     double *xcol = lds_words;
-    uint16_t *tile = reinterpret_cast<uint16_t *>(lds_words + nmax * 64);
-    const int lane = threadIdx.x, grp = lane / kGroup, r = lane % kGroup;
-    const unsigned count = args.strag_ctr[0];
+    uint16_t *tile = reinterpret_cast<uint16_t *>(lds_words + 2 * nmax * 64);
+    constexpr int lk = LK, kGroup = 1 << LK, kMask = kGroup - 1;
+    const int lane = threadIdx.x, grp = lane >> lk, r = lane & kMask;
     unsigned *head = args.strag_ctr + 1;
     const Parked<P> *park = static_cast<const Parked<P> *>(args.strag);
 
@@ -527,11 +533,11 @@ lm_straggler(KernelArgs args) {
             for (int t = r; t < 2 * n; t += kGroup) {
                 const bool isx = t < n;
                 const double u = (double)((float)(isx ? t : t - n) - (isx ? q.x0 : q.y0));
-                xcol[(t / kGroup) * 64 + lane] = dexp(-q.c0d * u * u);
+                xcol[(t >> lk) * 64 + lane] = dexp(-q.c0d * u * u);
             }
         }
         __syncthreads();
-        const int mmax = __builtin_amdgcn_readfirstlane(wave_max((n2 + kGroup - 1) / kGroup));
+        const int mmax = __builtin_amdgcn_readfirstlane(wave_max((n2 + kMask) >> lk));
         const double *gx = xcol + grp * kGroup;
         for (int m = 0; m < mmax; ++m) {
             const int pp = r + kGroup * m;
@@ -542,8 +548,8 @@ lm_straggler(KernelArgs args) {
                     pixel_ell(q, (float)i, (float)j, D, acc);
                 } else {
                     const int ty = n + j;
-                    const double ex = gx[(i / kGroup) * 64 + i % kGroup];
-                    const double ey = gx[(ty / kGroup) * 64 + ty % kGroup];
+                    const double ex = gx[(i >> lk) * 64 + (i & kMask)];
+                    const double ey = gx[(ty >> lk) * 64 + (ty & kMask)];
                     const float uy0 = (float)j - q.y0;
                     const float uy1 = uy0 - q.dyc;
                     const float ddy = -q.dyc * (uy1 + uy0);
@@ -551,13 +557,30 @@ lm_straggler(KernelArgs args) {
                 }
             }
         }
-        group_reduce<P, Acc>(acc);
+        group_reduce<P, Acc>(acc, kGroup);
 
         if (active) {
             LzrRawFit *o = r == 0 ? reinterpret_cast<LzrRawFit *>(args.out + s.roi) : nullptr;
             if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, o)) s.phase = open ? kIdle : kFinished;
         }
         __syncthreads();
+    }
+}
+
+template <int P, typename Acc, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
+lm_straggler(KernelArgs args) {
+    extern __shared__ double lds_words[];
+    const unsigned count = args.strag_ctr[0];
+    const unsigned lanes_total = gridDim.x * 64u;
+    int lk = kMaxGroup == 16 ? 4 : 3; /* log2 of the group size */
+    while (lk > 0 && (unsigned long long)count << lk > 2ull * lanes_total) --lk; /* <= 2 fits per group */
+    switch (__builtin_amdgcn_readfirstlane(lk)) {
+    case 4: straggler_groups<P, Acc, 4>(args, count, lds_words); break;
+    case 3: straggler_groups<P, Acc, 3>(args, count, lds_words); break;
+    case 2: straggler_groups<P, Acc, 2>(args, count, lds_words); break;
+    case 1: straggler_groups<P, Acc, 1>(args, count, lds_words); break;
+    default: straggler_groups<P, Acc, 0>(args, count, lds_words); break;
     }
 }
 
@@ -668,10 +691,12 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
         hipLaunchKernelGGL((lm_persistent<P, Acc, F64, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
     }
     if (a.strag_trials > 0) {
+        const int nmax = 2 * ctx->cfg.max_half_width + 1;
+        const size_t slds = lds + (size_t)nmax * 64 * sizeof(double); /* 2 nmax factor rows */
         int per_cu = 0;
-        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1>, 64, lds));
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1>, 64, slds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1>), dim3(per_cu * ctx->num_cus), dim3(64), lds, stream, a);
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1>), dim3(per_cu * ctx->num_cus), dim3(64), slds, stream, a);
     }
 }
 
