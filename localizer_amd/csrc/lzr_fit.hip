@@ -600,7 +600,8 @@ struct LzrContext {
     int device = 0;
     LaneConfig cfg{};
     int waves_per_cu = 0;
-    int occupancy5 = 1; /* register budget of the 5-parameter kernel: 1 or 2 waves/SIMD */
+    int occupancy5 = 0; /* register budget of the 5-parameter kernel: 1 or 2 waves/SIMD, 0 = by LDS */
+    size_t lds_per_cu = 160 * 1024;
     int rounds = 1;          /* launches per class; fits are parked between them */
     int park_threshold = 16; /* park a wave's fits when this few lanes are still busy */
     void *d_park[kClasses][2] = {}; /* [class][buffer] */
@@ -719,11 +720,21 @@ static void ensure_strag(LzrContext *ctx, int ncands) {
     ctx->strag_cap = cap;
 }
 
+/* Waves per SIMD the 5-parameter kernel is built for.  Automatic: 2 when the LDS tiles
+ * of at least 6 waves fit a CU (windows up to 11x11), so most SIMDs hold two: the
+ * two-wave build spills a few refill-path values but hides the pass's latencies (500k
+ * 9x9 fits: 6.57 -> 5.91 ms); with larger windows LDS allows about one wave per SIMD and
+ * the one-wave build has no spills. */
+static int occupancy5(const LzrContext *ctx) {
+    if (ctx->occupancy5 > 0) return ctx->occupancy5;
+    return launch_lds(ctx) * 6 <= ctx->lds_per_cu ? 2 : 1;
+}
+
 static void prepare(LzrContext *ctx, int ncands) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
     ensure_strag(ctx, ncands);
     const size_t lds = launch_lds(ctx);
-    if (ctx->occupancy5 == 1)
+    if (occupancy5(ctx) == 1)
         ensure_park<5, AccT, kF64Residual, 1>(ctx, 0, lds);
     else
         ensure_park<5, AccT, kF64Residual, 2>(ctx, 0, lds);
@@ -772,7 +783,7 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.strag = ctx->d_strag;
     a.strag_ctr = ctx->d_strag_ctr;
     const size_t lds = launch_lds(ctx);
-    if (ctx->occupancy5 == 1)
+    if (occupancy5(ctx) == 1)
         run_class<5, AccT, kF64Residual, 1>(ctx, a, 0, lds, stream);
     else
         run_class<5, AccT, kF64Residual, 2>(ctx, a, 0, lds, stream);
@@ -824,6 +835,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         hipDeviceProp_t prop;
         LZR_CHECK(hipGetDeviceProperties(&prop, ctx->device));
         ctx->num_cus = prop.multiProcessorCount;
+        if (prop.maxSharedMemoryPerMultiProcessor > 0) ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
         LZR_CHECK(hipMalloc((void **)&ctx->d_counters, kCounterWords * sizeof(unsigned)));
         LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 4 * kClasses * sizeof(unsigned)));
         LZR_CHECK(hipMalloc((void **)&ctx->d_strag_ctr, 2 * sizeof(unsigned)));
@@ -898,7 +910,7 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
         ctx->waves_per_cu = (int)value;
         return 0;
     case LZR_OPT_OCCUPANCY:
-        if (!(value == 1 || value == 2)) return fail(ctx, "occupancy must be 1 or 2");
+        if (!(value >= 0 && value <= 2)) return fail(ctx, "occupancy must be 0 (automatic), 1 or 2");
         ctx->occupancy5 = (int)value;
         return 0;
     case LZR_OPT_ROUNDS:
