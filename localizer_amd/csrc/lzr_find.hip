@@ -12,8 +12,9 @@
  * integers); then the reference's formula order m = S1/N, V = S2 - m*m*N,
  * alpha = G/sum(gc^2), T = alpha*alpha/V*sum(gc^2), segmented iff thr < T < 1 with
  * thr = 1 - exp(-PFA/N), over x in [h+1, X-h), y in [h+1, Y-h), X, Y rounded down to
- * even.  One 256-thread block per 64 x 16 tile: the tile and its halo are staged in LDS,
- * row sums first, then column sums.  Segmented pixels are appended to a per-frame list.
+ * even.  One 256-thread block per 64 x 64 tile: the tile and its halo are staged in LDS,
+ * each thread walks a column keeping the window's row sums in a register ring.
+ * Segmented pixels are appended to a per-frame list.
  *
  * Particles: one 1024-thread block per frame sorts the frame's segmented pixels (bitonic,
  * LDS) -- raster order -- and joins 8-neighbours with a lock-free union-find whose roots
@@ -42,10 +43,7 @@
 
 namespace lzr_find {
 
-constexpr int kTX = 64, kTY = 16;   /* output tile */
 constexpr int kMaxWin = 15;         /* largest GLRT window side accepted */
-constexpr int kMaxHalo = kMaxWin / 2;
-constexpr int kInX = kTX + 2 * kMaxHalo, kInY = kTY + 2 * kMaxHalo;
 
 struct GlrtParams {
     int ws, h;
@@ -74,66 +72,80 @@ __device__ inline bool glrt_test(const GlrtParams &P, double S1, double S2, doub
     return T < 1.0 && T > P.thr;
 }
 
-__global__ void __launch_bounds__(256) glrt_kernel(const uint16_t *frames, int nx, int ny, GlrtParams P,
-                                                   int cap, int *fg_list, FrameInfo *info) {
-    __shared__ double vin[kInY][kInX];
-    __shared__ double s1[kInY][kTX], s2[kInY][kTX], sg[kInY][kTX];
-    __shared__ unsigned long long part[256];
+/* One 256-thread block per 64 x 64 tile of one frame; the tile and its halo are staged
+ * in LDS as floats (exact for uint16).  Thread (column c, row group q) walks down its
+ * column over 16 output rows: the row sums of each input row over the window's columns
+ * (sum x, sum x^2, sum E(d) x) go into a ring of WS registers per sum (WS is a template
+ * parameter, so the ring is unrolled), and each output pixel adds the ring: S1, S2 exactly
+ * (integers below 2^53) and Sg = sum_d E(d) ring_g[d].  No LDS traffic beyond one read per
+ * (input row, window column). */
+constexpr int kGTX = 64, kGTY = 64, kGRows = kGTY / 4;
+
+template <int WS>
+__global__ void __launch_bounds__(256) glrt_kernel(const uint16_t *frames, int nx, int ny, GlrtParams P, int cap,
+                                                   int *fg_list, FrameInfo *info) {
+    constexpr int H = WS / 2;
+    constexpr int IX = kGTX + 2 * H, IY = kGTY + 2 * H;
+    __shared__ float vin[IY][IX];
     const int f = blockIdx.z;
-    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
-    const int h = P.h, ws = P.ws;
+    const int x0 = blockIdx.x * kGTX, y0 = blockIdx.y * kGTY;
     const int X = nx - (nx % 2), Y = ny - (ny % 2);
     const uint16_t *src = frames + (size_t)f * nx * ny;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     /* stage the tile with its halo; pixels outside the frame read as 0 (never tested) */
     unsigned long long own = 0;
-    for (int k = tid; k < (kTY + 2 * h) * (kTX + 2 * h); k += 256) {
-        const int r = k / (kTX + 2 * h), c = k - r * (kTX + 2 * h);
-        const int gx = x0 - h + c, gy = y0 - h + r;
+    for (int k = tid; k < IY * IX; k += 256) {
+        const int r = k / IX, c = k - r * IX;
+        const int gx = x0 - H + c, gy = y0 - H + r;
         uint16_t v = 0;
         if (gx >= 0 && gx < nx && gy >= 0 && gy < ny) v = src[(size_t)gy * nx + gx];
-        vin[r][c] = (double)v;
+        vin[r][c] = (float)v;
         /* the tile's own pixels feed the frame sum (each pixel once) */
-        if (r >= h && r < h + kTY && c >= h && c < h + kTX && gx < nx && gy < ny) own += v;
+        if (r >= H && r < H + kGTY && c >= H && c < H + kGTX && gx < nx && gy < ny) own += v;
     }
-    part[tid] = own;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) own += __shfl_xor(own, off, 64);
+    if (lane == 0 && own) atomicAdd(&info[f].frame_sum, own);
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) part[tid] += part[tid + s];
-        __syncthreads();
-    }
-    if (tid == 0 && part[0]) atomicAdd(&info[f].frame_sum, part[0]);
-    /* row sums over the window's columns */
-    for (int k = tid; k < (kTY + 2 * h) * kTX; k += 256) {
-        const int r = k / kTX, c = k - r * kTX;
-        double a = 0.0, b = 0.0, g = 0.0;
-        for (int d = 0; d < ws; ++d) {
-            const double v = vin[r][c + d];
+    double e[WS];
+#pragma unroll
+    for (int d = 0; d < WS; ++d) e[d] = P.e[d];
+    const int c = lane, q = tid >> 6;
+    const int rb = q * kGRows; /* first output row of this thread (tile-relative) */
+    const unsigned long long below = (1ull << lane) - 1ull;
+    double r1[WS], r2[WS], rg[WS];
+    auto row_sums = [&](int ir, double &a, double &b, double &g) {
+        a = 0.0;
+        b = 0.0;
+        g = 0.0;
+#pragma unroll
+        for (int d = 0; d < WS; ++d) {
+            const double v = (double)vin[ir][c + d];
             a += v;
             b += v * v;
-            g += P.e[d] * v;
+            g += e[d] * v;
         }
-        s1[r][c] = a;
-        s2[r][c] = b;
-        sg[r][c] = g;
-    }
-    __syncthreads();
-    const int lane = tid & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    for (int k = tid; k < kTX * kTY; k += 256) {
-        const int r = k / kTX, c = k - r * kTX;
-        const int x = x0 + c, y = y0 + r;
+    };
+#pragma unroll
+    for (int d = 0; d < WS - 1; ++d) row_sums(rb + d, r1[d], r2[d], rg[d]);
+#pragma unroll
+    for (int k = 0; k < kGRows; ++k) {
+        const int slot = (k + WS - 1) % WS;
+        row_sums(rb + k + WS - 1, r1[slot], r2[slot], rg[slot]);
+        const int x = x0 + c, y = y0 + rb + k;
         bool seg = false;
-        if (x >= h + 1 && x < X - h && y >= h + 1 && y < Y - h) {
+        if (x >= H + 1 && x < X - H && y >= H + 1 && y < Y - H) {
             double S1 = 0.0, S2 = 0.0, Sg = 0.0;
-            for (int d = 0; d < ws; ++d) {
-                S1 += s1[r + d][c];
-                S2 += s2[r + d][c];
-                Sg += P.e[d] * sg[r + d][c];
+#pragma unroll
+            for (int d = 0; d < WS; ++d) {
+                const int sl = (k + d) % WS; /* input row rb + k + d */
+                S1 += r1[sl];
+                S2 += r2[sl];
+                Sg += e[d] * rg[sl];
             }
             seg = glrt_test(P, S1, S2, Sg);
         }
-        /* wave-aggregated append to the frame's list (k advances uniformly per wave) */
+        /* wave-aggregated append to the frame's list (the wave walks one row at a time) */
         const unsigned long long m = __ballot(seg);
         if (m) {
             const int leader = __ffsll((long long)m) - 1;
@@ -141,8 +153,8 @@ __global__ void __launch_bounds__(256) glrt_kernel(const uint16_t *frames, int n
             if (lane == leader) base = atomicAdd(&info[f].fg_count, (unsigned)__popcll(m));
             base = __shfl(base, leader, 64);
             if (seg) {
-                const unsigned slot = base + (unsigned)__popcll(m & below);
-                if (slot < (unsigned)cap) fg_list[(size_t)f * cap + slot] = y * nx + x;
+                const unsigned slot2 = base + (unsigned)__popcll(m & below);
+                if (slot2 < (unsigned)cap) fg_list[(size_t)f * cap + slot2] = y * nx + x;
             }
         }
     }
@@ -541,8 +553,24 @@ LZR_EXPORT int lzr_find_candidates_device(LzrFinder *f, const uint16_t *d_frames
         }
         hipLaunchKernelGGL(reset_info, dim3((nf + 255) / 256), dim3(256), 0, s, f->d_info, nf);
         if (nframes > 0) {
-            hipLaunchKernelGGL(glrt_kernel, dim3((nx + kTX - 1) / kTX, (ny + kTY - 1) / kTY, nframes), dim3(256), 0, s,
-                               d_frames, nx, ny, f->P, f->cap, f->d_fg, f->d_info);
+            const dim3 grid((nx + kGTX - 1) / kGTX, (ny + kGTY - 1) / kGTY, nframes);
+            switch (f->P.ws) {
+#define LZR_GLRT_CASE(W)                                                                                     \
+    case W:                                                                                                  \
+        hipLaunchKernelGGL(glrt_kernel<W>, grid, dim3(256), 0, s, d_frames, nx, ny, f->P, f->cap, f->d_fg, f->d_info); \
+        break;
+                LZR_GLRT_CASE(1)
+                LZR_GLRT_CASE(3)
+                LZR_GLRT_CASE(5)
+                LZR_GLRT_CASE(7)
+                LZR_GLRT_CASE(9)
+                LZR_GLRT_CASE(11)
+                LZR_GLRT_CASE(13)
+                LZR_GLRT_CASE(15)
+#undef LZR_GLRT_CASE
+            default:
+                throw std::string("GLRT window side must be odd and at most 15");
+            }
             hipLaunchKernelGGL(scan_frames_kernel, dim3(1), dim3(1024), 0, s, f->d_info, nframes, 0, f->cap,
                                f->d_fg_off, f->d_fg_total, 0);
             hipLaunchKernelGGL(cc_kernel, dim3(nframes), dim3(kCCThreads), 2 * f->cap * sizeof(int), s, d_frames, nx,
