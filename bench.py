@@ -285,6 +285,7 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
 DENSE_PFA = 20.0
 
 
+DENSE_PIPELINES = 3  # frame blocks in flight (one stream each)
 RENDER_SCALE = 10.0
 RENDER_DEVIATION = 0.15
 
@@ -339,42 +340,63 @@ def dense_leg(nframes, block_frames, latency_frames, log):
     net = native.Net(w.initial_psf_width, seed=5)
     tr = netinit.train(net, synth.DENSE_ROIS_11, 1 << 20, epochs=1.0, batch=2048, lr=2e-3)
     log(f"dense: network trained on {tr['rois']} ROIs in {tr['train_s']:.2f} s")
-    finder = native.Finder(w.initial_psf_width, DENSE_PFA)
-    ctx = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, 1.0, w.half_width[0])
     cap = block_frames * 2048
-    d_cands = device.empty(abi.CANDIDATE_DTYPE, cap)
-    d_fits = device.empty(abi.FIT_DTYPE, cap)
-    d_work = device.empty(abi.START_DTYPE, cap)
-    d_count = torch.zeros(1, dtype=torch.int32, device="cuda")
-    stream = torch.cuda.current_stream()
-    s = stream.cuda_stream
+    status_off = abi.FIT_DTYPE.fields["status"][1]
 
-    def run_block(f0, nb, netinit_fit=True):
-        ptr = d_frames.ptr + f0 * npx * 2
-        finder.find_device(ptr, w.nx, w.ny, nb, d_cands.ptr, cap, d_count.data_ptr(), s)
-        n = int(d_count.item())  # synchronizes
-        if n < 0:
-            raise RuntimeError("dense stream: candidate capacity exceeded")
-        if n:
-            if netinit_fit:
-                ctx.fit_device_netinit(net, ptr, w.nx, w.ny, nb, d_cands.ptr, n, d_fits.ptr, d_work.ptr, 3.0,
-                                       stream=s)
-            else:
-                ctx.fit_device(ptr, w.nx, w.ny, nb, d_cands.ptr, n, d_fits.ptr, s)
-        return n
+    class Pipe:
+        """One block in flight: its own finder, context, buffers and stream (the network
+        is shared; its calls on different streams are ordered by the library)."""
+
+        def __init__(self):
+            self.finder = native.Finder(w.initial_psf_width, DENSE_PFA)
+            self.ctx = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, 1.0, w.half_width[0])
+            self.d_cands = device.empty(abi.CANDIDATE_DTYPE, cap)
+            self.d_fits = device.empty(abi.FIT_DTYPE, cap)
+            self.d_work = device.empty(abi.START_DTYPE, cap)
+            self.d_count = torch.zeros(1, dtype=torch.int32, device="cuda")
+            self.stream = torch.cuda.Stream()
+            self.acc = torch.zeros((), dtype=torch.int64, device="cuda")
+
+        def run(self, f0, nb, netinit_fit):
+            """Segmentation, then (after a read of this block's candidate count, which waits
+            for this pipeline's stream only) the fit; accepted fits counted on the device."""
+            s = self.stream.cuda_stream
+            with torch.cuda.stream(self.stream):
+                ptr = d_frames.ptr + f0 * npx * 2
+                self.finder.find_device(ptr, w.nx, w.ny, nb, self.d_cands.ptr, cap, self.d_count.data_ptr(), s)
+                n = int(self.d_count.item())
+                if n < 0:
+                    raise RuntimeError("dense stream: candidate capacity exceeded")
+                if n:
+                    if netinit_fit:
+                        self.ctx.fit_device_netinit(net, ptr, w.nx, w.ny, nb, self.d_cands.ptr, n, self.d_fits.ptr,
+                                                    self.d_work.ptr, 3.0, stream=s)
+                    else:
+                        self.ctx.fit_device(ptr, w.nx, w.ny, nb, self.d_cands.ptr, n, self.d_fits.ptr, s)
+                    st = self.d_fits.t[: n * abi.FIT_DTYPE.itemsize].view(n, abi.FIT_DTYPE.itemsize)
+                    st = st[:, status_off:status_off + 4].contiguous().view(torch.int32)
+                    self.acc += (st == 0).sum()
+            return n
+
+        def close(self):
+            self.finder.close()
+            self.ctx.close()
+
+    pipes = [Pipe() for _ in range(DENSE_PIPELINES)]
 
     def stream_all(netinit_fit):
-        run_block(0, min(block_frames, nframes), netinit_fit)  # warm-up
+        pipes[0].run(0, min(block_frames, nframes), netinit_fit)  # warm-up
         torch.cuda.synchronize()
-        cands = acc = 0
+        for p in pipes:
+            p.acc.zero_()
+        torch.cuda.synchronize()
+        cands = 0
         t = time.perf_counter()
-        for f0 in range(0, nframes, block_frames):
-            nb = min(block_frames, nframes - f0)
-            n = run_block(f0, nb, netinit_fit)
-            torch.cuda.synchronize()
-            cands += n
-            acc += int(np.sum(d_fits.t[: n * abi.FIT_DTYPE.itemsize].cpu().numpy().view(abi.FIT_DTYPE)["status"] == 0))
-        return time.perf_counter() - t, cands, acc
+        for b, f0 in enumerate(range(0, nframes, block_frames)):
+            cands += pipes[b % len(pipes)].run(f0, min(block_frames, nframes - f0), netinit_fit)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t
+        return sec, cands, int(sum(int(p.acc.item()) for p in pipes))
 
     sec_net, cands, acc_net = stream_all(True)
     sec_plain, _, acc_plain = stream_all(False)
@@ -382,7 +404,7 @@ def dense_leg(nframes, block_frames, latency_frames, log):
     for f in range(min(latency_frames, nframes)):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        run_block(f, 1, True)
+        pipes[0].run(f, 1, True)
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t) * 1e3)
     lat = np.array(lat)
@@ -393,6 +415,7 @@ def dense_leg(nframes, block_frames, latency_frames, log):
         "glrt_pfa": DENSE_PFA,
         "window": f"{2 * w.half_width[0] + 1}x{2 * w.half_width[0] + 1}",
         "block_frames": block_frames,
+        "pipelines": DENSE_PIPELINES,
         "candidates_per_frame": cands / nframes,
         "netinit": {"frames_per_s": nframes / sec_net, "localizations_per_s": acc_net / sec_net,
                     "accepted": acc_net, "seconds": sec_net},
@@ -404,8 +427,8 @@ def dense_leg(nframes, block_frames, latency_frames, log):
     log(f"dense: {cands / nframes:.1f} candidates/frame; net-init {nframes / sec_net:.1f} frames/s "
         f"({acc_net / sec_net:.4g} loc/s), plain {nframes / sec_plain:.1f} frames/s; latency p50 "
         f"{out['latency_ms_per_frame']['p50']:.2f} ms p99 {out['latency_ms_per_frame']['p99']:.2f} ms")
-    finder.close()
-    ctx.close()
+    for p in pipes:
+        p.close()
     net.close()
     return out
 

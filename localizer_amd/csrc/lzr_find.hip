@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "../../include/localizer_hip.h"
+#include "lzr_order.h"
 
 #define FIND_CHECK(call)                                                                    \
     do {                                                                                    \
@@ -418,6 +419,7 @@ struct LzrFinder {
     Particle *d_scratch = nullptr;
     LzrCandidate *d_staged = nullptr;
     int *d_fg_total = nullptr;
+    LzrStreamOrder order; /* calls on different streams share the buffers above */
 };
 
 static thread_local std::string g_finder_error;
@@ -502,6 +504,7 @@ LZR_EXPORT int lzr_finder_destroy(LzrFinder *f) {
     void *ptrs[] = {f->d_fg, f->d_info, f->d_fg_off, f->d_out_off, f->d_scratch, f->d_staged, f->d_fg_total};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    f->order.destroy();
     delete f;
     return 0;
 }
@@ -551,6 +554,7 @@ LZR_EXPORT int lzr_find_candidates_device(LzrFinder *f, const uint16_t *d_frames
             falloc(f->d_fg_total, 1);
             f->frames_cap = (size_t)nf;
         }
+        FIND_CHECK(f->order.begin(s));
         hipLaunchKernelGGL(reset_info, dim3((nf + 255) / 256), dim3(256), 0, s, f->d_info, nf);
         if (nframes > 0) {
             const dim3 grid((nx + kGTX - 1) / kGTX, (ny + kGTY - 1) / kGTY, nframes);
@@ -582,6 +586,7 @@ LZR_EXPORT int lzr_find_candidates_device(LzrFinder *f, const uint16_t *d_frames
             hipLaunchKernelGGL(compact_kernel, dim3(4, nframes), dim3(256), 0, s, f->d_staged, f->d_fg_off,
                                f->d_out_off, f->d_info, nframes, d_count, d_candidates);
         FIND_CHECK(hipGetLastError());
+        FIND_CHECK(f->order.end(s));
     } catch (const std::string &e) {
         f->err = e;
         return -1;

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "lzr_lane.h"
+#include "lzr_order.h"
 
 #ifndef LZR_STRAGGLER_GROUP
 #define LZR_STRAGGLER_GROUP 16 /* lanes per fit in the straggler kernel */
@@ -611,9 +612,7 @@ struct LzrContext {
     uint8_t *d_strag = nullptr;        /* handed-off fits of the running class */
     size_t strag_cap = 0;              /* bytes */
     unsigned *d_strag_ctr = nullptr;   /* (count, head) */
-    hipEvent_t done = nullptr;         /* end of the last call's kernels */
-    bool done_recorded = false;
-    hipStream_t done_stream = nullptr;
+    LzrStreamOrder order;              /* calls on different streams share the buffers above */
     int num_cus = 0;
     std::string err;
     std::mutex mu;
@@ -749,16 +748,8 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
                    const int *d_count = nullptr) {
     ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
     ensure_strag(ctx, ncands);
-    /* The context's queue, counters and hand-off buffers are reused by every call: a call
-     * on another stream waits for the previous call's kernels (not inside a graph
-     * capture, whose replays the caller orders). */
-    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    LZR_CHECK(hipStreamIsCapturing(stream, &cap));
-    const bool capturing = cap != hipStreamCaptureStatusNone;
-    if (!capturing) {
-        if (!ctx->done) LZR_CHECK(hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming));
-        if (ctx->done_recorded && ctx->done_stream != stream) LZR_CHECK(hipStreamWaitEvent(stream, ctx->done, 0));
-    }
+    /* the context's queue, counters and hand-off buffers are reused by every call */
+    LZR_CHECK(ctx->order.begin(stream));
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
     const int tpb = 256;
     const int nb = (ncands + tpb - 1) / tpb;
@@ -792,11 +783,7 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, d_count,
                        out, ctx->cfg);
     LZR_CHECK(hipGetLastError());
-    if (!capturing) {
-        LZR_CHECK(hipEventRecord(ctx->done, stream));
-        ctx->done_recorded = true;
-        ctx->done_stream = stream;
-    }
+    LZR_CHECK(ctx->order.end(stream));
 }
 
 static int fail(LzrContext *ctx, const std::string &msg) {
@@ -866,7 +853,7 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_park_ctr);
     (void)hipFree(ctx->d_strag_ctr);
-    if (ctx->done) (void)hipEventDestroy(ctx->done);
+    ctx->order.destroy();
     (void)hipFree(ctx->d_strag);
     for (int c = 0; c < kClasses; ++c)
         for (int b = 0; b < 2; ++b) (void)hipFree(ctx->d_park[c][b]);

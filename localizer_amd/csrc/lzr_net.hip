@@ -40,6 +40,7 @@
 #include <vector>
 
 #include "../../include/localizer_hip.h"
+#include "lzr_order.h"
 
 #define NET_CHECK(call)                                                                     \
     do {                                                                                    \
@@ -813,6 +814,7 @@ struct LzrNet {
     size_t train_cap = 0;
     uint16_t *d_xT = nullptr, *d_a1T = nullptr, *d_a2T = nullptr, *d_h = nullptr;
     uint16_t *d_dh = nullptr, *d_dhT = nullptr, *d_da2 = nullptr, *d_da2T = nullptr, *d_da1T = nullptr;
+    LzrStreamOrder order; /* calls on different streams share the buffers above */
 };
 
 static thread_local std::string g_net_create_error;
@@ -954,6 +956,7 @@ LZR_EXPORT int lzr_net_destroy(LzrNet *net) {
                     net->d_a1T, net->d_a2T, net->d_h, net->d_dh, net->d_dhT, net->d_da2, net->d_da2T, net->d_da1T};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    net->order.destroy();
     delete net;
     return 0;
 }
@@ -1039,10 +1042,12 @@ LZR_EXPORT int lzr_net_forward_device(LzrNet *net, const uint16_t *d_frames, int
     try {
         NET_CHECK(hipSetDevice(net->device));
         hipStream_t s = (hipStream_t)stream;
+        NET_CHECK(net->order.begin(s));
         run_forward<false>(net, d_frames, nx, ny, nframes, d_cands, ncands, ncands, s);
         hipLaunchKernelGGL(decode_kernel, dim3((ncands + 255) / 256), dim3(256), 0, s, net->d_out, net->d_stats,
                            ncands, net->w, (LzrStart *)nullptr, d_outputs);
         NET_CHECK(hipGetLastError());
+        NET_CHECK(net->order.end(s));
     } catch (const std::string &e) {
         return net_fail(net, e);
     }
@@ -1059,10 +1064,12 @@ LZR_EXPORT int lzr_net_predict_device(LzrNet *net, const uint16_t *d_frames, int
     try {
         NET_CHECK(hipSetDevice(net->device));
         hipStream_t s = (hipStream_t)stream;
+        NET_CHECK(net->order.begin(s));
         run_forward<false>(net, d_frames, nx, ny, nframes, d_cands, ncands, ncands, s);
         hipLaunchKernelGGL(decode_kernel, dim3((ncands + 255) / 256), dim3(256), 0, s, net->d_out, net->d_stats,
                            ncands, net->w, d_starts, (float *)nullptr);
         NET_CHECK(hipGetLastError());
+        NET_CHECK(net->order.end(s));
     } catch (const std::string &e) {
         return net_fail(net, e);
     }
@@ -1084,6 +1091,7 @@ LZR_EXPORT int lzr_net_train_step_device(LzrNet *net, const uint16_t *d_frames, 
         const int B = (ncands + 63) / 64 * 64; /* padded rows carry weight 0 */
         ensure_rows(net, (size_t)B);
         ensure_train(net, (size_t)B);
+        NET_CHECK(net->order.begin(s));
         /* fill kernels, not memsets: the step stays capturable in a HIP graph */
         hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, net->d_loss, 1);
         hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<float *>(net->d_counter), 1);
@@ -1123,6 +1131,7 @@ LZR_EXPORT int lzr_net_train_step_device(LzrNet *net, const uint16_t *d_frames, 
         if (grads) {
             NET_CHECK(hipMemcpyAsync(grads, net->d_grad, NPARAMS * sizeof(float), hipMemcpyDeviceToHost, s));
         }
+        NET_CHECK(net->order.end(s));
         if (loss || grads) NET_CHECK(hipStreamSynchronize(s));
     } catch (const std::string &e) {
         return net_fail(net, e);

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/localizer_hip.h"
+#include "lzr_order.h"
 
 #define RND_CHECK(call)                                                                     \
     do {                                                                                    \
@@ -238,6 +239,7 @@ struct LzrRenderer {
     size_t all_cap = 0;
     char *d_flag = nullptr;
     size_t flag_cap = 0;
+    LzrStreamOrder order; /* calls on different streams share the buffers above */
 };
 
 static thread_local std::string g_render_create_error;
@@ -285,6 +287,7 @@ LZR_EXPORT int lzr_renderer_destroy(LzrRenderer *r) {
                     r->d_vals, r->d_keys2, r->d_vals2, r->d_temp, r->d_error, r->d_all, r->d_flag};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    r->order.destroy();
     delete r;
     return 0;
 }
@@ -336,6 +339,7 @@ LZR_EXPORT int lzr_render_device(LzrRenderer *r, const LzrPosition *d_positions,
         const size_t ntiles = (size_t)g.ntr * g.ntc;
         if (ntiles >= (size_t)INT_MAX) throw std::string("image too large");
         const int n = n_positions;
+        RND_CHECK(r->order.begin(s));
         grow(r->d_items, r->items_cap, (size_t)n);
         grow(r->d_counts, r->counts_cap, (size_t)n + 1);
         grow(r->d_offsets, r->offsets_cap, (size_t)n + 1);
@@ -387,6 +391,7 @@ LZR_EXPORT int lzr_render_device(LzrRenderer *r, const LzrPosition *d_positions,
         hipLaunchKernelGGL(render_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, r->d_items, r->d_vals2,
                            r->d_tile_start, g, r->d_table, d_image);
         RND_CHECK(hipGetLastError());
+        RND_CHECK(r->order.end(s));
     } catch (const std::string &e) {
         return rfail(r, "lzr_render_device: " + e);
     }
@@ -438,6 +443,7 @@ LZR_EXPORT int lzr_positions_from_fits_device(LzrRenderer *r, const LzrFit *d_fi
             hipLaunchKernelGGL(fill<int>, dim3(1), dim3(64), 0, s, d_count, 1, 0);
             return 0;
         }
+        RND_CHECK(r->order.begin(s));
         grow(r->d_all, r->all_cap, (size_t)n_fits);
         grow(r->d_flag, r->flag_cap, (size_t)n_fits);
         const int tpb = 256;
@@ -448,6 +454,7 @@ LZR_EXPORT int lzr_positions_from_fits_device(LzrRenderer *r, const LzrFit *d_fi
         ensure_temp(r, t);
         RND_CHECK(hipcub::DeviceSelect::Flagged(r->d_temp, t, r->d_all, r->d_flag, d_positions, d_count, n_fits, s));
         RND_CHECK(hipGetLastError());
+        RND_CHECK(r->order.end(s));
     } catch (const std::string &e) {
         return rfail(r, "lzr_positions_from_fits_device: " + e);
     }
