@@ -101,7 +101,7 @@ struct LaneState {
     double diag[P]; /* column scaling D                             */
     float H[NP<P>::S]; /* J^T J at th (packed)                      */
     float g[P];        /* J^T f at th                               */
-    double delta, par, fnorm, xnorm, gnorm;
+    double delta, par, fnorm, xnorm;
     int xoff, yoff, n, frame;
     int roi;
     int phase;
@@ -461,6 +461,25 @@ LZR_HD LZR_INLINE double drsqrt(double v) {
     return 1.0 / std::sqrt(v);
 #endif
 }
+/* 1/v: the hardware estimate refined by two Newton steps (e = 1 - v r, r <- r + r e),
+ * within ~1 ulp; 0, +-inf and NaN take the estimate as is (inf, 0, NaN).  Used for the
+ * LM logic's divisions, where a last-bit difference from a correctly rounded quotient
+ * is far below every tolerance the reference tests. */
+LZR_HD LZR_INLINE double drcp(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(v);
+    if (__builtin_isfinite(r) && r != 0.0) {
+        double e = __builtin_fma(-v, r, 1.0);
+        r = __builtin_fma(r, e, r);
+        e = __builtin_fma(-v, r, 1.0);
+        r = __builtin_fma(r, e, r);
+    }
+    return r;
+#else
+    return 1.0 / v;
+#endif
+}
+LZR_HD LZR_INLINE double ddiv(double a, double b) { return a * drcp(b); }
 LZR_HD LZR_INLINE double dabs(double v) { return v < 0 ? -v : v; }
 LZR_HD LZR_INLINE double dmax(double a, double b) { return a > b ? a : b; }
 LZR_HD LZR_INLINE double dmin(double a, double b) { return a < b ? a : b; }
@@ -564,24 +583,25 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
     }
     double parl = 0.0;
     if (full_rank) {
-        const double inv_dxnorm = 1.0 / dxnorm;
+        const double inv_dxnorm = drcp(dxnorm);
 #pragma unroll
         for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] * inv_dxnorm);
         forward<P>(L, id, u, y);
         const double phider = plain_norm2<P>(y);
-        parl = phider > 0.0 ? fp / (delta * phider) : 0.0;
+        parl = phider > 0.0 ? ddiv(fp, delta * phider) : 0.0;
     }
 #pragma unroll
-    for (int i = 0; i < P; ++i) u[i] = g[i] / diag[i];
+    for (int i = 0; i < P; ++i) u[i] = ddiv(g[i], diag[i]);
     const double gnorm = plain_norm<P>(u);
-    double paru = gnorm / delta;
+    const double inv_delta = drcp(delta);
+    double paru = gnorm * inv_delta;
     if (paru == 0.0) paru = kDblMin / dmin(delta, 0.1);
     double par = par_io;
     if (par > paru)
         par = paru;
     else if (par < parl)
         par = parl;
-    if (par == 0.0 && full_rank) par = gnorm / dxnorm;
+    if (par == 0.0 && full_rank) par = ddiv(gnorm, dxnorm);
     double M[S];
     for (int iter = 1;; ++iter) {
         if (par == 0.0) par = dmax(0.001 * paru, kDblMin);
@@ -601,11 +621,11 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
         if (dabs(fp) <= 0.1 * delta) break;
         if (parl == 0.0 && fp <= fp_old && fp_old < 0.0) break;
         if (iter == 10) break;
-        const double inv_dxnorm = 1.0 / dxnorm;
+        const double inv_dxnorm = drcp(dxnorm);
 #pragma unroll
         for (int i = 0; i < P; ++i) u[i] = diag[i] * (diag[i] * x[i] * inv_dxnorm);
         forward<P>(L, id, u, y);
-        const double parc = fp / (delta * plain_norm2<P>(y));
+        const double parc = ddiv(fp, delta * plain_norm2<P>(y));
         if (fp > 0.0) {
             if (par > parl) parl = par;
         } else if (fp < 0.0) {
@@ -635,7 +655,8 @@ LZR_HD LZR_INLINE void absolute(const LaneState<P> &s, const double (&t)[P], dou
 /* w: the fixed width of the 4-parameter model (unused by the others). */
 LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[4], const double (&dx)[4], double sigma, double w,
                                     PassIn<4> &q) {
-    const double c0 = 1.0 / (2.0 * w * w);
+    const double iw = drcp(w), is = drcp(sigma);
+    const double c0 = 0.5 * iw * iw;
     q.A0 = (float)t0[0];
     q.x0 = (float)t0[1];
     q.y0 = (float)t0[2];
@@ -646,19 +667,21 @@ LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[4], const double (&dx)[4]
     q.dyc = (float)dx[2];
     q.db = (float)dx[3];
     q.A1 = (float)(t0[0] + dx[0]);
-    q.inv_sigma = (float)(1.0 / sigma);
+    q.inv_sigma = (float)is;
     q.A0d = t0[0];
     q.b0d = t0[3];
     q.c0d = c0;
-    q.inv_sigma_d = 1.0 / sigma;
-    q.kx = (float)(1.0 / (w * w * sigma));
+    q.inv_sigma_d = is;
+    q.kx = (float)(iw * iw * is);
 }
 
 LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[5], const double (&dx)[5], double sigma, double,
                                     PassIn<5> &q) {
-    const double r0 = t0[1], r1 = t0[1] + dx[1];
+    const double r1 = t0[1] + dx[1];
     const double A1 = t0[0] + dx[0];
-    const double c0 = 1.0 / (2.0 * r0 * r0), c1 = 1.0 / (2.0 * r1 * r1);
+    const double ir0 = drcp(t0[1]), ir1 = drcp(r1);
+    const double c0 = 0.5 * ir0 * ir0, c1 = 0.5 * ir1 * ir1;
+    const double is = drcp(sigma);
     q.A0 = (float)t0[0];
     q.x0 = (float)t0[2];
     q.y0 = (float)t0[3];
@@ -670,22 +693,23 @@ LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[5], const double (&dx)[5]
     q.db = (float)dx[4];
     q.dc = (float)(c1 - c0);
     q.A1 = (float)A1;
-    q.inv_sigma = (float)(1.0 / sigma);
+    q.inv_sigma = (float)is;
     q.A0d = t0[0];
     q.b0d = t0[4];
     q.c0d = c0;
-    q.inv_sigma_d = 1.0 / sigma;
+    q.inv_sigma_d = is;
     /* d f/d r = A e d^2 / (sqrt(2) r^3 sigma)   (source/Localization.cpp:1151-1166)
      * d f/d x0 = A e (x - x0) / (r^2 sigma)                                        */
-    q.kr = (float)(1.0 / (kSqrt2 * r1 * r1 * r1 * sigma));
-    q.kx = (float)(1.0 / (r1 * r1 * sigma));
+    q.kr = (float)(ir1 * ir1 * ir1 * is * (1.0 / kSqrt2));
+    q.kx = (float)(ir1 * ir1 * is);
 }
 
 LZR_HD LZR_INLINE void quad_coeffs(double sx, double sy, double rho, double &qa, double &qc, double &qb) {
-    const double pre = 1.0 / (2.0 * (1.0 - rho * rho));
-    qa = pre / (sx * sx);
-    qb = pre / (sy * sy);
-    qc = -pre * rho / (sx * sy);
+    const double pre = 0.5 * drcp(1.0 - rho * rho);
+    const double isx = drcp(sx), isy = drcp(sy);
+    qa = pre * isx * isx;
+    qb = pre * isy * isy;
+    qc = -pre * rho * isx * isy;
 }
 
 LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7], double sigma, double,
@@ -709,25 +733,28 @@ LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7]
     q.dqc = (float)(qc1 - qc0);
     q.dqb = (float)(qb1 - qb0);
     q.A1 = (float)(t0[0] + dx[0]);
-    q.inv_sigma = (float)(1.0 / sigma);
+    const double is = drcp(sigma);
+    q.inv_sigma = (float)is;
     q.A0d = t0[0];
     q.b0d = t0[6];
     q.qa0d = qa0;
     q.qc0d = qc0;
     q.qb0d = qb0;
-    q.inv_sigma_d = 1.0 / sigma;
-    /* reference columns (source/Localization.cpp:1275-1280) with den = 2(1-rho^2) sigma */
-    const double omr = 1.0 - rho * rho;
-    const double den = 2.0 * omr * sigma;
-    q.k1 = (float)(2.0 / (den * sx * sx * sx));
-    q.k2 = (float)(2.0 * rho / (den * sx * sx * sy));
-    q.k3 = (float)(2.0 / (den * sy * sy * sy));
-    q.k4 = (float)(2.0 * rho / (den * sx * sy * sy));
-    q.k5 = (float)(2.0 / (den * sx * sx));
-    q.k6 = (float)(2.0 * rho / (den * sx * sy));
-    q.k7 = (float)(2.0 / (den * sy * sy));
-    q.k8 = (float)(1.0 / (sigma * omr * sx * sy));
-    q.k9 = (float)(rho / (sigma * sx * sy * omr * omr));
+    q.inv_sigma_d = is;
+    /* reference columns (source/Localization.cpp:1275-1280) with den = 2(1-rho^2) sigma;
+     * k = 2/(den ...) = 1/(omr sigma ...) */
+    const double iomr = drcp(1.0 - rho * rho);
+    const double isx = drcp(sx), isy = drcp(sy);
+    const double t = iomr * is; /* 2 / den */
+    q.k1 = (float)(t * isx * isx * isx);
+    q.k2 = (float)(t * rho * isx * isx * isy);
+    q.k3 = (float)(t * isy * isy * isy);
+    q.k4 = (float)(t * rho * isx * isy * isy);
+    q.k5 = (float)(t * isx * isx);
+    q.k6 = (float)(t * rho * isx * isy);
+    q.k7 = (float)(t * isy * isy);
+    q.k8 = (float)(t * isx * isy);
+    q.k9 = (float)(t * iomr * rho * isx * isy);
     q.two_rho = (float)(2.0 * rho);
 }
 
@@ -960,6 +987,17 @@ LZR_HD LZR_INLINE void propose(LaneState<P> &s) {
     s.phase = kTrial;
 }
 
+/* The scaled gradient norm max_i |g_i / d_i| / |f| that iterate() computes at its
+ * start; g, d and |f| change only when a step is accepted, so evaluating it where the
+ * ETOLG test reads it gives the same value. */
+template <int P>
+LZR_HD LZR_INLINE double gradient_norm(const LaneState<P> &s) {
+    double gmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) gmax = dmax(gmax, dabs((double)s.g[i] / s.diag[i]));
+    return gmax / s.fnorm;
+}
+
 /* Start of one gsl_multifit_fdfsolver_iterate call of the reference loop
  * (source/Localization.cpp:148-155).  Returns true when a trial is to be proposed;
  * false when the loop ended (status in *st). */
@@ -975,10 +1013,6 @@ LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int
             return false;
         }
     }
-    double gmax = 0.0;
-#pragma unroll
-    for (int i = 0; i < P; ++i) gmax = dmax(gmax, dabs((double)s.g[i] / s.diag[i]));
-    s.gnorm = gmax / s.fnorm;
     s.inner = 0;
     return true;
 }
@@ -1038,8 +1072,9 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
     const double dsse = (double)a.dsse;
     const bool finite = dsse == dsse && dabs(dsse) <= 1.0e300;
     const double fnorm1 = finite ? dsqrt(dmax(fn2 + dsse, 0.0)) : 1.0e300;
+    const double inv_fn2 = drcp(fn2);
     double actred = -1.0;
-    if (0.1 * fnorm1 < s.fnorm) actred = -dsse / fn2; /* = 1 - (fnorm1/fnorm)^2 */
+    if (0.1 * fnorm1 < s.fnorm) actred = -dsse * inv_fn2; /* = 1 - (fnorm1/fnorm)^2 */
     double H[NP<P>::S];
     double gd[P];
     to_double<P>(s.H, s.g, H, gd);
@@ -1053,11 +1088,11 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
         jdx2 += s.dx[i] * row;
     }
     const double pnorm = scaled_norm<P>(s.diag, s.dx);
-    const double t1sq = dmax(jdx2, 0.0) / fn2;
-    const double t2 = (dsqrt(s.par) * pnorm) / s.fnorm;
-    const double prered = t1sq + t2 * t2 / 0.5;
-    const double dirder = -(t1sq + t2 * t2);
-    const double ratio = prered > 0.0 ? actred / prered : 0.0;
+    const double t1sq = dmax(jdx2, 0.0) * inv_fn2;
+    const double t2sq = s.par * pnorm * pnorm * inv_fn2; /* (sqrt(par) |D dx| / |f|)^2 */
+    const double prered = t1sq + t2sq / 0.5;
+    const double dirder = -(t1sq + t2sq);
+    const double ratio = prered > 0.0 ? ddiv(actred, prered) : 0.0;
     if (ratio > 0.25) {
         if (s.par == 0.0 || ratio >= 0.75) {
             s.delta = pnorm / 0.5;
@@ -1107,7 +1142,7 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
         st = LZR_LM_ETOLF;
     } else if (s.delta <= kDblEpsilon * s.xnorm) {
         st = LZR_LM_ETOLX;
-    } else if (s.gnorm <= kDblEpsilon) {
+    } else if (gradient_norm<P>(s) <= kDblEpsilon) {
         st = LZR_LM_ETOLG;
     } else if (s.inner >= 10) {
         st = LZR_LM_ENOPROG;
