@@ -53,8 +53,10 @@ struct KernelArgs {
     unsigned *head;        /* queue head (device, zeroed per call) */
     LzrFit *out;
     LaneConfig cfg;
-    /* long fits: a fit still running after strag_trials trials is handed to the
-     * straggler kernel (0: never) through strag[] and strag_ctr = (count, head) */
+    /* long fits: a fit still running after strag_trials trials is handed off to the
+     * straggler kernel (0: never) through strag[] (class c's region starts at entry
+     * first_c of kParkedMax bytes, first_c = the queue entries of the classes before c)
+     * and strag_ctr[2c], [2c+1] = (count, head) */
     int strag_trials;
     void *strag;
     unsigned *strag_ctr;
@@ -179,6 +181,16 @@ struct Parked {
     LaneState<P> s;
     int model;
 };
+constexpr size_t kParkedMax = sizeof(Parked<7>) > sizeof(Parked<5>) ? sizeof(Parked<7>) : sizeof(Parked<5>);
+static_assert(sizeof(Parked<4>) <= kParkedMax, "hand-off entry size");
+
+/* A class's hand-off region. */
+template <int P>
+__device__ __forceinline__ Parked<P> *strag_region(const KernelArgs &args, int cls) {
+    unsigned first = 0;
+    for (int c = 0; c < cls; ++c) first += args.count[c];
+    return reinterpret_cast<Parked<P> *>(static_cast<char *>(args.strag) + (size_t)first * kParkedMax);
+}
 
 /* One round of the fitting kernel.  Round 0 takes fresh candidates from the sorted
  * queue; later rounds resume fits parked by the previous round.  A wave parks its
@@ -368,10 +380,10 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         if (hmask) {
             const int leader = __ffsll((long long)hmask) - 1;
             unsigned base = 0;
-            if (lane == leader) base = atomicAdd(args.strag_ctr, (unsigned)__popcll(hmask));
+            if (lane == leader) base = atomicAdd(args.strag_ctr + 2 * cls, (unsigned)__popcll(hmask));
             base = __shfl(base, leader, 64);
             if (hand_off) {
-                Parked<P> &pk = static_cast<Parked<P> *>(args.strag)[base + (unsigned)__popcll(hmask & below)];
+                Parked<P> &pk = strag_region<P>(args, cls)[base + (unsigned)__popcll(hmask & below)];
                 pk.s = s;
                 pk.model = model;
                 s.phase = open ? kIdle : kFinished;
@@ -449,7 +461,8 @@ __device__ __forceinline__ void group_reduce(PassAcc<P, Acc> &a, int k) {
 
 /* The straggler loop for groups of 2^LK lanes. */
 template <int P, typename Acc, int LK>
-__device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, double *lds_words) {
+__device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
+                                                 const Parked<P> *park, double *lds_words) {
     /* LDS: per-group FP64 factors E_x(0..n-1), E_y(0..n-1) (slot t at row t/k, column
      * k g + t%k; 2 nmax rows), then the windows: pixel p of a group at row p/k, column
      * k g + p%k */
@@ -458,8 +471,6 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
     uint16_t *tile = reinterpret_cast<uint16_t *>(lds_words + 2 * nmax * 64);
     constexpr int lk = LK, kGroup = 1 << LK, kMask = kGroup - 1;
     const int lane = threadIdx.x, grp = lane >> lk, r = lane & kMask;
-    unsigned *head = args.strag_ctr + 1;
-    const Parked<P> *park = static_cast<const Parked<P> *>(args.strag);
 
     LaneState<P> s;
     s.phase = kIdle;
@@ -570,18 +581,20 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
 
 template <int P, typename Acc, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
-lm_straggler(KernelArgs args) {
+lm_straggler(KernelArgs args, int cls) {
     extern __shared__ double lds_words[];
-    const unsigned count = args.strag_ctr[0];
+    const unsigned count = args.strag_ctr[2 * cls];
+    unsigned *head = args.strag_ctr + 2 * cls + 1;
+    const Parked<P> *park = strag_region<P>(args, cls);
     const unsigned lanes_total = gridDim.x * 64u;
     int lk = kMaxGroup == 16 ? 4 : 3; /* log2 of the group size */
     while (lk > 0 && (unsigned long long)count << lk > 2ull * lanes_total) --lk; /* <= 2 fits per group */
     switch (__builtin_amdgcn_readfirstlane(lk)) {
-    case 4: straggler_groups<P, Acc, 4>(args, count, lds_words); break;
-    case 3: straggler_groups<P, Acc, 3>(args, count, lds_words); break;
-    case 2: straggler_groups<P, Acc, 2>(args, count, lds_words); break;
-    case 1: straggler_groups<P, Acc, 1>(args, count, lds_words); break;
-    default: straggler_groups<P, Acc, 0>(args, count, lds_words); break;
+    case 4: straggler_groups<P, Acc, 4>(args, count, head, park, lds_words); break;
+    case 3: straggler_groups<P, Acc, 3>(args, count, head, park, lds_words); break;
+    case 2: straggler_groups<P, Acc, 2>(args, count, head, park, lds_words); break;
+    case 1: straggler_groups<P, Acc, 1>(args, count, head, park, lds_words); break;
+    default: straggler_groups<P, Acc, 0>(args, count, head, park, lds_words); break;
     }
 }
 
@@ -611,7 +624,9 @@ struct LzrContext {
     int strag_trials = 48;             /* hand a fit to the straggler kernel after this many trials */
     uint8_t *d_strag = nullptr;        /* handed-off fits of the running class */
     size_t strag_cap = 0;              /* bytes */
-    unsigned *d_strag_ctr = nullptr;   /* (count, head) */
+    unsigned *d_strag_ctr = nullptr;   /* per class (count, head) */
+    hipStream_t side[kClasses - 1] = {}; /* classes 1, 2 run beside class 0 */
+    hipEvent_t fork = nullptr, join[kClasses - 1] = {};
     LzrStreamOrder order;              /* calls on different streams share the buffers above */
     int num_cus = 0;
     std::string err;
@@ -686,8 +701,6 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
         r.out = ctx->d_park[cls][ob];
         r.out_count = ctr + 2 * ob;
         r.threshold = round == ctx->rounds - 1 ? 0 : ctx->park_threshold;
-        if (round == 0 && a.strag_trials > 0)
-            hipLaunchKernelGGL(reset_park, dim3(1), dim3(64), 0, stream, a.strag_ctr);
         hipLaunchKernelGGL((lm_persistent<P, Acc, F64, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
     }
     if (a.strag_trials > 0) {
@@ -696,7 +709,7 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
         int per_cu = 0;
         LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1>, 64, slds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1>), dim3(per_cu * ctx->num_cus), dim3(64), slds, stream, a);
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1>), dim3(per_cu * ctx->num_cus), dim3(64), slds, stream, a, cls);
     }
 }
 
@@ -710,8 +723,6 @@ static size_t launch_lds(const LzrContext *ctx) {
 
 /* Allocates everything a launch of up to ncands candidates needs, so that the launch
  * itself allocates nothing (stream capture). */
-constexpr size_t kParkedMax = std::max(sizeof(Parked<4>), std::max(sizeof(Parked<5>), sizeof(Parked<7>)));
-
 static void ensure_strag(LzrContext *ctx, int ncands) {
     if (ctx->strag_trials <= 0) return;
     size_t cap = ctx->strag_cap;
@@ -774,12 +785,22 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.strag = ctx->d_strag;
     a.strag_ctr = ctx->d_strag_ctr;
     const size_t lds = launch_lds(ctx);
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, stream, ctx->d_strag_ctr, 2 * kClasses);
+    /* The classes share nothing but the sorted queue: classes 1 and 2 run on the
+     * context's side streams beside class 0, so one class's straggler tail overlaps
+     * another class's main kernel (fork/join by events; also inside a graph capture). */
+    LZR_CHECK(hipEventRecord(ctx->fork, stream));
+    for (int c = 0; c < kClasses - 1; ++c) LZR_CHECK(hipStreamWaitEvent(ctx->side[c], ctx->fork, 0));
+    run_class<7, AccT, kF64Residual, 1>(ctx, a, 1, lds, ctx->side[0]);
+    run_class<4, AccT, kF64Residual, 2>(ctx, a, 2, lds, ctx->side[1]);
     if (occupancy5(ctx) == 1)
         run_class<5, AccT, kF64Residual, 1>(ctx, a, 0, lds, stream);
     else
         run_class<5, AccT, kF64Residual, 2>(ctx, a, 0, lds, stream);
-    run_class<7, AccT, kF64Residual, 1>(ctx, a, 1, lds, stream);
-    run_class<4, AccT, kF64Residual, 2>(ctx, a, 2, lds, stream);
+    for (int c = 0; c < kClasses - 1; ++c) {
+        LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
+        LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
+    }
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, d_count,
                        out, ctx->cfg);
     LZR_CHECK(hipGetLastError());
@@ -825,7 +846,12 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         if (prop.maxSharedMemoryPerMultiProcessor > 0) ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
         LZR_CHECK(hipMalloc((void **)&ctx->d_counters, kCounterWords * sizeof(unsigned)));
         LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 4 * kClasses * sizeof(unsigned)));
-        LZR_CHECK(hipMalloc((void **)&ctx->d_strag_ctr, 2 * sizeof(unsigned)));
+        LZR_CHECK(hipMalloc((void **)&ctx->d_strag_ctr, 2 * kClasses * sizeof(unsigned)));
+        for (int c = 0; c < kClasses - 1; ++c) {
+            LZR_CHECK(hipStreamCreateWithFlags(&ctx->side[c], hipStreamNonBlocking));
+            LZR_CHECK(hipEventCreateWithFlags(&ctx->join[c], hipEventDisableTiming));
+        }
+        LZR_CHECK(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
     } catch (const std::string &e) {
         g_create_error = "lzr_create: " + e;
         lzr_destroy(ctx); /* frees whatever was created */
@@ -853,6 +879,11 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_park_ctr);
     (void)hipFree(ctx->d_strag_ctr);
+    for (int c = 0; c < kClasses - 1; ++c) {
+        if (ctx->side[c]) (void)hipStreamDestroy(ctx->side[c]);
+        if (ctx->join[c]) (void)hipEventDestroy(ctx->join[c]);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     ctx->order.destroy();
     (void)hipFree(ctx->d_strag);
     for (int c = 0; c < kClasses; ++c)
