@@ -859,6 +859,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
     }
     ctx->cfg.initial_psf_width = initial_psf_width;
     ctx->cfg.sigma = sigma;
+    ctx->cfg.inv_sigma = 1.0 / sigma;
     ctx->cfg.epsabs = 1e-4;
     ctx->cfg.epsrel = 1e-4;
     ctx->cfg.max_iterations = 200;

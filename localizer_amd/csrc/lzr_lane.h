@@ -60,6 +60,7 @@ enum Phase : int { kIdle = 0, kInit = 1, kTrial = 2, kFinished = 3 };
 struct LaneConfig {
     double initial_psf_width; /* w */
     double sigma;             /* residual weight */
+    double inv_sigma;         /* 1.0 / sigma, rounded once on the host */
     double epsabs, epsrel;    /* test_delta tolerances */
     int max_iterations;       /* 200 */
     int default_half_width;   /* window half-width when a candidate gives none */
@@ -102,6 +103,7 @@ struct LaneState {
     float H[NP<P>::S]; /* J^T J at th (packed)                      */
     float g[P];        /* J^T f at th                               */
     double delta, par, fnorm, xnorm;
+    double pnorm;   /* |D dx| of the proposed step (lmpar's dxnorm) */
     int xoff, yoff, n, frame;
     int roi;
     int phase;
@@ -555,10 +557,10 @@ LZR_HD LZR_INLINE double plain_norm(const double (&v)[P]) {
 }
 
 /* Levenberg-Marquardt parameter (MINPACK lmpar on the normal equations).
- * Returns x with (H + par D^2) x = g; the LM step is -x. */
+ * Returns x with (H + par D^2) x = g (the LM step is -x) and |D x| in dxnorm_out. */
 template <int P>
 LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], const double (&diag)[P],
-                             double delta, double &par_io, double (&x)[P]) {
+                             double delta, double &par_io, double (&x)[P], double &dxnorm_out) {
     constexpr int S = NP<P>::S;
     double L[S], id[P], y[P], u[P];
     /* Gauss-Newton step.  A Cholesky breakdown is MINPACK's rank-deficient case: the
@@ -573,6 +575,7 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
         fp = dxnorm - delta;
         if (fp <= 0.1 * delta) {
             par_io = 0.0;
+            dxnorm_out = dxnorm;
             return;
         }
     } else {
@@ -634,6 +637,7 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
         par = dmax(parl, par + parc);
     }
     par_io = par;
+    dxnorm_out = dxnorm;
 }
 
 /* ---- model-specific pieces -------------------------------------------------- */
@@ -761,12 +765,11 @@ LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7]
 /* Column norms of the first `rows` Jacobian rows at t (GSL 1.16 update_diag reads the
  * first p rows of J).  Pixel k = i + n j, window coordinates; w is the fixed width. */
 template <int P>
-LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double sigma, double w, int rows,
+LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double is, double w, int rows,
                                             double (&cn)[P]) {
 #pragma unroll
     for (int c = 0; c < P; ++c) cn[c] = 0.0;
-    /* reciprocals once per call; the rows then take products only */
-    const double is = 1.0 / sigma;
+    /* reciprocals once per call (is = 1/sigma); the rows then take products only */
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     if constexpr (P == 4) {
         r0 = 1.0 / (w * w); /* 1/w^2 */
@@ -779,8 +782,7 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
         r2 = 1.0 / (1.0 - t[5] * t[5]);     /* 1/(1 - rho^2) */
         r3 = 0.5 * r2 * is;                 /* 1/den */
     }
-    for (int k = 0; k < rows; ++k) {
-        const int i = k % n, j = k / n;
+    for (int k = 0, i = 0, j = 0; k < rows; ++k) {
         double row[P];
         if constexpr (P == 4) {
             const double A = t[0];
@@ -822,6 +824,11 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
         }
 #pragma unroll
         for (int c = 0; c < P; ++c) cn[c] += row[c] * row[c];
+        /* pixel k + 1 = i + n j, without a division */
+        if (++i == n) {
+            i = 0;
+            ++j;
+        }
     }
 #pragma unroll
     for (int c = 0; c < P; ++c) cn[c] = dsqrt(cn[c] == 0.0 ? 1.0 : cn[c]);
@@ -829,8 +836,7 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
 
 /* Full normal equations from the pass accumulators. */
 template <int P, typename Acc>
-LZR_HD LZR_INLINE void assemble(const PassAcc<P, Acc> &a, int npix, double sigma, float (&H)[NP<P>::S], float (&g)[P]) {
-    const double is = 1.0 / sigma;
+LZR_HD LZR_INLINE void assemble(const PassAcc<P, Acc> &a, int npix, double is, float (&H)[NP<P>::S], float (&g)[P]) {
 #pragma unroll
     for (int i = 0; i < P - 1; ++i) {
 #pragma unroll
@@ -979,10 +985,11 @@ LZR_HD LZR_INLINE void propose(LaneState<P> &s) {
     double H[NP<P>::S], g[P], x[P];
     to_double<P>(s.H, s.g, H, g);
     s.inner++;
-    lmpar<P>(H, g, s.diag, s.delta, s.par, x);
+    double pnorm; /* |D dx| = |D x|, bit for bit */
+    lmpar<P>(H, g, s.diag, s.delta, s.par, x, pnorm);
 #pragma unroll
     for (int i = 0; i < P; ++i) s.dx[i] = -x[i];
-    const double pnorm = scaled_norm<P>(s.diag, s.dx);
+    s.pnorm = pnorm;
     if (s.iter_gsl == 1 && pnorm < s.delta) s.delta = pnorm;
     s.phase = kTrial;
 }
@@ -1046,7 +1053,7 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
     const int npix = s.n * s.n;
     if (s.phase == kInit) {
         /* lmsder set(): f, J at the start point */
-        assemble<P, Acc>(a, npix, cfg.sigma, s.H, s.g);
+        assemble<P, Acc>(a, npix, cfg.inv_sigma, s.H, s.g);
         s.fnorm = dsqrt((double)a.sse0);
 #pragma unroll
         for (int i = 0; i < P; ++i) {
@@ -1087,7 +1094,7 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
         for (int j = 0; j < P; ++j) row += H[tri(i, j)] * s.dx[j];
         jdx2 += s.dx[i] * row;
     }
-    const double pnorm = scaled_norm<P>(s.diag, s.dx);
+    const double pnorm = s.pnorm; /* D is unchanged since the step was proposed */
     const double t1sq = dmax(jdx2, 0.0) * inv_fn2;
     const double t2sq = s.par * pnorm * pnorm * inv_fn2; /* (sqrt(par) |D dx| / |f|)^2 */
     const double prered = t1sq + t2sq / 0.5;
@@ -1110,7 +1117,7 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
 #pragma unroll
         for (int i = 0; i < P; ++i) s.th[i] += s.dx[i];
         s.accepted++;
-        assemble<P, Acc>(a, npix, cfg.sigma, s.H, s.g);
+        assemble<P, Acc>(a, npix, cfg.inv_sigma, s.H, s.g);
         double absx[P];
         absolute<P>(s, s.th, absx);
         s.xnorm = scaled_norm<P>(s.diag, absx);
@@ -1125,7 +1132,7 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
                 cn[i] = dsqrt(hii == 0.0 ? 1.0 : hii);
             }
         } else {
-            partial_column_norms<P>(s.th, s.n, cfg.sigma, cfg.initial_psf_width, rows, cn);
+            partial_column_norms<P>(s.th, s.n, cfg.inv_sigma, cfg.initial_psf_width, rows, cn);
         }
 #pragma unroll
         for (int i = 0; i < P; ++i)

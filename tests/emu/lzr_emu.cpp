@@ -49,6 +49,7 @@ extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, i
     LaneConfig cfg;
     cfg.initial_psf_width = w;
     cfg.sigma = sigma;
+    cfg.inv_sigma = 1.0 / sigma;
     cfg.epsabs = epsabs;
     cfg.epsrel = epsrel;
     cfg.max_iterations = max_iter;
