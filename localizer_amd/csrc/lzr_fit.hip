@@ -339,19 +339,9 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         }
 
         /* ---- one pixel pass ---- */
-        PassIn<P> q;
         PassAcc<P, Acc> acc;
         acc.zero();
         const int n = active ? s.n : 0;
-        if (active) {
-            lane_pass_in<P>(s, args.cfg, q);
-        } else {
-            /* idle lanes run the loop on benign constants; their sums are discarded */
-            double z[P];
-#pragma unroll
-            for (int k = 0; k < P; ++k) z[k] = 1.0;
-            make_pass_in(z, z, 1.0, 1.0, q);
-        }
         const int nw = __builtin_amdgcn_readfirstlane(wave_max(n)); /* wave-uniform loop bound */
         const bool mixed = __any(active && n != nw);
         struct {
@@ -365,7 +355,27 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             __device__ void set(int i, double v) { c[i * 64] = v; }
             __device__ double get(int i) const { return c[i * 64]; }
         } xb{xcol + lane};
-        window_pass(q, n, nw, mixed, pix, xb, acc);
+        /* idle lanes run the loop on benign constants; their sums are discarded */
+        double z[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) z[k] = 1.0;
+        if constexpr (P == 7) {
+            PassIn<P> q;
+            if (active)
+                lane_pass_in<P>(s, args.cfg, q);
+            else
+                make_pass_in(z, z, 1.0, 1.0, q);
+            window_pass(q, n, nw, mixed, pix, xb, acc);
+        } else {
+            SepIn q;
+            if (active)
+                lane_sep_in<P>(s, args.cfg, q);
+            else
+                make_sep_in(z, z, 1.0, 1.0, q);
+            double v[kSepSums];
+            sep_pass(q, n, nw, mixed, pix, xb, v);
+            sep_finish<P, Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, acc);
+        }
 
         /* ---- LM logic ---- */
         bool hand_off = false;
@@ -527,49 +537,73 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
         const bool active = s.phase == kTrial;
         if (!__ballot(active)) break;
 
-        PassIn<P> q;
         PassAcc<P, Acc> acc;
         acc.zero();
         const int n = active ? s.n : 1;
-        if (active) {
-            lane_pass_in<P>(s, args.cfg, q);
-        } else {
-            double z[P];
-#pragma unroll
-            for (int k = 0; k < P; ++k) z[k] = 1.0;
-            make_pass_in(z, z, 1.0, 1.0, q);
-        }
         const int n2 = n * n, inv = 65536 / n + 1;
-        if constexpr (P != 7) {
-            /* separable factors, slots t < n: exp(-c (t - x0)^2), n <= t < 2n: y */
-            for (int t = r; t < 2 * n; t += kGroup) {
-                const bool isx = t < n;
-                const double u = (double)((float)(isx ? t : t - n) - (isx ? q.x0 : q.y0));
-                xcol[(t >> lk) * 64 + lane] = dexp(-q.c0d * u * u);
-            }
-        }
-        __syncthreads();
         const int mmax = __builtin_amdgcn_readfirstlane(wave_max((n2 + kMask) >> lk));
         const double *gx = xcol + grp * kGroup;
-        for (int m = 0; m < mmax; ++m) {
-            const int pp = r + kGroup * m;
-            if (pp < n2) {
-                const int j = (pp * inv) >> 16, i = pp - j * n;
-                const float D = (float)tile[m * 64 + lane];
-                if constexpr (P == 7) {
-                    pixel_ell(q, (float)i, (float)j, D, acc);
-                } else {
-                    const int ty = n + j;
-                    const double ex = gx[(i >> lk) * 64 + (i & kMask)];
-                    const double ey = gx[(ty >> lk) * 64 + (ty & kMask)];
-                    const float uy0 = (float)j - q.y0;
-                    const float uy1 = uy0 - q.dyc;
-                    const float ddy = -q.dyc * (uy1 + uy0);
-                    pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, D, ex * ey, acc);
+        double z[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) z[k] = 1.0;
+        if constexpr (P == 7) {
+            PassIn<P> q;
+            if (active)
+                lane_pass_in<P>(s, args.cfg, q);
+            else
+                make_pass_in(z, z, 1.0, 1.0, q);
+            __syncthreads();
+            for (int m = 0; m < mmax; ++m) {
+                const int pp = r + kGroup * m;
+                if (pp < n2) {
+                    const int j = (pp * inv) >> 16, i = pp - j * n;
+                    pixel_ell(q, (float)i, (float)j, (float)tile[m * 64 + lane], acc);
                 }
             }
+            group_reduce<P, Acc>(acc, kGroup);
+        } else {
+            /* separable pass split over the group: lane r computes the factors of slots
+             * t = r, r + k, ... (t < n: column t, else row t - n) with their 1-D moments and
+             * the data sums of pixels r, r + k, ...; the butterfly adds the partial sums */
+            SepIn q;
+            if (active)
+                lane_sep_in<P>(s, args.cfg, q);
+            else
+                make_sep_in(z, z, 1.0, 1.0, q);
+            double v[kSepSums];
+#pragma unroll
+            for (int k = 0; k < kSepSums; ++k) v[k] = 0.0;
+            for (int t = r; t < 2 * n; t += kGroup) {
+                const bool isx = t < n;
+                const double u = (double)(isx ? t : t - n) - (isx ? q.x1 : q.y1);
+                const double e = dexp(-(q.c1 * u) * u);
+                xcol[(t >> lk) * 64 + lane] = e;
+                double m8[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) m8[k] = 0.0;
+                axis_moments(e, u, m8);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    v[kMx0 + k] += isx ? m8[k] : 0.0;
+                    v[kMy0 + k] += isx ? 0.0 : m8[k];
+                }
+            }
+            __syncthreads();
+            for (int m = 0; m < mmax; ++m) {
+                const int pp = r + kGroup * m;
+                if (pp < n2) {
+                    const int j = (pp * inv) >> 16, i = pp - j * n;
+                    const int ty = n + j;
+                    const double e = gx[(i >> lk) * 64 + (i & kMask)] * gx[(ty >> lk) * 64 + (ty & kMask)];
+                    sep_pixel(e, (double)i - q.x1, (double)j - q.y1, (double)tile[m * 64 + lane], v);
+                }
+            }
+            if constexpr (kGroup > 1) {
+#pragma unroll
+                for (int k = 0; k < kSepSums; ++k) v[k] = group_sum(v[k], kGroup);
+            }
+            sep_finish<P, Acc>(q, n2, v, s.phase == kInit, s.fnorm * s.fnorm, acc);
         }
-        group_reduce<P, Acc>(acc, kGroup);
 
         if (active) {
             LzrRawFit *o = r == 0 ? reinterpret_cast<LzrRawFit *>(args.out + s.roi) : nullptr;

@@ -15,16 +15,18 @@
  *   rank deficiency (nsing < p)     Cholesky breakdown
  * so the trial points, trust radius, Levenberg-Marquardt parameter, acceptance
  * ratio, status codes and the reference driver's test_delta / 200-iteration loop
- * are those of the reference.  The residuals, Jacobian and normal-equation
- * accumulation run per pixel in FP32 (FP64 accumulation optional, template Acc);
- * the p x p solves run in FP64.
+ * are those of the reference.  The p x p solves run in FP64.
  *
- * Numerics of the acceptance test.  Near convergence the actual reduction
- * 1 - |f1|^2/|f0|^2 is ~1e-7, below FP32 resolution of |f|^2.  The pass therefore
- * evaluates the residual change f1 - f0 = M(theta1) - M(theta0) directly: the change
- * of the Gaussian exponent is formed from the parameter step (no cancellation) and
- * the exponential change as e0 * expm1(-d_arg); the SSE change is
- * sum (f1 - f0)(f1 + f0).  That keeps the ratio test accurate to ~1e-3 relative.
+ * Pixel passes.  Symmetric and fixed-width models (separable exponential): the pass
+ * at the trial point is a set of FP64 1-D moments over the window's columns and rows
+ * plus five FP64 data sums over the pixels, from which J^T J, J^T f and |f|^2 follow
+ * in closed form (sep_pass / sep_finish; about a dozen FP64 operations per pixel).
+ * Ellipsoidal model: residuals, Jacobian and normal equations per pixel in FP32 with
+ * FP64 accumulation (template Acc).  Near convergence the actual reduction
+ * 1 - |f1|^2/|f0|^2 is ~1e-7, below FP32 resolution of |f|^2, so that pass evaluates
+ * the residual change f1 - f0 = M(theta1) - M(theta0) directly: the change of the
+ * Gaussian exponent is formed from the parameter step (no cancellation) and the
+ * exponential change as e0 * expm1(-d_arg); the SSE change is sum (f1 - f0)(f1 + f0).
  *
  * The file compiles as HIP device code (kernel) and as plain C++ (the CPU
  * emulation used by the tests to check the logic against the FP64 oracle).
@@ -116,23 +118,6 @@ struct LaneState {
 /* FP32 constants of one pixel pass: current point theta0, trial point theta1. */
 template <int P>
 struct PassIn;
-
-template <>
-struct PassIn<5> {
-    float A0, x0, y0, b0, c0; /* c = 1/(2 r^2) */
-    float dA, dxc, dyc, db, dc;
-    float A1, kr, kx, inv_sigma;
-    double A0d, b0d, c0d, inv_sigma_d; /* FP64 copies for the current-point residual */
-};
-
-/* Fixed width (FitPositions_FixedWidthGaussian): c = 1/(2 w^2) is a constant. */
-template <>
-struct PassIn<4> {
-    float A0, x0, y0, b0, c0;
-    float dA, dxc, dyc, db;
-    float A1, kx, inv_sigma;
-    double A0d, b0d, c0d, inv_sigma_d;
-};
 
 template <>
 struct PassIn<7> {
@@ -252,81 +237,6 @@ LZR_HD LZR_INLINE void residual0(double qd_arg0, float arg0_f, double A0d, doubl
     }
 }
 
-/* One pixel of the symmetric model.  ux0/uy0: pixel minus current centre (exact in
- * FP32), e0d: exp(-c0 (ux0^2 + uy0^2)) in FP64 (from separable factors). */
-template <typename Acc>
-LZR_HD LZR_INLINE void pixel_sym(const PassIn<5> &q, float ux0, float uy0, float uy1, float ddy, float D, double e0d,
-                                 PassAcc<5, Acc> &a) {
-    LZR_NO_CONTRACT
-    const float ux1 = ux0 - q.dxc;
-    const float d0 = ffma(ux0, ux0, uy0 * uy0);
-    const float d1 = ffma(ux1, ux1, uy1 * uy1);
-    const float arg0 = d0 * q.c0;
-    const float e0 = (float)e0d;
-    const float f0 = (float)(dfma(q.A0d, e0d, q.b0d - (double)D) * q.inv_sigma_d);
-    /* exponent change, free of cancellation: (d1-d0) c1 + d0 (c1-c0) */
-    const float dd = ffma(-q.dxc, ux1 + ux0, ddy);
-    const float darg = ffma(dd, q.c0 + q.dc, d0 * q.dc);
-    float e1, e0m;
-    exp_change(e0, arg0 + darg, darg, e1, e0m);
-    const float df = ffma(q.dA, e1, ffma(q.A0, e0m, q.db)) * q.inv_sigma;
-    const float f1 = f0 + df;
-    a.sse0 = acc_fma(Acc(f0), Acc(f0), a.sse0);
-    a.dsse = acc_fma(Acc(df), Acc(f0 + f1), a.dsse);
-    /* Jacobian at the trial point, reference columns (A, r, x0, y0) */
-    const float jA = e1 * q.inv_sigma;
-    const float ae = q.A1 * e1;
-    const float kae = q.kx * ae;
-    const float jr = (q.kr * ae) * d1;
-    const float jx = kae * ux1;
-    const float jy = kae * uy1;
-    const Acc J[4] = {Acc(jA), Acc(jr), Acc(jx), Acc(jy)};
-    const Acc F = Acc(f1);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] = acc_fma(J[i], J[j], a.hq[tri(i, j)]);
-        a.sj[i] += J[i];
-        a.gq[i] = acc_fma(J[i], F, a.gq[i]);
-    }
-    a.sf += F;
-}
-
-/* One pixel of the fixed-width model: the symmetric pixel with dc = 0 and no width
- * column (source/Localization.cpp:1183-1225 keeps columns A, x0, y0, b). */
-template <typename Acc>
-LZR_HD LZR_INLINE void pixel_sym(const PassIn<4> &q, float ux0, float uy0, float uy1, float ddy, float D, double e0d,
-                                 PassAcc<4, Acc> &a) {
-    LZR_NO_CONTRACT
-    const float ux1 = ux0 - q.dxc;
-    const float d0 = ffma(ux0, ux0, uy0 * uy0);
-    const float arg0 = d0 * q.c0;
-    const float e0 = (float)e0d;
-    const float f0 = (float)(dfma(q.A0d, e0d, q.b0d - (double)D) * q.inv_sigma_d);
-    const float darg = ffma(-q.dxc, ux1 + ux0, ddy) * q.c0;
-    float e1, e0m;
-    exp_change(e0, arg0 + darg, darg, e1, e0m);
-    const float df = ffma(q.dA, e1, ffma(q.A0, e0m, q.db)) * q.inv_sigma;
-    const float f1 = f0 + df;
-    a.sse0 = acc_fma(Acc(f0), Acc(f0), a.sse0);
-    a.dsse = acc_fma(Acc(df), Acc(f0 + f1), a.dsse);
-    const float jA = e1 * q.inv_sigma;
-    const float ae = q.A1 * e1;
-    const float kae = q.kx * ae;
-    const float jx = kae * ux1;
-    const float jy = kae * uy1;
-    const Acc J[3] = {Acc(jA), Acc(jx), Acc(jy)};
-    const Acc F = Acc(f1);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-#pragma unroll
-        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] = acc_fma(J[i], J[j], a.hq[tri(i, j)]);
-        a.sj[i] += J[i];
-        a.gq[i] = acc_fma(J[i], F, a.gq[i]);
-    }
-    a.sf += F;
-}
-
 template <typename Acc>
 LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D, PassAcc<7, Acc> &a) {
     LZR_NO_CONTRACT
@@ -371,62 +281,10 @@ LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D
     a.sf += F;
 }
 
-/* One pass over a lane's window: residuals at the current point, residual change and
- * Jacobian at the trial point, normal equations.  nloop >= n is the loop bound shared
- * by the wave; `guard` masks pixels beyond the lane's own n (mixed window sizes).
- * pix(i, j) returns pixel (i, j) as float, pix.raw(i, j) as stored; xb holds one FP64
- * value per column.
- * Symmetric and fixed-width models: exp(-c0 (ux^2 + uy^2)) = Ex[i] Ey[j], 2n FP64
- * exponentials per pass. */
-template <int P, typename Acc, typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void window_pass_separable(const PassIn<P> &q, int n, int nloop, bool guard, const Pix &pix,
-                                             XBuf &xb, PassAcc<P, Acc> &a) {
-    for (int i = 0; i < nloop; ++i) {
-        const double ux = (double)((float)i - q.x0);
-        xb.set(i, dexp(-q.c0d * ux * ux));
-    }
-    for (int j = 0; j < nloop; ++j) {
-        const float uy0 = (float)j - q.y0;
-        const float uy1 = uy0 - q.dyc;
-        const float ddy = -q.dyc * (uy1 + uy0);
-        const double Ey = dexp(-q.c0d * (double)uy0 * (double)uy0);
-        /* the next pixel's LDS reads are issued before this pixel's arithmetic */
-        auto Dn = pix.raw(0, j); /* converted at use, a full pixel later */
-        double xn = xb.get(0);
-        if (guard) {
-            for (int i = 0; i < nloop; ++i) {
-                const float D = (float)Dn;
-                const double xv = xn;
-                const int inext = i + 1 < nloop ? i + 1 : i; /* unconditional, in-bounds read */
-                Dn = pix.raw(inext, j);
-                xn = xb.get(inext);
-                if (i < n && j < n) pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, D, xv * Ey, a);
-            }
-        } else {
-            for (int i = 0; i < nloop; ++i) {
-                const float D = (float)Dn;
-                const double xv = xn;
-                const int inext = i + 1 < nloop ? i + 1 : i;
-                Dn = pix.raw(inext, j);
-                xn = xb.get(inext);
-                pixel_sym(q, (float)i - q.x0, uy0, uy1, ddy, D, xv * Ey, a);
-            }
-        }
-    }
-}
-
-template <typename Acc, typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void window_pass(const PassIn<5> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
-                                   PassAcc<5, Acc> &a) {
-    window_pass_separable<5>(q, n, nloop, guard, pix, xb, a);
-}
-
-template <typename Acc, typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void window_pass(const PassIn<4> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
-                                   PassAcc<4, Acc> &a) {
-    window_pass_separable<4>(q, n, nloop, guard, pix, xb, a);
-}
-
+/* One pass of the ellipsoidal model over a lane's window: residuals at the current point,
+ * residual change and Jacobian at the trial point, normal equations.  nloop >= n is the
+ * loop bound shared by the wave; `guard` masks pixels beyond the lane's own n (mixed
+ * window sizes); pix(i, j) returns pixel (i, j) as float. */
 template <typename Acc, typename Pix, typename XBuf>
 LZR_HD LZR_INLINE void window_pass(const PassIn<7> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &,
                                    PassAcc<7, Acc> &a) {
@@ -656,58 +514,6 @@ LZR_HD LZR_INLINE void absolute(const LaneState<P> &s, const double (&t)[P], dou
     a[x_index<P>() + 1] += (double)s.yoff;
 }
 
-/* w: the fixed width of the 4-parameter model (unused by the others). */
-LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[4], const double (&dx)[4], double sigma, double w,
-                                    PassIn<4> &q) {
-    const double iw = drcp(w), is = drcp(sigma);
-    const double c0 = 0.5 * iw * iw;
-    q.A0 = (float)t0[0];
-    q.x0 = (float)t0[1];
-    q.y0 = (float)t0[2];
-    q.b0 = (float)t0[3];
-    q.c0 = (float)c0;
-    q.dA = (float)dx[0];
-    q.dxc = (float)dx[1];
-    q.dyc = (float)dx[2];
-    q.db = (float)dx[3];
-    q.A1 = (float)(t0[0] + dx[0]);
-    q.inv_sigma = (float)is;
-    q.A0d = t0[0];
-    q.b0d = t0[3];
-    q.c0d = c0;
-    q.inv_sigma_d = is;
-    q.kx = (float)(iw * iw * is);
-}
-
-LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[5], const double (&dx)[5], double sigma, double,
-                                    PassIn<5> &q) {
-    const double r1 = t0[1] + dx[1];
-    const double A1 = t0[0] + dx[0];
-    const double ir0 = drcp(t0[1]), ir1 = drcp(r1);
-    const double c0 = 0.5 * ir0 * ir0, c1 = 0.5 * ir1 * ir1;
-    const double is = drcp(sigma);
-    q.A0 = (float)t0[0];
-    q.x0 = (float)t0[2];
-    q.y0 = (float)t0[3];
-    q.b0 = (float)t0[4];
-    q.c0 = (float)c0;
-    q.dA = (float)dx[0];
-    q.dxc = (float)dx[2];
-    q.dyc = (float)dx[3];
-    q.db = (float)dx[4];
-    q.dc = (float)(c1 - c0);
-    q.A1 = (float)A1;
-    q.inv_sigma = (float)is;
-    q.A0d = t0[0];
-    q.b0d = t0[4];
-    q.c0d = c0;
-    q.inv_sigma_d = is;
-    /* d f/d r = A e d^2 / (sqrt(2) r^3 sigma)   (source/Localization.cpp:1151-1166)
-     * d f/d x0 = A e (x - x0) / (r^2 sigma)                                        */
-    q.kr = (float)(ir1 * ir1 * ir1 * is * (1.0 / kSqrt2));
-    q.kx = (float)(ir1 * ir1 * is);
-}
-
 LZR_HD LZR_INLINE void quad_coeffs(double sx, double sy, double rho, double &qa, double &qc, double &qb) {
     const double pre = 0.5 * drcp(1.0 - rho * rho);
     const double isx = drcp(sx), isy = drcp(sy);
@@ -760,6 +566,203 @@ LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7]
     q.k8 = (float)(t * isx * isy);
     q.k9 = (float)(t * iomr * rho * isx * isy);
     q.two_rho = (float)(2.0 * rho);
+}
+
+/* ---- separable-moment pass (symmetric and fixed-width models) -----------------------
+ * At the trial point the model's exponential is separable, e1(i, j) = Ex(i) Ey(j) with
+ * Ex(i) = exp(-c1 u^2), u = i - x1 (and Ey over rows, v = j - y1).  Every sum of a pass
+ * that does not involve the data is therefore a product of 1-D moments over the window's
+ * columns and rows,
+ *   M_a = sum Ex^2 u^a (a = 0..4),   N_a = sum Ex u^a (a = 0..2),
+ * and only five data sums need the 2-D loop over pixels:
+ *   U1 = sum e1 D, Uu = sum e1 u D, Uv = sum e1 v D, Ud = sum e1 (u^2 + v^2) D,
+ *   SD = sum D, SDD = sum D^2.
+ * J^T J, J^T 1, J^T f1, sum f1 and |f1|^2 then follow in closed form, all in FP64
+ * (sep_finish); |f0|^2 is the current point's |f|^2, known from the step that moved there.
+ * Slots of the sums array: */
+enum SepSum : int {
+    kMx0 = 0,  /* M_0..M_4 over columns, then N_0..N_2 */
+    kMy0 = 8,  /* the same over rows */
+    kU1 = 16,
+    kUu = 17,
+    kUv = 18,
+    kUd = 19,
+    kSD = 20,
+    kSDD = 21,
+    kSepSums = 22
+};
+
+/* FP64 constants of a separable pass. */
+struct SepIn {
+    double x1, y1, c1; /* trial centre (window coordinates), c1 = 1/(2 r1^2) */
+    double A1, b1, is; /* trial amplitude and offset, 1/sigma */
+    double kr, kx;     /* J columns: d f/d r = kr A1 e1 d^2, d f/d x0 = kx A1 e1 u */
+};
+
+/* Trial point t0 + dx (dx = 0 for the start point).  Symmetric: (A, r, x0, y0, b),
+ * d f/d r = A e d^2 / (sqrt(2) r^3 sigma), d f/d x0 = A e (x - x0) / (r^2 sigma)
+ * (source/Localization.cpp:1151-1166); fixed width: (A, x0, y0, b) with r = w. */
+LZR_HD LZR_INLINE void make_sep_in(const double (&t0)[5], const double (&dx)[5], double is, double, SepIn &q) {
+    const double ir1 = drcp(t0[1] + dx[1]);
+    q.A1 = t0[0] + dx[0];
+    q.x1 = t0[2] + dx[2];
+    q.y1 = t0[3] + dx[3];
+    q.b1 = t0[4] + dx[4];
+    q.c1 = 0.5 * ir1 * ir1;
+    q.is = is;
+    q.kr = ir1 * ir1 * ir1 * is * (1.0 / kSqrt2);
+    q.kx = ir1 * ir1 * is;
+}
+LZR_HD LZR_INLINE void make_sep_in(const double (&t0)[4], const double (&dx)[4], double is, double w, SepIn &q) {
+    const double iw = drcp(w);
+    q.A1 = t0[0] + dx[0];
+    q.x1 = t0[1] + dx[1];
+    q.y1 = t0[2] + dx[2];
+    q.b1 = t0[3] + dx[3];
+    q.c1 = 0.5 * iw * iw;
+    q.is = is;
+    q.kr = 0.0;
+    q.kx = iw * iw * is;
+}
+
+/* Adds one column (or row) with factor e at offset u to the moments m[0..7]. */
+LZR_HD LZR_INLINE void axis_moments(double e, double u, double *m) {
+    LZR_NO_CONTRACT
+    const double e2 = e * e;
+    const double e2u = e2 * u, e2u2 = e2u * u, e2u3 = e2u2 * u;
+    const double eu = e * u;
+    m[0] += e2;
+    m[1] += e2u;
+    m[2] += e2u2;
+    m[3] += e2u3;
+    m[4] = dfma(e2u3, u, m[4]);
+    m[5] += e;
+    m[6] += eu;
+    m[7] = dfma(eu, u, m[7]);
+}
+
+/* Data sums of one pixel row j (ey = Ey(j), w = j - y1) from the row's inner sums
+ * r1 = sum_i Ex D, ru = sum_i Ex u D, ru2 = sum_i Ex u^2 D. */
+LZR_HD LZR_INLINE void row_sums(double ey, double w, double r1, double ru, double ru2, double rd, double rdd,
+                                double (&v)[kSepSums]) {
+    LZR_NO_CONTRACT
+    v[kU1] = dfma(ey, r1, v[kU1]);
+    v[kUu] = dfma(ey, ru, v[kUu]);
+    v[kUv] = dfma(ey * w, r1, v[kUv]);
+    v[kUd] = dfma(ey, dfma(w * w, r1, ru2), v[kUd]);
+    v[kSD] += rd;
+    v[kSDD] += rdd;
+}
+
+/* The separable pass of one lane over its n x n window (nloop >= n: the loop bound shared
+ * by the wave; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
+ * returns pixel (i, j) as stored, xb holds one FP64 value per column. */
+template <typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
+                                double (&v)[kSepSums]) {
+    LZR_NO_CONTRACT
+#pragma unroll
+    for (int k = 0; k < kSepSums; ++k) v[k] = 0.0;
+    for (int i = 0; i < nloop; ++i) {
+        const double u = (double)i - q.x1;
+        const double e = i < n ? dexp(-(q.c1 * u) * u) : 0.0;
+        xb.set(i, e);
+        axis_moments(e, u, v + kMx0);
+    }
+    for (int j = 0; j < nloop; ++j) {
+        const double w = (double)j - q.y1;
+        const double ey = j < n ? dexp(-(q.c1 * w) * w) : 0.0;
+        axis_moments(ey, w, v + kMy0);
+        double r1 = 0.0, ru = 0.0, ru2 = 0.0, rd = 0.0, rdd = 0.0;
+        /* the next pixel's LDS reads are issued before this pixel's arithmetic */
+        auto Dn = pix.raw(0, j);
+        double xn = xb.get(0);
+        if (guard) {
+            const bool row_in = j < n;
+            for (int i = 0; i < nloop; ++i) {
+                const double D = row_in && i < n ? (double)Dn : 0.0;
+                const double ex = xn;
+                const int inext = i + 1 < nloop ? i + 1 : i; /* unconditional, in-bounds read */
+                Dn = pix.raw(inext, j);
+                xn = xb.get(inext);
+                const double u = (double)i - q.x1;
+                const double t = ex * D;
+                const double tu = t * u;
+                r1 += t;
+                ru += tu;
+                ru2 = dfma(tu, u, ru2);
+                rd += D;
+                rdd = dfma(D, D, rdd);
+            }
+        } else {
+            for (int i = 0; i < nloop; ++i) {
+                const double D = (double)Dn;
+                const double ex = xn;
+                const int inext = i + 1 < nloop ? i + 1 : i;
+                Dn = pix.raw(inext, j);
+                xn = xb.get(inext);
+                const double u = (double)i - q.x1;
+                const double t = ex * D;
+                const double tu = t * u;
+                r1 += t;
+                ru += tu;
+                ru2 = dfma(tu, u, ru2);
+                rd += D;
+                rdd = dfma(D, D, rdd);
+            }
+        }
+        row_sums(ey, w, r1, ru, ru2, rd, rdd, v);
+    }
+}
+
+/* Normal equations and residual sums of a pass from its moments and data sums (npix
+ * pixels).  J columns in the reference's order without the constant offset column:
+ * (A, r, x0, y0) symmetric, (A, x0, y0) fixed width; f1 = (A1 e1 + b1 - D)/sigma.
+ * init: the pass is at the start point (|f0|^2 = |f1|^2); otherwise fn2 = |f0|^2. */
+template <int P, typename Acc>
+LZR_HD LZR_INLINE void sep_finish(const SepIn &q, int npix, const double (&v)[kSepSums], bool init, double fn2,
+                                  PassAcc<P, Acc> &a) {
+    LZR_NO_CONTRACT
+    static_assert(P == 4 || P == 5, "separable models only");
+    const double *mx = v + kMx0, *my = v + kMy0;
+    const double S1 = mx[0] * my[0], Su = mx[1] * my[0], Sv = mx[0] * my[1];
+    const double Suu = mx[2] * my[0], Suv = mx[1] * my[1], Svv = mx[0] * my[2];
+    const double T1 = mx[5] * my[5], Tu = mx[6] * my[5], Tv = mx[5] * my[6];
+    const double is = q.is, A1 = q.A1, b1 = q.b1, N = (double)npix;
+    const double bx = q.kx * A1;
+    constexpr int X = P == 5 ? 2 : 1; /* column of x0 */
+    a.hq[tri(0, 0)] = Acc(is * is * S1);
+    a.hq[tri(X, 0)] = Acc(is * bx * Su);
+    a.hq[tri(X, X)] = Acc(bx * bx * Suu);
+    a.hq[tri(X + 1, 0)] = Acc(is * bx * Sv);
+    a.hq[tri(X + 1, X)] = Acc(bx * bx * Suv);
+    a.hq[tri(X + 1, X + 1)] = Acc(bx * bx * Svv);
+    a.sj[0] = Acc(is * T1);
+    a.sj[X] = Acc(bx * Tu);
+    a.sj[X + 1] = Acc(bx * Tv);
+    a.gq[0] = Acc(is * is * ((A1 * S1 + b1 * T1) - v[kU1]));
+    a.gq[X] = Acc(is * bx * ((A1 * Su + b1 * Tu) - v[kUu]));
+    a.gq[X + 1] = Acc(is * bx * ((A1 * Sv + b1 * Tv) - v[kUv]));
+    if constexpr (P == 5) {
+        const double Sd = mx[2] * my[0] + mx[0] * my[2];
+        const double Sdu = mx[3] * my[0] + mx[1] * my[2];
+        const double Sdv = mx[2] * my[1] + mx[0] * my[3];
+        const double Sdd = (mx[4] * my[0] + 2.0 * (mx[2] * my[2])) + mx[0] * my[4];
+        const double Td = mx[7] * my[5] + mx[5] * my[7];
+        const double ar = q.kr * A1;
+        a.hq[tri(1, 0)] = Acc(is * ar * Sd);
+        a.hq[tri(1, 1)] = Acc(ar * ar * Sdd);
+        a.hq[tri(2, 1)] = Acc(ar * bx * Sdu);
+        a.hq[tri(3, 1)] = Acc(ar * bx * Sdv);
+        a.sj[1] = Acc(ar * Td);
+        a.gq[1] = Acc(is * ar * ((A1 * Sd + b1 * Td) - v[kUd]));
+    }
+    a.sf = Acc(is * ((A1 * T1 + N * b1) - v[kSD]));
+    /* |f1|^2 sigma^2 = A1 (A1 S1 + 2 b1 T1 - 2 U1) + b1 (N b1 - 2 SD) + SDD */
+    const double sse1 =
+        is * is * ((A1 * ((A1 * S1 + 2.0 * (b1 * T1)) - 2.0 * v[kU1]) + b1 * (N * b1 - 2.0 * v[kSD])) + v[kSDD]);
+    a.sse0 = Acc(init ? sse1 : 0.0);
+    a.dsse = Acc(init ? 0.0 : sse1 - fn2);
 }
 
 /* Column norms of the first `rows` Jacobian rows at t (GSL 1.16 update_diag reads the
@@ -965,6 +968,27 @@ LZR_HD LZR_INLINE void lane_pass_in(const LaneState<P> &s, const LaneConfig &cfg
 #pragma unroll
     for (int i = 0; i < P; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
     make_pass_in(s.th, step, cfg.sigma, cfg.initial_psf_width, q);
+}
+
+/* Constants of a separable pass at the lane's trial point (the start point at kInit). */
+template <int P>
+LZR_HD LZR_INLINE void lane_sep_in(const LaneState<P> &s, const LaneConfig &cfg, SepIn &q) {
+    double step[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
+    make_sep_in(s.th, step, cfg.inv_sigma, cfg.initial_psf_width, q);
+}
+
+/* One data sum term of pixel (u, w) with e = e1 and data D (the group-split pass). */
+LZR_HD LZR_INLINE void sep_pixel(double e, double u, double w, double D, double (&v)[kSepSums]) {
+    LZR_NO_CONTRACT
+    const double t = e * D;
+    v[kU1] += t;
+    v[kUu] = dfma(t, u, v[kUu]);
+    v[kUv] = dfma(t, w, v[kUv]);
+    v[kUd] = dfma(t, dfma(u, u, w * w), v[kUd]);
+    v[kSD] += D;
+    v[kSDD] = dfma(D, D, v[kSDD]);
 }
 
 template <int P>

@@ -1,6 +1,6 @@
 // Host build of the GPU lane logic (localizer_amd/csrc/lzr_lane.h) for the CPU tests:
-// every candidate runs the same state machine and the same FP32 pixel pass as one GPU
-// lane, sequentially.  TEST INFRASTRUCTURE ONLY -- the product never calls it.
+// every candidate runs the same state machine and the same pixel pass as one GPU lane,
+// sequentially.  TEST INFRASTRUCTURE ONLY -- the product never calls it.
 #include <cstring>
 
 #include "../../localizer_amd/csrc/lzr_lane.h"
@@ -19,8 +19,6 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) D[i + n * j] = (float)src[(size_t)j * nx + i];
     for (;;) {
-        PassIn<P> q;
-        lane_pass_in<P>(s, cfg, q);
         PassAcc<P, Acc> a;
         a.zero();
         struct {
@@ -34,7 +32,17 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             void set(int i, double x) { v[i] = x; }
             double get(int i) const { return v[i]; }
         } xb;
-        window_pass(q, n, n, false, pix, xb, a);
+        if constexpr (P == 7) {
+            PassIn<P> q;
+            lane_pass_in<P>(s, cfg, q);
+            window_pass(q, n, n, false, pix, xb, a);
+        } else {
+            SepIn q;
+            lane_sep_in<P>(s, cfg, q);
+            double v[kSepSums];
+            sep_pass(q, n, n, false, pix, xb, v);
+            sep_finish<P, Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
+        }
         if (!lane_after_pass<P, Acc>(s, cfg, a, model, reinterpret_cast<LzrRawFit *>(&out))) {
             finish_any(out, cfg, c);
             return;
