@@ -654,6 +654,34 @@ LZR_HD LZR_INLINE void row_sums(double ey, double w, double r1, double ru, doubl
     v[kSDD] += rdd;
 }
 
+/* Gaussian factors E(t) = exp(-c (t - x)^2), t = 0, 1, ..., by the recurrence
+ * E(t+1) = E(t) R(t), R(t+1) = R(t) Q with R(0) = exp(-c (1 - 2x)), Q = exp(-2c): two
+ * exponentials per axis (Q is shared) instead of one per column, ~t ulps of rounding.
+ * `ok` is false when an exponent at the ends of the window leaves +-690 (intermediate
+ * overflow or underflow, e.g. a collapsed width); the caller then evaluates each factor
+ * directly. */
+struct GaussRec {
+    double e, r, q;
+    bool ok;
+    LZR_HD LZR_INLINE void start(double c, double x, int n, double q_exp) {
+        LZR_NO_CONTRACT
+        const double xe = (double)(n - 1) - x;
+        const double a0 = (c * x) * x, a1 = (c * xe) * xe;
+        const double b0 = c * (1.0 - 2.0 * x), b1 = c * (2.0 * (double)n - 1.0 - 2.0 * x);
+        ok = c >= 0.0 && 2.0 * c < 690.0 && a0 < 690.0 && a1 < 690.0 && dabs(b0) < 690.0 && dabs(b1) < 690.0;
+        e = dexp(-a0);
+        r = dexp(-b0);
+        q = q_exp;
+    }
+    LZR_HD LZR_INLINE double next() {
+        LZR_NO_CONTRACT
+        const double v = e;
+        e = e * r;
+        r = r * q;
+        return v;
+    }
+};
+
 /* The separable pass of one lane over its n x n window (nloop >= n: the loop bound shared
  * by the wave; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
  * returns pixel (i, j) as stored, xb holds one FP64 value per column. */
@@ -663,15 +691,23 @@ LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, co
     LZR_NO_CONTRACT
 #pragma unroll
     for (int k = 0; k < kSepSums; ++k) v[k] = 0.0;
+    const double qexp = dexp(-2.0 * q.c1);
+    GaussRec gx, gy;
+    gx.start(q.c1, q.x1, n, qexp);
+    gy.start(q.c1, q.y1, n, qexp);
     for (int i = 0; i < nloop; ++i) {
         const double u = (double)i - q.x1;
-        const double e = i < n ? dexp(-(q.c1 * u) * u) : 0.0;
+        double e = gx.next();
+        if (!gx.ok) e = dexp(-(q.c1 * u) * u);
+        e = i < n ? e : 0.0;
         xb.set(i, e);
         axis_moments(e, u, v + kMx0);
     }
     for (int j = 0; j < nloop; ++j) {
         const double w = (double)j - q.y1;
-        const double ey = j < n ? dexp(-(q.c1 * w) * w) : 0.0;
+        double ey = gy.next();
+        if (!gy.ok) ey = dexp(-(q.c1 * w) * w);
+        ey = j < n ? ey : 0.0;
         axis_moments(ey, w, v + kMy0);
         double r1 = 0.0, ru = 0.0, ru2 = 0.0, rd = 0.0, rdd = 0.0;
         /* the next pixel's LDS reads are issued before this pixel's arithmetic */
