@@ -238,7 +238,11 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         /* ---- refill idle lanes from the queue ---- */
         while (open) {
             const unsigned long long want = __ballot(s.phase == kIdle);
-            if (!want) break;
+            /* A refill costs a queue atomic, the candidates' loads and a window gather,
+             * mostly latency: wait until kRefillMin lanes are idle (a lane's fit does not
+             * depend on when it starts).  500k 9x9 fits: 3.86 -> 3.73 ms. */
+            constexpr int kRefillMin = 8;
+            if (!want || __popcll(want) < kRefillMin) break;
             const int cnt = __popcll(want);
             const int leader = __ffsll((long long)want) - 1;
             unsigned base = 0;
