@@ -86,3 +86,51 @@ extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, i
     }
     return accepted;
 }
+
+// One separable pass (symmetric P = 5, fixed width P = 4) at theta + dx over an n x n
+// window D (x fast), looping to nloop >= n with the pixels beyond n masked as in a wave
+// of mixed window sizes.  out = [hq (packed lower, P-1 columns), sj, gq, sf, |f1|^2].
+template <int P>
+static void sep_one(const double *theta, const double *dx, double sigma, double w, int n, int nloop,
+                    const uint16_t *D, double *out) {
+    double t0[P], st[P];
+    for (int i = 0; i < P; ++i) {
+        t0[i] = theta[i];
+        st[i] = dx[i];
+    }
+    SepIn q;
+    make_sep_in(t0, st, 1.0 / sigma, w, q);
+    struct {
+        const uint16_t *D;
+        int n;
+        uint16_t raw(int i, int j) const { return i < n && j < n ? D[i + n * j] : (uint16_t)0; }
+    } pix{D, n};
+    struct {
+        double v[64];
+        void set(int i, double x) { v[i] = x; }
+        double get(int i) const { return v[i]; }
+    } xb;
+    double v[kSepSums];
+    sep_pass(q, n, nloop, nloop != n, pix, xb, v);
+    PassAcc<P, double> a;
+    a.zero();
+    sep_finish<P, double>(q, n * n, v, true, 0.0, a);
+    int k = 0;
+    for (int i = 0; i < NP<P>::SQ; ++i) out[k++] = a.hq[i];
+    for (int i = 0; i < P - 1; ++i) out[k++] = a.sj[i];
+    for (int i = 0; i < P - 1; ++i) out[k++] = a.gq[i];
+    out[k++] = a.sf;
+    out[k++] = a.sse0;
+}
+
+extern "C" int emu_sep_pass(int P, const double *theta, const double *dx, double sigma, double w, int n, int nloop,
+                            const uint16_t *D, double *out) {
+    if (n < 1 || nloop < n || nloop > 64) return -1;
+    if (P == 5)
+        sep_one<5>(theta, dx, sigma, w, n, nloop, D, out);
+    else if (P == 4)
+        sep_one<4>(theta, dx, sigma, w, n, nloop, D, out);
+    else
+        return -1;
+    return 0;
+}

@@ -19,6 +19,9 @@ def lib():
                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_int, ctypes.c_void_p]
         L.emu_fit_batch.restype = ctypes.c_int
+        L.emu_sep_pass.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.emu_sep_pass.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -36,3 +39,19 @@ def fit_batch(frames, cands, model, w, sigma=1.0, default_half_width=0, max_half
                         int(max_iter), float(epsabs), float(epsrel), int(bool(acc_double)) | (2 if f64_residual else 0), frames.ctypes.data,
                         nx, ny, nf, cands.ctypes.data, cands.size, out.ctypes.data)
     return out
+
+
+def sep_pass(theta, dx, sigma, w, window, nloop=None):
+    """Normal-equation sums of one separable pass (lzr_lane.h sep_pass + sep_finish) at
+    theta + dx over `window` (n x n, [j][i]); returns (hq, sj, gq, sf, sse)."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    dx = np.ascontiguousarray(dx, np.float64)
+    p = theta.size
+    window = np.ascontiguousarray(window, np.uint16)
+    n = window.shape[0]
+    sq = (p - 1) * p // 2
+    out = np.zeros(sq + 2 * (p - 1) + 2)
+    rc = lib().emu_sep_pass(p, theta.ctypes.data, dx.ctypes.data, float(sigma), float(w), n, int(nloop or n),
+                            window.ctypes.data, out.ctypes.data)
+    assert rc == 0
+    return out[:sq], out[sq:sq + p - 1], out[sq + p - 1:sq + 2 * (p - 1)], out[-2], out[-1]

@@ -1,0 +1,118 @@
+"""The separable-moment pass of the symmetric and fixed-width models
+(localizer_amd/csrc/lzr_lane.h sep_pass / sep_finish, host build) against brute-force
+per-pixel FP64 sums.
+
+The pass never forms the Jacobian: it combines 1-D moments of the Gaussian factors with
+five data sums.  Here the same quantities are summed pixel by pixel from the reference's
+model and Jacobian columns (source/Localization.cpp:1007-1048, :1128-1181; the width
+column is the true derivative / sqrt(2)), at trial points that exercise the factor
+recurrence, its direct-exponential fallback (collapsed widths, centres far outside the
+window) and the masking of a lane's window inside a larger loop bound."""
+import math
+
+import numpy as np
+import pytest
+
+from tests.emu import pyemu
+
+
+def _brute(theta, dx, sigma, w, window):
+    t = np.asarray(theta, np.float64) + np.asarray(dx, np.float64)
+    p = t.size
+    n = window.shape[0]
+    jj, ii = np.mgrid[0:n, 0:n].astype(np.float64)
+    D = window.astype(np.float64)
+    if p == 5:
+        A, r, x0, y0, b = t
+    else:
+        A, x0, y0, b = t
+        r = w
+    u, v = ii - x0, jj - y0
+    d2 = u * u + v * v
+    e = np.exp(-d2 / (2 * r * r))
+    f = (A * e + b - D) / sigma
+    cols = [e / sigma]
+    if p == 5:
+        cols.append(A * e * d2 / (math.sqrt(2.0) * r ** 3 * sigma))
+    cols += [A * e * u / (r * r * sigma), A * e * v / (r * r * sigma)]
+    J = np.stack([c.ravel() for c in cols], axis=1)
+    H = J.T @ J
+    hq = np.array([H[i, j] for i in range(p - 1) for j in range(i + 1)])
+    want = (hq, J.sum(axis=0), J.T @ f.ravel(), f.sum(), float(f.ravel() @ f.ravel()))
+    # magnitudes of the terms each sum cancels (model against data): the FP64 floor
+    fa = ((np.abs(A * e) + abs(b) + D) / sigma).ravel()
+    aJ = np.abs(J)
+    mag = (np.array([(aJ[:, i] * aJ[:, j]).sum() for i in range(p - 1) for j in range(i + 1)]), aJ.sum(axis=0),
+           aJ.T @ fa, fa.sum(), float(fa @ fa))
+    return want, mag
+
+
+def _check(theta, dx, sigma, w, window, nloop=None):
+    got = pyemu.sep_pass(theta, dx, sigma, w, window, nloop)
+    want, mag = _brute(theta, dx, sigma, w, window)
+    names = ("J^T J", "J^T 1", "J^T f", "sum f", "|f|^2")
+    for name, g, r, m in zip(names, got, want, mag):
+        g, r, m = np.atleast_1d(g), np.atleast_1d(r), np.atleast_1d(m)
+        assert np.all(np.isfinite(g)), name
+        # within 1e-12 of the summed term magnitudes (and 1e-11 relative where nothing cancels)
+        assert np.all(np.abs(g - r) <= 1e-11 * np.abs(r) + 1e-12 * m + 1e-300), (name, g, r, m)
+
+
+def _window(rng, n, A, r, x0, y0, b):
+    jj, ii = np.mgrid[0:n, 0:n]
+    lam = b + A * np.exp(-((ii - x0) ** 2 + (jj - y0) ** 2) / (2 * r * r))
+    return rng.poisson(lam).astype(np.uint16)
+
+
+@pytest.mark.parametrize("n", [5, 7, 9, 15])
+def test_symmetric_pass_matches_pixel_sums(n):
+    rng = np.random.default_rng(n)
+    for _ in range(25):
+        c = (n - 1) / 2
+        theta = [rng.uniform(100, 3000), rng.uniform(0.8, 2.5), c + rng.uniform(-1.5, 1.5), c + rng.uniform(-1.5, 1.5),
+                 rng.uniform(50, 200)]
+        dx = rng.normal(0, [30, 0.05, 0.2, 0.2, 3])
+        window = _window(rng, n, theta[0], theta[1], theta[2], theta[3], theta[4])
+        _check(theta, dx, 1.0, 1.3, window)
+        _check(theta, np.zeros(5), 1.0, 1.3, window)  # the start point (dx = 0)
+
+
+def test_fixed_width_pass_matches_pixel_sums():
+    rng = np.random.default_rng(4)
+    for n, w in ((5, 1.0), (9, 1.35), (11, 1.8)):
+        c = (n - 1) / 2
+        for _ in range(20):
+            theta = [rng.uniform(100, 3000), c + rng.uniform(-2, 2), c + rng.uniform(-2, 2), rng.uniform(50, 200)]
+            dx = rng.normal(0, [30, 0.2, 0.2, 3])
+            _check(theta, dx, 1.0, w, _window(rng, n, theta[0], w, theta[1], theta[2], theta[3]))
+
+
+def test_recurrence_fallback_and_extreme_trial_points():
+    """Exponents beyond the recurrence's safe range (a collapsed width: c = 1/(2 r^2) up to
+    5e3; centres far outside the window) take the direct exponentials; very wide spots
+    stay on the recurrence.  A residual weight sigma != 1 scales every column."""
+    rng = np.random.default_rng(9)
+    window = _window(rng, 9, 800, 1.3, 4.2, 3.7, 100)
+    for theta in ([800, 0.01, 4.2, 3.7, 100],   # c = 5000: direct factors, most underflow
+                  [800, 0.08, 4.5, 4.5, 100],   # c ~ 78: direct
+                  [800, 0.3, -6.0, 4.0, 100],   # centre 6 px left of the window
+                  [800, 1.3, 4.0, 30.0, 100],   # centre far below
+                  [800, 40.0, 4.0, 4.0, 100],   # nearly flat
+                  [800, 1.3, 4.0, 4.0, 100]):
+        _check(theta, np.zeros(5), 1.0, 1.3, window)
+        _check(theta, np.zeros(5), 2.5, 1.3, window)
+
+
+def test_masked_window_inside_a_larger_loop():
+    """A lane with an n x n window in a wave whose loop runs to nloop > n: columns and rows
+    beyond n must contribute nothing (the GPU masks them the same way)."""
+    rng = np.random.default_rng(3)
+    for n, nloop in ((5, 9), (7, 15), (9, 11)):
+        c = (n - 1) / 2
+        theta = [1500.0, 1.2, c + 0.3, c - 0.4, 120.0]
+        window = _window(rng, n, *theta)
+        full = pyemu.sep_pass(theta, np.zeros(5), 1.0, 1.3, window, n)
+        masked = pyemu.sep_pass(theta, np.zeros(5), 1.0, 1.3, window, nloop)
+        for a, b in zip(full, masked):
+            np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-9)
+        _check(theta, np.zeros(5), 1.0, 1.3, window, nloop)
