@@ -575,34 +575,50 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
             else
                 make_sep_in(z, z, 1.0, 1.0, q);
             double v[kSepSums];
+            if constexpr (kGroup == 1) {
+                /* one lane per fit: the first kernel's pass (factor recurrence, row sums) over
+                 * the same tile and column layout */
+                const int nw = __builtin_amdgcn_readfirstlane(wave_max(n));
+                const bool mixed = __any(active && n != nw);
+                struct {
+                    const uint16_t *t;
+                    int n;
+                    __device__ uint16_t raw(int i, int j) const { return t[(i + j * n) * 64]; }
+                } pix{tile + lane, n};
+                struct {
+                    double *c;
+                    __device__ void set(int i, double x) { c[i * 64] = x; }
+                    __device__ double get(int i) const { return c[i * 64]; }
+                } xb{xcol + lane};
+                sep_pass(q, n, nw, mixed, pix, xb, v);
+            } else {
 #pragma unroll
-            for (int k = 0; k < kSepSums; ++k) v[k] = 0.0;
-            for (int t = r; t < 2 * n; t += kGroup) {
-                const bool isx = t < n;
-                const double u = (double)(isx ? t : t - n) - (isx ? q.x1 : q.y1);
-                const double e = dexp(-(q.c1 * u) * u);
-                xcol[(t >> lk) * 64 + lane] = e;
-                double m8[8];
+                for (int k = 0; k < kSepSums; ++k) v[k] = 0.0;
+                for (int t = r; t < 2 * n; t += kGroup) {
+                    const bool isx = t < n;
+                    const double u = (double)(isx ? t : t - n) - (isx ? q.x1 : q.y1);
+                    const double e = dexp(-(q.c1 * u) * u);
+                    xcol[(t >> lk) * 64 + lane] = e;
+                    double m8[8];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) m8[k] = 0.0;
-                axis_moments(e, u, m8);
+                    for (int k = 0; k < 8; ++k) m8[k] = 0.0;
+                    axis_moments(e, u, m8);
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    v[kMx0 + k] += isx ? m8[k] : 0.0;
-                    v[kMy0 + k] += isx ? 0.0 : m8[k];
+                    for (int k = 0; k < 8; ++k) {
+                        v[kMx0 + k] += isx ? m8[k] : 0.0;
+                        v[kMy0 + k] += isx ? 0.0 : m8[k];
+                    }
                 }
-            }
-            __syncthreads();
-            for (int m = 0; m < mmax; ++m) {
-                const int pp = r + kGroup * m;
-                if (pp < n2) {
-                    const int j = (pp * inv) >> 16, i = pp - j * n;
-                    const int ty = n + j;
-                    const double e = gx[(i >> lk) * 64 + (i & kMask)] * gx[(ty >> lk) * 64 + (ty & kMask)];
-                    sep_pixel(e, (double)i - q.x1, (double)j - q.y1, (double)tile[m * 64 + lane], v);
+                __syncthreads();
+                for (int m = 0; m < mmax; ++m) {
+                    const int pp = r + kGroup * m;
+                    if (pp < n2) {
+                        const int j = (pp * inv) >> 16, i = pp - j * n;
+                        const int ty = n + j;
+                        const double e = gx[(i >> lk) * 64 + (i & kMask)] * gx[(ty >> lk) * 64 + (ty & kMask)];
+                        sep_pixel(e, (double)i - q.x1, (double)j - q.y1, (double)tile[m * 64 + lane], v);
+                    }
                 }
-            }
-            if constexpr (kGroup > 1) {
 #pragma unroll
                 for (int k = 0; k < kSepSums; ++k) v[k] = group_sum(v[k], kGroup);
             }
