@@ -633,22 +633,35 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
     }
 }
 
-template <int P, typename Acc, int W>
+/* Which group sizes a straggler launch runs: all, groups of 2..16 lanes, or one lane per
+ * fit.  One lane per fit (many long fits) is throughput work and runs best at two waves
+ * per SIMD; groups are latency work and run best with the full register file, so the
+ * symmetric and fixed-width classes launch one build of each (the one that does not
+ * apply exits at once).  Both derive the group size from the same lane count. */
+enum StragglerRole : int { kAllGroups = 0, kMultiLane = 1, kOneLane = 2 };
+
+template <int P, typename Acc, int W, int Role>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
-lm_straggler(KernelArgs args, int cls) {
+lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     extern __shared__ double lds_words[];
     const unsigned count = args.strag_ctr[2 * cls];
     unsigned *head = args.strag_ctr + 2 * cls + 1;
     const Parked<P> *park = strag_region<P>(args, cls);
-    const unsigned lanes_total = gridDim.x * 64u;
     int lk = kMaxGroup == 16 ? 4 : 3; /* log2 of the group size */
     while (lk > 0 && (unsigned long long)count << lk > 2ull * lanes_total) --lk; /* <= 2 fits per group */
-    switch (__builtin_amdgcn_readfirstlane(lk)) {
+    lk = __builtin_amdgcn_readfirstlane(lk);
+    if constexpr (Role == kOneLane) {
+        if (lk == 0) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
+        return;
+    }
+    switch (lk) {
     case 4: straggler_groups<P, Acc, 4>(args, count, head, park, lds_words); break;
     case 3: straggler_groups<P, Acc, 3>(args, count, head, park, lds_words); break;
     case 2: straggler_groups<P, Acc, 2>(args, count, head, park, lds_words); break;
     case 1: straggler_groups<P, Acc, 1>(args, count, head, park, lds_words); break;
-    default: straggler_groups<P, Acc, 0>(args, count, head, park, lds_words); break;
+    default:
+        if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
+        break;
     }
 }
 
@@ -760,10 +773,20 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
     if (a.strag_trials > 0) {
         const int nmax = 2 * ctx->cfg.max_half_width + 1;
         const size_t slds = lds + (size_t)nmax * 64 * sizeof(double); /* 2 nmax factor rows */
+        constexpr int kRole = P == 7 ? kAllGroups : kMultiLane;
         int per_cu = 0;
-        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1>, 64, slds));
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1, kRole>, 64, slds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1>), dim3(per_cu * ctx->num_cus), dim3(64), slds, stream, a, cls);
+        const unsigned lanes = (unsigned)(per_cu * ctx->num_cus) * 64u; /* sets the group size in both launches */
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole>), dim3(per_cu * ctx->num_cus), dim3(64), slds, stream, a,
+                           cls, lanes);
+        if constexpr (P != 7) {
+            int per_cu2 = 0;
+            LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, lm_straggler<P, Acc, 2, kOneLane>, 64, slds));
+            if (per_cu2 < 1) throw std::string("straggler kernel cannot be resident with this window size");
+            hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane>), dim3(per_cu2 * ctx->num_cus), dim3(64), slds,
+                               stream, a, cls, lanes);
+        }
     }
 }
 
