@@ -12,9 +12,11 @@
  *     long LM runs do not stall the others;
  *   - a refilled lane's window is gathered cooperatively by the whole wave (rows of
  *     n contiguous uint16 -> coalesced global loads) into its LDS column;
- *   - each loop iteration is one pixel pass for all active lanes: residuals at the
- *     current and trial points, the residual change, the Jacobian at the trial point
- *     and its normal equations, all in registers (lzr_lane.h);
+ *   - each loop iteration is one pass over the windows of all active lanes at their
+ *     trial points: for the symmetric and fixed-width models FP64 1-D moments of the
+ *     separable Gaussian plus five data sums (the normal equations follow in closed
+ *     form), for the ellipsoidal model residuals, residual change, Jacobian and normal
+ *     equations per pixel (lzr_lane.h);
  *   - the per-lane LM logic (lmpar, ratio test, status) runs in FP64 between passes.
  * Candidates are split by parameter count (5: symmetric, 7: ellipsoidal, 4: fixed width)
  * on the device and each class runs its own kernel instantiation.
@@ -100,16 +102,26 @@ __device__ __forceinline__ int live_count(const int *d_count, int ncands) {
     return c < 0 ? 0 : (c < ncands ? c : ncands);
 }
 
-__global__ void histogram_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, const int *d_count,
-                                 int default_model, unsigned *counters) {
+/* The sort kernels take kSortItems candidates per thread, so a block flushes its LDS
+ * histogram to the global counters once per kSortThreads * kSortItems candidates (the
+ * global atomics on the few hundred counters were the cost). */
+constexpr int kSortThreads = 256, kSortItems = 8, kSortBlock = kSortThreads * kSortItems;
+
+__global__ void __launch_bounds__(kSortThreads) histogram_kernel(const LzrCandidate *cands, const LzrStart *starts,
+                                                                 int ncands, const int *d_count, int default_model,
+                                                                 unsigned *counters) {
     ncands = live_count(d_count, ncands);
     __shared__ unsigned h[kKeys];
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncands) {
-        const int key = sort_key(cands[i], starts ? starts + i : nullptr, default_model);
-        if (key >= 0) atomicAdd(&h[key], 1u);
+    const int first = blockIdx.x * kSortBlock + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const int i = first + k * kSortThreads;
+        if (i < ncands) {
+            const int key = sort_key(cands[i], starts ? starts + i : nullptr, default_model);
+            if (key >= 0) atomicAdd(&h[key], 1u);
+        }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
@@ -136,24 +148,33 @@ __global__ void scan_kernel(unsigned *counters) {
     }
 }
 
-__global__ void scatter_kernel(const LzrCandidate *cands, const LzrStart *starts, int ncands, const int *d_count,
-                               int default_model, unsigned *counters, int *order) {
+__global__ void __launch_bounds__(kSortThreads) scatter_kernel(const LzrCandidate *cands, const LzrStart *starts,
+                                                               int ncands, const int *d_count, int default_model,
+                                                               unsigned *counters, int *order) {
     ncands = live_count(d_count, ncands);
     __shared__ unsigned h[kKeys], base[kKeys];
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x) h[t] = 0u;
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int key = -1;
-    unsigned local = 0;
-    if (i < ncands) {
-        key = sort_key(cands[i], starts ? starts + i : nullptr, default_model);
-        if (key >= 0) local = atomicAdd(&h[key], 1u);
+    const int first = blockIdx.x * kSortBlock + threadIdx.x;
+    int key[kSortItems];
+    unsigned local[kSortItems];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        const int i = first + k * kSortThreads;
+        key[k] = -1;
+        local[k] = 0u;
+        if (i < ncands) {
+            key[k] = sort_key(cands[i], starts ? starts + i : nullptr, default_model);
+            if (key[k] >= 0) local[k] = atomicAdd(&h[key[k]], 1u);
+        }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < kKeys; t += blockDim.x)
         if (h[t]) base[t] = atomicAdd(&counters[kHist + kKeys + t], h[t]);
     __syncthreads();
-    if (key >= 0) order[base[key] + local] = i;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k)
+        if (key[k] >= 0) order[base[key[k]] + local[k]] = first + k * kSortThreads;
 }
 
 /* Turns the raw end states into the reference records (post-checks, covariance,
@@ -410,17 +431,17 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
 /* ---- straggler kernel ------------------------------------------------------------
  * Fits handed off after strag_trials trials are the long tail of a batch: one lane per
  * fit would leave the end of the launch to a few lanes running ~200 more passes each.
- * Here a fit is held by a group of k lanes: lane r of the group sums pixels p = r + k m
- * of the window, a butterfly over the group adds the partial sums (every lane of the
+ * Here a fit is held by a group of k lanes: lane r of the group takes the window's
+ * pixels p = r + k m (and, for the separable models, the column and row factors
+ * t = r + k m), a butterfly over the group adds the partial sums (every lane of the
  * group ends with the same bits), and all lanes of the group run the same LM logic on
  * the same inputs.  k is the largest power of two <= kMaxGroup for which the handed-off
  * fits need at most two groups' turns on the launch's lanes: a function of the hand-off
- * count, which the
- * fits alone determine, so a fit's result does not depend on which wave or group runs
- * it or on timing.  When many fits are long (dim data), k falls to 1: one lane per fit,
- * the same pixel order and arithmetic as the first kernel.  (500k 9x9 symmetric fits:
- * 7.90 ms without hand-off, 6.72 ms with 8-lane groups after 32 trials, 6.48 ms with
- * 16-lane groups after 48.) */
+ * count, which the fits alone determine, so a fit's result does not depend on which
+ * wave or group runs it or on timing.  When many fits are long (dim data), k falls to
+ * 1: one lane per fit, with the first kernel's pass.  (500k 9x9 symmetric fits, per-pixel
+ * pass: 7.90 ms without hand-off, 6.72 ms with 8-lane groups after 32 trials, 6.48 ms
+ * with 16-lane groups after 48.) */
 constexpr int kMaxGroup = LZR_STRAGGLER_GROUP;
 static_assert(kMaxGroup == 8 || kMaxGroup == 16, "group sums are written for up to 16 lanes");
 
@@ -839,12 +860,11 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     /* the context's queue, counters and hand-off buffers are reused by every call */
     LZR_CHECK(ctx->order.begin(stream));
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
-    const int tpb = 256;
-    const int nb = (ncands + tpb - 1) / tpb;
-    hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, d_count,
+    const int nb = (ncands + kSortBlock - 1) / kSortBlock;
+    hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(kSortThreads), 0, stream, cands, starts, ncands, d_count,
                        ctx->cfg.model, ctx->d_counters);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kKeys), 0, stream, ctx->d_counters);
-    hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(tpb), 0, stream, cands, starts, ncands, d_count,
+    hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(kSortThreads), 0, stream, cands, starts, ncands, d_count,
                        ctx->cfg.model, ctx->d_counters, ctx->d_order);
     KernelArgs a;
     a.frames = frames;
@@ -878,8 +898,8 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
         LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
         LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
     }
-    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + tpb - 1) / tpb), dim3(tpb), 0, stream, cands, ncands, d_count,
-                       out, ctx->cfg);
+    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
+                       ctx->cfg);
     LZR_CHECK(hipGetLastError());
     LZR_CHECK(ctx->order.end(stream));
 }
