@@ -112,6 +112,34 @@ def test_gpu_fixed_width_matches_oracle(plain_gpu_sample):
     assert set(np.unique(got["status"])) <= {0, 2, 3, 9}
 
 
+@pytest.fixture(scope="module")
+def plain_100k():
+    w = synth.PLAIN_GPU_256.scaled(2000)
+    frames, _, cands = synth.generate(w, nthreads=16)
+    ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.35, default_half_width=4)
+    return frames, cands, ref
+
+
+@pytest.mark.parametrize("trials", [4, 11, 17, 24, 48])
+def test_gpu_straggler_group_sizes(plain_100k, trials):
+    """Fits still running after `trials` trials go to the straggler kernel, whose group size
+    follows their count (at most two turns of the launch's 65536 lanes on a 256-CU MI355X):
+    after 4 / 11 / 17 / 24 / 48 trials about 99k / 46k / 25k / 13k / 1.7k of the 100k fits
+    are handed off, which selects groups of 1 (the two-wave build), 2, 4, 8 and 16 lanes.
+    Every group size must match the oracle, and repeat calls must be bit-identical."""
+    frames, cands, ref = plain_100k
+    _say(f"case: symmetric 9x9, 100k fits, hand-off after {trials} trials")
+    with native.Context(abi.MODEL_SYMMETRIC, 1.35, default_half_width=4) as ctx:
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, trials)
+        got = ctx.fit(frames, cands)
+        again = ctx.fit(frames, cands)
+    handed = int(np.sum(ref["trials"] > trials))
+    st = compare(ref, got, abi.MODEL_SYMMETRIC)
+    _say(format_stats(f"gpu vs oracle, ~{handed} fits handed off", st))
+    _check_sym(st)
+    assert np.array_equal(got.view(np.uint8), again.view(np.uint8))
+
+
 def test_gpu_mixed_models_and_windows():
     _say("case: mixed models (symmetric, ellipsoidal, fixed width) and windows h=2..7")
     w = synth.MIXED_2M.scaled(40)
