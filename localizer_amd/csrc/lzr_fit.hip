@@ -14,9 +14,9 @@
  *     n contiguous uint16 -> coalesced global loads) into its LDS column;
  *   - each loop iteration is one pass over the windows of all active lanes at their
  *     trial points: for the symmetric and fixed-width models FP64 1-D moments of the
- *     separable Gaussian plus five data sums (the normal equations follow in closed
- *     form), for the ellipsoidal model residuals, residual change, Jacobian and normal
- *     equations per pixel (lzr_lane.h);
+ *     separable Gaussian plus five data sums, for the ellipsoidal model FP64 moments up
+ *     to degree 4 accumulated row by row; the normal equations follow in closed form
+ *     (lzr_lane.h);
  *   - the per-lane LM logic (lmpar, ratio test, status) runs in FP64 between passes.
  * Candidates are split by parameter count (5: symmetric, 7: ellipsoidal, 4: fixed width)
  * on the device and each class runs its own kernel instantiation.
@@ -385,12 +385,17 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
 #pragma unroll
         for (int k = 0; k < P; ++k) z[k] = 1.0;
         if constexpr (P == 7) {
-            PassIn<P> q;
-            if (active)
-                lane_pass_in<P>(s, args.cfg, q);
-            else
-                make_pass_in(z, z, 1.0, 1.0, q);
-            window_pass(q, n, nw, mixed, pix, xb, acc);
+            EllIn q;
+            if (active) {
+                lane_ell_in(s, args.cfg, q);
+            } else {
+                /* rho = 0: the factored path (an unfactored lane would slow the wave) */
+                const double t0[7] = {1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 1.0}, d0[7] = {};
+                make_ell_in(t0, d0, 1.0, q);
+            }
+            double v7[kEllSums];
+            ell_pass(q, n, nw, pix, xb, v7);
+            ell_finish<Acc>(q, n * n, v7, s.phase == kInit, s.fnorm * s.fnorm, acc);
         } else {
             SepIn q;
             if (active)
@@ -439,8 +444,8 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
  * fits need at most two groups' turns on the launch's lanes: a function of the hand-off
  * count, which the fits alone determine, so a fit's result does not depend on which
  * wave or group runs it or on timing.  When many fits are long (dim data), k falls to
- * 1: one lane per fit, with the first kernel's pass.  (500k 9x9 symmetric fits, per-pixel
- * pass: 7.90 ms without hand-off, 6.72 ms with 8-lane groups after 32 trials, 6.48 ms
+ * 1: one lane per fit, with the first kernel's pass.  (500k 9x9 symmetric fits, with the
+ * earlier per-pixel pass: 7.90 ms without hand-off, 6.72 ms with 8-lane groups after 32 trials, 6.48 ms
  * with 16-lane groups after 48.) */
 constexpr int kMaxGroup = LZR_STRAGGLER_GROUP;
 static_assert(kMaxGroup == 8 || kMaxGroup == 16, "group sums are written for up to 16 lanes");
@@ -477,21 +482,6 @@ __device__ __forceinline__ double group_sum(double v, int k) {
     if (k >= 8) v += step(v, 2);
     if (k >= 16) v += step(v, 3);
     return v;
-}
-
-template <int P, typename Acc>
-__device__ __forceinline__ void group_reduce(PassAcc<P, Acc> &a, int k) {
-    if (k == 1) return;
-#pragma unroll
-    for (int i = 0; i < NP<P>::SQ; ++i) a.hq[i] = group_sum(a.hq[i], k);
-#pragma unroll
-    for (int i = 0; i < P - 1; ++i) {
-        a.sj[i] = group_sum(a.sj[i], k);
-        a.gq[i] = group_sum(a.gq[i], k);
-    }
-    a.sf = group_sum(a.sf, k);
-    a.sse0 = group_sum(a.sse0, k);
-    a.dsse = group_sum(a.dsse, k);
 }
 
 /* The straggler loop for groups of 2^LK lanes. */
@@ -572,20 +562,68 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
 #pragma unroll
         for (int k = 0; k < P; ++k) z[k] = 1.0;
         if constexpr (P == 7) {
-            PassIn<P> q;
-            if (active)
-                lane_pass_in<P>(s, args.cfg, q);
-            else
-                make_pass_in(z, z, 1.0, 1.0, q);
-            __syncthreads();
-            for (int m = 0; m < mmax; ++m) {
-                const int pp = r + kGroup * m;
-                if (pp < n2) {
-                    const int j = (pp * inv) >> 16, i = pp - j * n;
-                    pixel_ell(q, (float)i, (float)j, (float)tile[m * 64 + lane], acc);
-                }
+            EllIn q;
+            if (active) {
+                lane_ell_in(s, args.cfg, q);
+            } else {
+                const double t0[7] = {1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 1.0}, d0[7] = {};
+                make_ell_in(t0, d0, 1.0, q);
             }
-            group_reduce<P, Acc>(acc, kGroup);
+            double v7[kEllSums];
+            if constexpr (kGroup == 1) {
+                const int nw = __builtin_amdgcn_readfirstlane(wave_max(n));
+                struct {
+                    const uint16_t *t;
+                    int n;
+                    __device__ uint16_t raw(int i, int j) const { return t[(i + j * n) * 64]; }
+                } pix{tile + lane, n};
+                struct {
+                    double *c;
+                    __device__ void set(int i, double x) { c[i * 64] = x; }
+                    __device__ double get(int i) const { return c[i * 64]; }
+                } xb{xcol + lane};
+                ell_pass(q, n, nw, pix, xb, v7);
+            } else {
+                /* moment pass split over the group: lane r computes the column factors of
+                 * columns r, r + k, ... and the rows r, r + k, ... of the window; the
+                 * butterfly adds the moments */
+#pragma unroll
+                for (int k = 0; k < kEllSums; ++k) v7[k] = 0.0;
+                for (int t = r; t < n; t += kGroup) {
+                    const double u = (double)t - q.x1;
+                    xcol[(t >> lk) * 64 + lane] = dexp(-(q.qa * u) * u);
+                }
+                __syncthreads();
+                const bool fac = ell_factored(q, n);
+                for (int j = r; j < n; j += kGroup) {
+                    const double vv = (double)j - q.y1;
+                    double w = 1.0, g = 1.0, c = 1.0;
+                    if (fac) {
+                        w = dexp(-(q.qb * vv) * vv);
+                        g = dexp(-2.0 * q.qc * vv);
+                        c = dexp(2.0 * q.qc * q.x1 * vv);
+                    }
+                    double Pm[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Rm[3] = {0.0, 0.0, 0.0}, Wm[3] = {0.0, 0.0, 0.0};
+                    double rd = 0.0, rdd = 0.0;
+                    for (int i = 0; i < n; ++i) {
+                        const int p = i + n * j;
+                        const double D = (double)tile[(p >> lk) * 64 + grp * kGroup + (p & kMask)];
+                        const double u = (double)i - q.x1;
+                        double e;
+                        if (fac) {
+                            e = gx[(i >> lk) * 64 + (i & kMask)] * c;
+                            c = c * g;
+                        } else {
+                            e = dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * vv + (q.qb * vv) * vv));
+                        }
+                        ell_pixel(e, u, D, Pm, Rm, Wm, rd, rdd);
+                    }
+                    ell_row(w, vv, Pm, Rm, Wm, rd, rdd, v7);
+                }
+#pragma unroll
+                for (int k = 0; k < kEllSums; ++k) v7[k] = group_sum(v7[k], kGroup);
+            }
+            ell_finish<Acc>(q, n2, v7, s.phase == kInit, s.fnorm * s.fnorm, acc);
         } else {
             /* separable pass split over the group: lane r computes the factors of slots
              * t = r, r + k, ... (t < n: column t, else row t - n) with their 1-D moments and

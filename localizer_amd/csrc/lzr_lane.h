@@ -21,12 +21,10 @@
  * at the trial point is a set of FP64 1-D moments over the window's columns and rows
  * plus five FP64 data sums over the pixels, from which J^T J, J^T f and |f|^2 follow
  * in closed form (sep_pass / sep_finish; about a dozen FP64 operations per pixel).
- * Ellipsoidal model: residuals, Jacobian and normal equations per pixel in FP32 with
- * FP64 accumulation (template Acc).  Near convergence the actual reduction
- * 1 - |f1|^2/|f0|^2 is ~1e-7, below FP32 resolution of |f|^2, so that pass evaluates
- * the residual change f1 - f0 = M(theta1) - M(theta0) directly: the change of the
- * Gaussian exponent is formed from the parameter step (no cancellation) and the
- * exponential change as e0 * expm1(-d_arg); the SSE change is sum (f1 - f0)(f1 + f0).
+ * Ellipsoidal model: its Jacobian columns are the exponential times polynomials of
+ * degree <= 2 in the pixel offsets, so the pass accumulates FP64 moments of e1 and e1^2
+ * up to degree 4 row by row (the cross factor exp(-2 qc u v) by recurrence along a row)
+ * and forms the normal equations from them (ell_pass / ell_finish).
  *
  * The file compiles as HIP device code (kernel) and as plain C++ (the CPU
  * emulation used by the tests to check the logic against the FP64 oracle).
@@ -115,18 +113,6 @@ struct LaneState {
     int trials, accepted;
 };
 
-/* FP32 constants of one pixel pass: current point theta0, trial point theta1. */
-template <int P>
-struct PassIn;
-
-template <>
-struct PassIn<7> {
-    float A0, x0, y0, b0, qa0, qc0, qb0; /* exponent a ux^2 + 2c ux uy + b uy^2 */
-    float dA, dxc, dyc, db, dqa, dqc, dqb;
-    float A1, k1, k2, k3, k4, k5, k6, k7, k8, k9, two_rho, inv_sigma;
-    double A0d, b0d, qa0d, qc0d, qb0d, inv_sigma_d;
-};
-
 /* Accumulators of one pass.  The offset column of J is the constant 1/sigma, so its
  * normal-equation entries come from plain sums. */
 template <int P, typename Acc>
@@ -159,14 +145,6 @@ struct PassAcc {
 #define LZR_NO_CONTRACT
 #endif
 
-LZR_HD LZR_INLINE float ffma(float a, float b, float c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_fmaf(a, b, c);
-#else
-    return std::fma(a, b, c);
-#endif
-}
-
 LZR_HD LZR_INLINE double dfma(double a, double b, double c) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_fma(a, b, c);
@@ -174,9 +152,6 @@ LZR_HD LZR_INLINE double dfma(double a, double b, double c) {
     return std::fma(a, b, c);
 #endif
 }
-
-LZR_HD LZR_INLINE float acc_fma(float a, float b, float c) { return ffma(a, b, c); }
-LZR_HD LZR_INLINE double acc_fma(double a, double b, double c) { return dfma(a, b, c); }
 
 LZR_HD LZR_INLINE float fexp(float a) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -192,107 +167,6 @@ LZR_HD LZR_INLINE double dexp(double a) {
 #else
     return std::exp(a);
 #endif
-}
-
-/* Exponential change e1 - e0 = e0 (exp(-darg) - 1) and e1 = exp(-arg1).  For a small
- * exponent change the difference comes from the Taylor series of expm1 (full relative
- * accuracy); for a large one e1 is evaluated directly, where the subtraction e1 - e0
- * loses at most ~1e-6 relative and e0 may have underflowed. */
-LZR_HD LZR_INLINE void exp_change(float e0, float arg1, float darg, float &e1, float &e0m) {
-    LZR_NO_CONTRACT
-    const float a = -darg;
-    float p = ffma(a, 1.0f / 120.0f, 1.0f / 24.0f);
-    p = ffma(a, p, 1.0f / 6.0f);
-    p = ffma(a, p, 0.5f);
-    p = ffma(a, p, 1.0f);
-    float em = a * p;
-#if defined(__HIP_DEVICE_COMPILE__)
-    /* both branches are a handful of FMAs: keep them unconditional (two v_cndmask) rather
-     * than a divergent branch per pixel */
-    asm volatile("" : "+v"(em));
-#endif
-    const float e1d = fexp(-arg1);
-    const bool small = darg > -0.0625f && darg < 0.0625f;
-    e0m = small ? e0 * em : e1d - e0;
-    e1 = small ? e0 + e0m : e1d;
-}
-
-/* ---- per-pixel work -------------------------------------------------------- */
-
-/* Residual at the current point, f0 = (A e0 + b - D)/sigma.  With F64 the exponent,
- * exponential and sum run in FP64 (ux0, uy0 are exact in FP32, so the quadratic form is
- * exact in FP64); the near-converged gradient J^T f is a small sum of large terms and
- * inherits the residual's absolute error. */
-template <bool F64>
-LZR_HD LZR_INLINE void residual0(double qd_arg0, float arg0_f, double A0d, double b0d, double isd, float A0,
-                                 float b0, float is, float D, float &e0, float &f0) {
-    LZR_NO_CONTRACT
-    if constexpr (F64) {
-        const double e = dexp(-qd_arg0);
-        e0 = (float)e;
-        f0 = (float)(dfma(A0d, e, b0d - (double)D) * isd);
-    } else {
-        e0 = fexp(-arg0_f);
-        f0 = (ffma(A0, e0, b0) - D) * is;
-    }
-}
-
-template <typename Acc>
-LZR_HD LZR_INLINE void pixel_ell(const PassIn<7> &q, float fi, float fj, float D, PassAcc<7, Acc> &a) {
-    LZR_NO_CONTRACT
-    const float ux0 = fi - q.x0, uy0 = fj - q.y0;
-    const float ux1 = ux0 - q.dxc, uy1 = uy0 - q.dyc;
-    const float xx0 = ux0 * ux0, yy0 = uy0 * uy0, xy0 = ux0 * uy0;
-    const float xx1 = ux1 * ux1, yy1 = uy1 * uy1, xy1 = ux1 * uy1;
-    const float arg0 = ffma(q.qa0, xx0, ffma(2.0f * q.qc0, xy0, q.qb0 * yy0));
-    float e0, f0;
-    {
-        const double dx = ux0, dy = uy0;
-        const double qd = dfma(q.qa0d * dx, dx, dfma(2.0 * q.qc0d * dx, dy, (q.qb0d * dy) * dy));
-        residual0<true>(qd, arg0, q.A0d, q.b0d, q.inv_sigma_d, q.A0, q.b0, q.inv_sigma, D, e0, f0);
-    }
-    const float dxx = -q.dxc * (ux1 + ux0);
-    const float dyy = -q.dyc * (uy1 + uy0);
-    const float dxy = -ffma(ux1, q.dyc, uy0 * q.dxc);
-    const float dqc = ffma(q.dqc, xy1, q.qc0 * dxy);
-    const float darg = ffma(q.dqa, xx1, ffma(q.qa0, dxx, ffma(2.0f, dqc, ffma(q.dqb, yy1, q.qb0 * dyy))));
-    float e1, e0m;
-    exp_change(e0, arg0 + darg, darg, e1, e0m);
-    const float df = ffma(q.dA, e1, ffma(q.A0, e0m, q.db)) * q.inv_sigma;
-    const float f1 = f0 + df;
-    a.sse0 = acc_fma(Acc(f0), Acc(f0), a.sse0);
-    a.dsse = acc_fma(Acc(df), Acc(f0 + f1), a.dsse);
-    const float ae = q.A1 * e1;
-    const float jA = e1 * q.inv_sigma;
-    const float jsx = ae * ffma(q.k1, xx1, -(q.k2 * xy1));
-    const float jsy = ae * ffma(q.k3, yy1, -(q.k4 * xy1));
-    const float jx = ae * ffma(q.k5, ux1, -(q.k6 * uy1));
-    const float jy = ae * ffma(q.k7, uy1, -(q.k6 * ux1));
-    const float jr = ae * ffma(q.k8, xy1, -(q.k9 * ffma(-q.two_rho, xy1, xx1 + yy1)));
-    const Acc J[6] = {Acc(jA), Acc(jsx), Acc(jsy), Acc(jx), Acc(jy), Acc(jr)};
-    const Acc F = Acc(f1);
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-#pragma unroll
-        for (int j = 0; j <= i; ++j) a.hq[tri(i, j)] = acc_fma(J[i], J[j], a.hq[tri(i, j)]);
-        a.sj[i] += J[i];
-        a.gq[i] = acc_fma(J[i], F, a.gq[i]);
-    }
-    a.sf += F;
-}
-
-/* One pass of the ellipsoidal model over a lane's window: residuals at the current point,
- * residual change and Jacobian at the trial point, normal equations.  nloop >= n is the
- * loop bound shared by the wave; `guard` masks pixels beyond the lane's own n (mixed
- * window sizes); pix(i, j) returns pixel (i, j) as float. */
-template <typename Acc, typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void window_pass(const PassIn<7> &q, int n, int nloop, bool guard, const Pix &pix, XBuf &,
-                                   PassAcc<7, Acc> &a) {
-    for (int j = 0; j < nloop; ++j) {
-        const float fj = (float)j;
-        for (int i = 0; i < nloop; ++i)
-            if (!guard || (i < n && j < n)) pixel_ell(q, (float)i, fj, pix(i, j), a);
-    }
 }
 
 /* ---- FP64 helpers ---------------------------------------------------------- */
@@ -520,52 +394,6 @@ LZR_HD LZR_INLINE void quad_coeffs(double sx, double sy, double rho, double &qa,
     qa = pre * isx * isx;
     qb = pre * isy * isy;
     qc = -pre * rho * isx * isy;
-}
-
-LZR_HD LZR_INLINE void make_pass_in(const double (&t0)[7], const double (&dx)[7], double sigma, double,
-                                    PassIn<7> &q) {
-    double qa0, qc0, qb0, qa1, qc1, qb1;
-    quad_coeffs(t0[1], t0[2], t0[5], qa0, qc0, qb0);
-    const double sx = t0[1] + dx[1], sy = t0[2] + dx[2], rho = t0[5] + dx[5];
-    quad_coeffs(sx, sy, rho, qa1, qc1, qb1);
-    q.A0 = (float)t0[0];
-    q.x0 = (float)t0[3];
-    q.y0 = (float)t0[4];
-    q.b0 = (float)t0[6];
-    q.qa0 = (float)qa0;
-    q.qc0 = (float)qc0;
-    q.qb0 = (float)qb0;
-    q.dA = (float)dx[0];
-    q.dxc = (float)dx[3];
-    q.dyc = (float)dx[4];
-    q.db = (float)dx[6];
-    q.dqa = (float)(qa1 - qa0);
-    q.dqc = (float)(qc1 - qc0);
-    q.dqb = (float)(qb1 - qb0);
-    q.A1 = (float)(t0[0] + dx[0]);
-    const double is = drcp(sigma);
-    q.inv_sigma = (float)is;
-    q.A0d = t0[0];
-    q.b0d = t0[6];
-    q.qa0d = qa0;
-    q.qc0d = qc0;
-    q.qb0d = qb0;
-    q.inv_sigma_d = is;
-    /* reference columns (source/Localization.cpp:1275-1280) with den = 2(1-rho^2) sigma;
-     * k = 2/(den ...) = 1/(omr sigma ...) */
-    const double iomr = drcp(1.0 - rho * rho);
-    const double isx = drcp(sx), isy = drcp(sy);
-    const double t = iomr * is; /* 2 / den */
-    q.k1 = (float)(t * isx * isx * isx);
-    q.k2 = (float)(t * rho * isx * isx * isy);
-    q.k3 = (float)(t * isy * isy * isy);
-    q.k4 = (float)(t * rho * isx * isy * isy);
-    q.k5 = (float)(t * isx * isx);
-    q.k6 = (float)(t * rho * isx * isy);
-    q.k7 = (float)(t * isy * isy);
-    q.k8 = (float)(t * isx * isy);
-    q.k9 = (float)(t * iomr * rho * isx * isy);
-    q.two_rho = (float)(2.0 * rho);
 }
 
 /* ---- separable-moment pass (symmetric and fixed-width models) -----------------------
@@ -801,6 +629,233 @@ LZR_HD LZR_INLINE void sep_finish(const SepIn &q, int npix, const double (&v)[kS
     a.dsse = Acc(init ? 0.0 : sse1 - fn2);
 }
 
+/* ---- moment pass of the ellipsoidal model -------------------------------------------
+ * e1 = exp(-(qa u^2 + 2 qc u v + qb v^2)) (u = i - x1, v = j - y1) and every column of the
+ * reference Jacobian (source/Localization.cpp:1275-1280) is e1 times a polynomial of degree
+ * <= 2 in (u, v) over the monomials 1, u, v, u^2, uv, v^2.  J^T J, J^T 1 and J^T f1 are
+ * therefore fixed combinations of the moments
+ *   S_ab = sum e1^2 u^a v^b (a + b <= 4), T_ab = sum e1 u^a v^b, U_ab = sum e1 D u^a v^b
+ * (a + b <= 2), plus SD = sum D and SDD = sum D^2.  With e1 = Ex(i) Ey(j) C(i, j),
+ * Ex = exp(-qa u^2), Ey = exp(-qb v^2), the cross factor C = exp(-2 qc u v) follows the
+ * recurrence C(i+1, j) = C(i, j) exp(-2 qc v) along a row, so a row needs 13 inner sums
+ * over its pixels and the rows are combined with the powers of v (ell_pass).  When a
+ * factor could overflow or underflow on its own (a collapsed width, |rho| near 1) the lane
+ * evaluates e1 per pixel instead and weights its rows by 1. */
+enum EllSum : int {
+    kES = 0,     /* S_ab, a + b <= 4: 15 slots, index ell_s(a, b) */
+    kET = 15,    /* T_ab, a + b <= 2: 6 slots, index ell_t(a, b) */
+    kEU = 21,    /* U_ab: 6 slots */
+    kESD = 27,
+    kESDD = 28,
+    kEllSums = 29
+};
+LZR_HD constexpr int ell_s(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
+LZR_HD constexpr int ell_t(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
+
+/* FP64 constants of an ellipsoidal moment pass at the trial point. */
+struct EllIn {
+    double x1, y1, qa, qb, qc; /* centre (window coordinates), exponent coefficients */
+    double A1, b1, is, rho;
+    double k1, k2, k3, k4, k5, k6, k7, k8, k9; /* reference column factors */
+};
+
+/* Trial point t0 + dx of (A, sx, sy, x0, y0, rho, b); columns with den = 2(1-rho^2) sigma,
+ * k = 2/(den ...) = 1/(omr sigma ...) as in source/Localization.cpp:1275-1280. */
+LZR_HD LZR_INLINE void make_ell_in(const double (&t0)[7], const double (&dx)[7], double is, EllIn &q) {
+    const double sx = t0[1] + dx[1], sy = t0[2] + dx[2], rho = t0[5] + dx[5];
+    quad_coeffs(sx, sy, rho, q.qa, q.qc, q.qb);
+    q.A1 = t0[0] + dx[0];
+    q.x1 = t0[3] + dx[3];
+    q.y1 = t0[4] + dx[4];
+    q.b1 = t0[6] + dx[6];
+    q.is = is;
+    q.rho = rho;
+    const double iomr = drcp(1.0 - rho * rho);
+    const double isx = drcp(sx), isy = drcp(sy);
+    const double t = iomr * is;
+    q.k1 = t * isx * isx * isx;
+    q.k2 = t * rho * isx * isx * isy;
+    q.k3 = t * isy * isy * isy;
+    q.k4 = t * rho * isx * isy * isy;
+    q.k5 = t * isx * isx;
+    q.k6 = t * rho * isx * isy;
+    q.k7 = t * isy * isy;
+    q.k8 = t * isx * isy;
+    q.k9 = t * iomr * rho * isx * isy;
+}
+
+/* True when the three factors Ex, Ey, C stay within exp(+-600) over an n x n window. */
+LZR_HD LZR_INLINE bool ell_factored(const EllIn &q, int n) {
+    const double um = dmax(dabs(q.x1), dabs((double)(n - 1) - q.x1));
+    const double vm = dmax(dabs(q.y1), dabs((double)(n - 1) - q.y1));
+    return q.qa >= 0.0 && q.qb >= 0.0 && q.qa * um * um <= 600.0 && q.qb * vm * vm <= 600.0 &&
+           2.0 * dabs(q.qc) * um * vm <= 600.0;
+}
+
+/* Row-end accumulation: inner sums P_a = sum e'^2 u^a (a <= 4), R_a = sum e' u^a,
+ * W_a = sum e' D u^a (a <= 2) of row v with row weight w (e1 = w e'). */
+LZR_HD LZR_INLINE void ell_row(double w, double v, const double (&P)[5], const double (&R)[3], const double (&W)[3],
+                               double rd, double rdd, double (&s)[kEllSums]) {
+    LZR_NO_CONTRACT
+    const double w2 = w * w;
+    double vb[5];
+    vb[0] = 1.0;
+#pragma unroll
+    for (int b = 1; b < 5; ++b) vb[b] = vb[b - 1] * v;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int b = 0; a + b < 5; ++b) s[kES + ell_s(a, b)] = dfma(w2 * vb[b], P[a], s[kES + ell_s(a, b)]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; a + b < 3; ++b) {
+            s[kET + ell_t(a, b)] = dfma(w * vb[b], R[a], s[kET + ell_t(a, b)]);
+            s[kEU + ell_t(a, b)] = dfma(w * vb[b], W[a], s[kEU + ell_t(a, b)]);
+        }
+    s[kESD] += rd;
+    s[kESDD] += rdd;
+}
+
+/* Inner sums of one pixel with factor e (e1 / row weight), offset u and data D. */
+LZR_HD LZR_INLINE void ell_pixel(double e, double u, double D, double (&P)[5], double (&R)[3], double (&W)[3],
+                                 double &rd, double &rdd) {
+    LZR_NO_CONTRACT
+    const double e2 = e * e, e2u = e2 * u, e2u2 = e2u * u, e2u3 = e2u2 * u;
+    P[0] += e2;
+    P[1] += e2u;
+    P[2] += e2u2;
+    P[3] += e2u3;
+    P[4] = dfma(e2u3, u, P[4]);
+    const double eu = e * u;
+    R[0] += e;
+    R[1] += eu;
+    R[2] = dfma(eu, u, R[2]);
+    const double ed = e * D, edu = ed * u;
+    W[0] += ed;
+    W[1] += edu;
+    W[2] = dfma(edu, u, W[2]);
+    rd += D;
+    rdd = dfma(D, D, rdd);
+}
+
+/* The ellipsoidal moment pass of one lane over its n x n window (nloop >= n: the wave's
+ * loop bound; pixels beyond n are masked).  pix.raw(i, j) returns pixel (i, j) as stored,
+ * xb holds one FP64 value per column. */
+template <typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, const Pix &pix, XBuf &xb,
+                                double (&s)[kEllSums]) {
+    LZR_NO_CONTRACT
+#pragma unroll
+    for (int k = 0; k < kEllSums; ++k) s[k] = 0.0;
+    const bool fac = ell_factored(q, n);
+    GaussRec gx, gy;
+    gx.start(q.qa, q.x1, n, dexp(-2.0 * q.qa));
+    gy.start(q.qb, q.y1, n, dexp(-2.0 * q.qb));
+    for (int i = 0; i < nloop; ++i) {
+        const double u = (double)i - q.x1;
+        double e = gx.next();
+        if (!gx.ok) e = dexp(-(q.qa * u) * u);
+        xb.set(i, i < n ? e : 0.0);
+    }
+    for (int j = 0; j < nloop; ++j) {
+        const double v = (double)j - q.y1;
+        double ey = gy.next();
+        if (!gy.ok) ey = dexp(-(q.qb * v) * v);
+        const bool row_in = j < n;
+        double w = 1.0, g = 1.0, c = 1.0;
+        if (fac) {
+            w = ey;
+            g = dexp(-2.0 * q.qc * v); /* C(i + 1, j) / C(i, j) */
+            c = dexp(2.0 * q.qc * q.x1 * v); /* C(0, j) */
+        }
+        double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0}, W[3] = {0.0, 0.0, 0.0};
+        double rd = 0.0, rdd = 0.0;
+        for (int i = 0; i < nloop; ++i) {
+            const bool in = row_in && i < n;
+            const double D = in ? (double)pix.raw(i, j) : 0.0;
+            const double u = (double)i - q.x1;
+            /* outside the lane's window the factors may overflow (the range check covers
+             * the window only): masked by a select, never by a product with 0 */
+            double e;
+            if (fac) {
+                const double ef = xb.get(i) * c;
+                c = c * g;
+                e = in ? ef : 0.0;
+            } else {
+                e = in ? dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * v + (q.qb * v) * v)) : 0.0;
+            }
+            ell_pixel(e, u, D, P, R, W, rd, rdd);
+        }
+        ell_row(row_in ? w : 0.0, v, P, R, W, rd, rdd, s);
+    }
+}
+
+/* Normal equations and residual sums from the moments (npix pixels); J columns
+ * (A, sx, sy, x0, y0, rho) = e1 times the polynomials of C below.  init: the pass is at
+ * the start point; otherwise fn2 = |f0|^2. */
+template <typename Acc>
+LZR_HD LZR_INLINE void ell_finish(const EllIn &q, int npix, const double (&s)[kEllSums], bool init, double fn2,
+                                  PassAcc<7, Acc> &a) {
+    LZR_NO_CONTRACT
+    constexpr int ma[6] = {0, 1, 0, 2, 1, 0}, mb[6] = {0, 0, 1, 0, 1, 2}; /* 1, u, v, uu, uv, vv */
+    const double is = q.is, A1 = q.A1, b1 = q.b1;
+    double C[6][6]; /* [monomial][column] */
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) C[m][k] = 0.0;
+    C[0][0] = is;
+    C[3][1] = A1 * q.k1;
+    C[4][1] = -(A1 * q.k2);
+    C[5][2] = A1 * q.k3;
+    C[4][2] = -(A1 * q.k4);
+    C[1][3] = A1 * q.k5;
+    C[2][3] = -(A1 * q.k6);
+    C[2][4] = A1 * q.k7;
+    C[1][4] = -(A1 * q.k6);
+    C[3][5] = -(A1 * q.k9);
+    C[4][5] = A1 * (q.k8 + 2.0 * q.rho * q.k9);
+    C[5][5] = -(A1 * q.k9);
+    /* MC = M2 C with M2[m][m'] = S(a_m + a_m', b_m + b_m') */
+    double MC[6][6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double t = 0.0;
+#pragma unroll
+            for (int mm = 0; mm < 6; ++mm) t = dfma(s[kES + ell_s(ma[m] + ma[mm], mb[m] + mb[mm])], C[mm][k], t);
+            MC[m][k] = t;
+        }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+        for (int l = 0; l <= k; ++l) {
+            double t = 0.0;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) t = dfma(C[m][k], MC[m][l], t);
+            a.hq[tri(k, l)] = Acc(t);
+        }
+        double tj = 0.0, tg = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const int it = ell_t(ma[m], mb[m]);
+            tj = dfma(C[m][k], s[kET + it], tj);
+            const double f = (A1 * s[kES + ell_s(ma[m], mb[m])] + b1 * s[kET + it]) - s[kEU + it];
+            tg = dfma(C[m][k], f, tg);
+        }
+        a.sj[k] = Acc(tj);
+        a.gq[k] = Acc(is * tg);
+    }
+    const double N = (double)npix, S00 = s[kES], T00 = s[kET], U00 = s[kEU], SD = s[kESD];
+    a.sf = Acc(is * ((A1 * T00 + N * b1) - SD));
+    const double sse1 =
+        is * is * ((A1 * ((A1 * S00 + 2.0 * (b1 * T00)) - 2.0 * U00) + b1 * (N * b1 - 2.0 * SD)) + s[kESDD]);
+    a.sse0 = Acc(init ? sse1 : 0.0);
+    a.dsse = Acc(init ? 0.0 : sse1 - fn2);
+}
+
 /* Column norms of the first `rows` Jacobian rows at t (GSL 1.16 update_diag reads the
  * first p rows of J).  Pixel k = i + n j, window coordinates; w is the fixed width. */
 template <int P>
@@ -998,14 +1053,6 @@ LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const 
 
 /* FP32 pass constants for the lane's next evaluation (trial point = th + dx for a
  * trial, th itself for the initial evaluation). */
-template <int P>
-LZR_HD LZR_INLINE void lane_pass_in(const LaneState<P> &s, const LaneConfig &cfg, PassIn<P> &q) {
-    double step[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
-    make_pass_in(s.th, step, cfg.sigma, cfg.initial_psf_width, q);
-}
-
 /* Constants of a separable pass at the lane's trial point (the start point at kInit). */
 template <int P>
 LZR_HD LZR_INLINE void lane_sep_in(const LaneState<P> &s, const LaneConfig &cfg, SepIn &q) {
@@ -1013,6 +1060,14 @@ LZR_HD LZR_INLINE void lane_sep_in(const LaneState<P> &s, const LaneConfig &cfg,
 #pragma unroll
     for (int i = 0; i < P; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
     make_sep_in(s.th, step, cfg.inv_sigma, cfg.initial_psf_width, q);
+}
+
+/* Constants of an ellipsoidal moment pass at the lane's trial point. */
+LZR_HD LZR_INLINE void lane_ell_in(const LaneState<7> &s, const LaneConfig &cfg, EllIn &q) {
+    double step[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
+    make_ell_in(s.th, step, cfg.inv_sigma, q);
 }
 
 /* One data sum term of pixel (u, w) with e = e1 and data D (the group-split pass). */

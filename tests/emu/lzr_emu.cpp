@@ -33,9 +33,11 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             double get(int i) const { return v[i]; }
         } xb;
         if constexpr (P == 7) {
-            PassIn<P> q;
-            lane_pass_in<P>(s, cfg, q);
-            window_pass(q, n, n, false, pix, xb, a);
+            EllIn q;
+            lane_ell_in(s, cfg, q);
+            double v[kEllSums];
+            ell_pass(q, n, n, pix, xb, v);
+            ell_finish<Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
         } else {
             SepIn q;
             lane_sep_in<P>(s, cfg, q);
@@ -121,6 +123,42 @@ static void sep_one(const double *theta, const double *dx, double sigma, double 
     for (int i = 0; i < P - 1; ++i) out[k++] = a.gq[i];
     out[k++] = a.sf;
     out[k++] = a.sse0;
+}
+
+// One ellipsoidal moment pass at theta + dx (A, sx, sy, x0, y0, rho, b) over an n x n
+// window, loop bound nloop >= n.  out = [hq (21, packed lower), sj (6), gq (6), sf, |f1|^2].
+extern "C" int emu_ell_pass(const double *theta, const double *dx, double sigma, int n, int nloop,
+                            const uint16_t *D, double *out) {
+    if (n < 1 || nloop < n || nloop > 64) return -1;
+    double t0[7], st[7];
+    for (int i = 0; i < 7; ++i) {
+        t0[i] = theta[i];
+        st[i] = dx[i];
+    }
+    EllIn q;
+    make_ell_in(t0, st, 1.0 / sigma, q);
+    struct {
+        const uint16_t *D;
+        int n;
+        uint16_t raw(int i, int j) const { return i < n && j < n ? D[i + n * j] : (uint16_t)0; }
+    } pix{D, n};
+    struct {
+        double v[64];
+        void set(int i, double x) { v[i] = x; }
+        double get(int i) const { return v[i]; }
+    } xb;
+    double v[kEllSums];
+    ell_pass(q, n, nloop, pix, xb, v);
+    PassAcc<7, double> a;
+    a.zero();
+    ell_finish<double>(q, n * n, v, true, 0.0, a);
+    int k = 0;
+    for (int i = 0; i < NP<7>::SQ; ++i) out[k++] = a.hq[i];
+    for (int i = 0; i < 6; ++i) out[k++] = a.sj[i];
+    for (int i = 0; i < 6; ++i) out[k++] = a.gq[i];
+    out[k++] = a.sf;
+    out[k++] = a.sse0;
+    return 0;
 }
 
 extern "C" int emu_sep_pass(int P, const double *theta, const double *dx, double sigma, double w, int n, int nloop,

@@ -22,6 +22,9 @@ def lib():
         L.emu_sep_pass.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.emu_sep_pass.restype = ctypes.c_int
+        L.emu_ell_pass.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+        L.emu_ell_pass.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -55,3 +58,17 @@ def sep_pass(theta, dx, sigma, w, window, nloop=None):
                             window.ctypes.data, out.ctypes.data)
     assert rc == 0
     return out[:sq], out[sq:sq + p - 1], out[sq + p - 1:sq + 2 * (p - 1)], out[-2], out[-1]
+
+
+def ell_pass(theta, dx, sigma, window, nloop=None):
+    """Normal-equation sums of one ellipsoidal moment pass (lzr_lane.h ell_pass +
+    ell_finish) at theta + dx over `window`; returns (hq, sj, gq, sf, sse)."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    dx = np.ascontiguousarray(dx, np.float64)
+    window = np.ascontiguousarray(window, np.uint16)
+    n = window.shape[0]
+    out = np.zeros(21 + 6 + 6 + 2)
+    rc = lib().emu_ell_pass(theta.ctypes.data, dx.ctypes.data, float(sigma), n, int(nloop or n), window.ctypes.data,
+                            out.ctypes.data)
+    assert rc == 0
+    return out[:21], out[21:27], out[27:33], out[-2], out[-1]

@@ -157,6 +157,25 @@ def test_gpu_mixed_models_and_windows():
         assert st["status_agreement"] >= 0.99
 
 
+def test_gpu_mixed_windows_bit_identical():
+    """Lanes with different window sizes share a wave (its loop runs to the largest window),
+    and which fits share a wave depends on timing: repeated calls must be bit-identical for
+    every model.  (A product 0 * inf outside a lane's window once made ~0.1 % of the
+    ellipsoidal fits depend on their wave neighbours.)"""
+    _say("case: mixed windows h=2..7, three models, 80k fits, three calls")
+    w = synth.MIXED_2M.scaled(1600)
+    frames, _, cands = synth.generate(w, nthreads=16)
+    cands["model"][::7] = abi.MODEL_FIXED_WIDTH
+    outs = []
+    for _ in range(3):
+        with native.Context(abi.MODEL_SYMMETRIC, 1.5) as ctx:
+            ctx.set_option(abi.OPT_MAX_HALF_WIDTH, 7)
+            outs.append(ctx.fit(frames, cands))
+    for o in outs[1:]:
+        diff = np.any(o.view(np.uint8).reshape(o.size, -1) != outs[0].view(np.uint8).reshape(o.size, -1), axis=1)
+        assert not diff.any(), (int(diff.sum()), np.unique(cands["model"][diff]))
+
+
 def test_gpu_edges_invalid_and_empty():
     _say("case: edge / invalid / empty")
     frames = np.full((2, 32, 40), 100, np.uint16)

@@ -116,3 +116,87 @@ def test_masked_window_inside_a_larger_loop():
         for a, b in zip(full, masked):
             np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-9)
         _check(theta, np.zeros(5), 1.0, 1.3, window, nloop)
+
+
+def _brute_ell(theta, dx, sigma, window):
+    """Per-pixel FP64 sums from the reference's ellipsoidal model and Jacobian columns
+    (source/Localization.cpp:1078-1126, :1227-1295; the rho column's quirk included)."""
+    A, sx, sy, x0, y0, rho, b = np.asarray(theta, np.float64) + np.asarray(dx, np.float64)
+    n = window.shape[0]
+    jj, ii = np.mgrid[0:n, 0:n].astype(np.float64)
+    D = window.astype(np.float64)
+    u, v = ii - x0, jj - y0
+    omr = 1.0 - rho * rho
+    Q = u * u / sx ** 2 + v * v / sy ** 2 - 2 * rho * u * v / (sx * sy)
+    e = np.exp(-Q / (2 * omr))
+    f = (A * e + b - D) / sigma
+    den = 2 * omr * sigma
+    cols = [e / sigma,
+            -A * (2 * rho * u * v / (sx * sx * sy) - 2 * u * u / sx ** 3) * e / den,
+            -A * (2 * rho * u * v / (sx * sy * sy) - 2 * v * v / sy ** 3) * e / den,
+            -A * (2 * rho * v / (sx * sy) - 2 * u / (sx * sx)) * e / den,
+            -A * (2 * rho * u / (sx * sy) - 2 * v / (sy * sy)) * e / den,
+            A * (u * v / (omr * sx * sy) - rho * (u * u + v * v - 2 * rho * u * v) / (sx * sy * omr * omr)) * e / sigma]
+    J = np.stack([c.ravel() for c in cols], axis=1)
+    H = J.T @ J
+    hq = np.array([H[i, j] for i in range(6) for j in range(i + 1)])
+    want = (hq, J.sum(axis=0), J.T @ f.ravel(), f.sum(), float(f.ravel() @ f.ravel()))
+    fa = ((np.abs(A * e) + abs(b) + D) / sigma).ravel()
+    aJ = np.abs(J)
+    mag = (np.array([(aJ[:, i] * aJ[:, j]).sum() for i in range(6) for j in range(i + 1)]), aJ.sum(axis=0),
+           aJ.T @ fa, fa.sum(), float(fa @ fa))
+    return want, mag
+
+
+def _check_ell(theta, dx, sigma, window, nloop=None, rtol=1e-11, mtol=1e-12):
+    got = pyemu.ell_pass(theta, dx, sigma, window, nloop)
+    want, mag = _brute_ell(theta, dx, sigma, window)
+    for name, g, r, m in zip(("J^T J", "J^T 1", "J^T f", "sum f", "|f|^2"), got, want, mag):
+        g, r, m = np.atleast_1d(g), np.atleast_1d(r), np.atleast_1d(m)
+        assert np.all(np.isfinite(g)), name
+        assert np.all(np.abs(g - r) <= rtol * np.abs(r) + mtol * m + 1e-300), (name, g, r, m)
+
+
+def _window_ell(rng, n, A, sx, sy, x0, y0, rho, b):
+    jj, ii = np.mgrid[0:n, 0:n]
+    u, v = ii - x0, jj - y0
+    Q = u * u / sx ** 2 + v * v / sy ** 2 - 2 * rho * u * v / (sx * sy)
+    return rng.poisson(b + A * np.exp(-Q / (2 * (1 - rho * rho)))).astype(np.uint16)
+
+
+@pytest.mark.parametrize("n", [5, 9, 15])
+def test_ellipsoidal_moment_pass_matches_pixel_sums(n):
+    """The ellipsoidal moment pass (Ex Ey C factorisation, the cross factor by recurrence
+    along rows) against the reference's columns summed pixel by pixel."""
+    rng = np.random.default_rng(20 + n)
+    c = (n - 1) / 2
+    for _ in range(25):
+        theta = [rng.uniform(100, 3000), rng.uniform(0.8, 2.2), rng.uniform(0.8, 2.2), c + rng.uniform(-1.5, 1.5),
+                 c + rng.uniform(-1.5, 1.5), rng.uniform(-0.6, 0.6), rng.uniform(50, 200)]
+        dx = rng.normal(0, [30, 0.05, 0.05, 0.2, 0.2, 0.05, 3])
+        window = _window_ell(rng, n, *theta)
+        _check_ell(theta, dx, 1.0, window)
+        _check_ell(theta, np.zeros(7), 1.0, window)
+
+
+def test_ellipsoidal_direct_fallback_and_masking():
+    """Trial points whose factors would overflow on their own (|rho| near 1, collapsed
+    widths) take per-pixel exponentials; a lane's window inside a larger loop bound
+    contributes nothing beyond n."""
+    rng = np.random.default_rng(31)
+    window = _window_ell(rng, 11, 900, 1.4, 1.1, 5.2, 4.6, 0.2, 100)
+    for theta in ([900, 1.4, 1.1, 5.2, 4.6, 0.2, 100],
+                  [900, 1.4, 1.1, 5.2, 4.6, 0.995, 100],   # 1/(1-rho^2) ~ 100: direct
+                  [900, 0.05, 1.1, 5.2, 4.6, 0.2, 100],    # collapsed sx: direct
+                  [900, 1.4, 1.1, -4.0, 4.6, -0.5, 100],   # centre outside the window
+                  [900, 30.0, 25.0, 5.0, 5.0, 0.0, 100]):  # nearly flat
+        _check_ell(theta, np.zeros(7), 1.0, window)
+        _check_ell(theta, np.zeros(7), 2.0, window, nloop=15)
+
+    # strong coupling (|qc| ~ 6.6): factored inside the 5 x 5 window, but the cross factor
+    # overflows in the loop's columns and rows beyond it; those must contribute exactly 0
+    small = _window_ell(rng, 5, 900, 0.6, 0.6, 2.0, 2.0, 0.9, 100)
+    theta = [900, 0.6, 0.6, 2.0, 2.0, 0.9, 100]
+    _check_ell(theta, np.zeros(7), 1.0, small, nloop=15, rtol=1e-10, mtol=1e-11)
+    for a, b in zip(pyemu.ell_pass(theta, np.zeros(7), 1.0, small, 5), pyemu.ell_pass(theta, np.zeros(7), 1.0, small, 15)):
+        assert np.array_equal(np.atleast_1d(a), np.atleast_1d(b))
