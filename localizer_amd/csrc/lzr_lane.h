@@ -758,6 +758,15 @@ LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, const Pix &pix
         if (!gx.ok) e = dexp(-(q.qa * u) * u);
         xb.set(i, i < n ? e : 0.0);
     }
+    /* row quantities by recurrence over rows: g_j = C(i + 1, j) / C(i, j) = exp(-2 qc v_j),
+     * c_j = C(0, j) = exp(2 qc x1 v_j); within the window both stay inside exp(+-600) */
+    double gj = 1.0, cj = 1.0, gr = 1.0, cr = 1.0;
+    if (fac) {
+        gj = dexp(2.0 * q.qc * q.y1);
+        cj = dexp(-2.0 * q.qc * q.x1 * q.y1);
+        gr = dexp(-2.0 * q.qc);
+        cr = dexp(2.0 * q.qc * q.x1);
+    }
     for (int j = 0; j < nloop; ++j) {
         const double v = (double)j - q.y1;
         double ey = gy.next();
@@ -766,8 +775,10 @@ LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, const Pix &pix
         double w = 1.0, g = 1.0, c = 1.0;
         if (fac) {
             w = ey;
-            g = dexp(-2.0 * q.qc * v); /* C(i + 1, j) / C(i, j) */
-            c = dexp(2.0 * q.qc * q.x1 * v); /* C(0, j) */
+            g = gj;
+            c = cj;
+            gj = gj * gr;
+            cj = cj * cr;
         }
         double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0}, W[3] = {0.0, 0.0, 0.0};
         double rd = 0.0, rdd = 0.0;
