@@ -323,7 +323,7 @@ def render_leg(d_fits, nfits, w, log, reps=5):
             "ms": round(ms, 3), "positions_per_s": n / ms * 1e3, "pixel_terms_per_s_approx": pairs / ms * 1e3}
 
 
-def dense_leg(nframes, block_frames, latency_frames, log):
+def dense_leg(nframes, block_frames, latency_frames, log, pipelines=DENSE_PIPELINES):
     """Dense end-to-end stream: frames -> GLRT segmentation -> 8-way particle finding ->
     11x11 patch gather -> CNN start -> LM with residual fallback, on the device; frames
     resident in HBM, one host read of the candidate count per block."""
@@ -382,7 +382,7 @@ def dense_leg(nframes, block_frames, latency_frames, log):
             self.finder.close()
             self.ctx.close()
 
-    pipes = [Pipe() for _ in range(DENSE_PIPELINES)]
+    pipes = [Pipe() for _ in range(pipelines)]
 
     def stream_all(netinit_fit):
         pipes[0].run(0, min(block_frames, nframes), netinit_fit)  # warm-up
@@ -415,7 +415,7 @@ def dense_leg(nframes, block_frames, latency_frames, log):
         "glrt_pfa": DENSE_PFA,
         "window": f"{2 * w.half_width[0] + 1}x{2 * w.half_width[0] + 1}",
         "block_frames": block_frames,
-        "pipelines": DENSE_PIPELINES,
+        "pipelines": pipelines,
         "candidates_per_frame": cands / nframes,
         "netinit": {"frames_per_s": nframes / sec_net, "localizations_per_s": acc_net / sec_net,
                     "accepted": acc_net, "seconds": sec_net},
@@ -455,6 +455,7 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the localization-bitmap leg")
     ap.add_argument("--dense-frames", type=int, default=1000)
     ap.add_argument("--dense-block", type=int, default=50, help="frames per block of the dense stream")
+    ap.add_argument("--dense-pipelines", type=int, default=DENSE_PIPELINES, help="frame blocks in flight")
     ap.add_argument("--net-train-rois", type=int, default=4 << 20)
     ap.add_argument("--net-eval-rois", type=int, default=4000000)
     ap.add_argument("--net-epochs", type=float, default=2.0)
@@ -542,7 +543,8 @@ def main():
         torch.cuda.empty_cache()
     dense = None
     if not args.no_dense:
-        dense = dense_leg(args.dense_frames, args.dense_block, 100, lambda m: print(m, file=sys.stderr, flush=True))
+        dense = dense_leg(args.dense_frames, args.dense_block, 100, lambda m: print(m, file=sys.stderr, flush=True),
+                          pipelines=args.dense_pipelines)
         torch.cuda.empty_cache()
     net = None
     if not args.no_net:

@@ -77,8 +77,8 @@ __device__ inline bool glrt_test(const GlrtParams &P, double S1, double S2, doub
  * in LDS as floats (exact for uint16).  Thread (column c, row group q) walks down its
  * column over 16 output rows: the row sums of each input row over the window's columns
  * (sum x, sum x^2, sum E(d) x) go into a ring of WS registers per sum (WS is a template
- * parameter, so the ring is unrolled), and each output pixel adds the ring: S1, S2 exactly
- * (integers below 2^53) and Sg = sum_d E(d) ring_g[d].  No LDS traffic beyond one read per
+ * parameter, so the ring is unrolled); S1 and S2 (integers below 2^53, exact) are running
+ * sums over the ring, Sg = sum_d E(d) ring_g[d] is added per output pixel.  No LDS traffic beyond one read per
  * (input row, window column). */
 constexpr int kGTX = 64, kGTY = 64, kGRows = kGTY / 4;
 
@@ -127,25 +127,31 @@ __global__ void __launch_bounds__(256) glrt_kernel(const uint16_t *frames, int n
             g += e[d] * v;
         }
     };
+    /* S1 and S2 are sums of integers below 2^53, exact in FP64 in any order: running sums
+     * over the window's rows (add the entering row, subtract the leaving one) */
+    double S1r = 0.0, S2r = 0.0;
 #pragma unroll
-    for (int d = 0; d < WS - 1; ++d) row_sums(rb + d, r1[d], r2[d], rg[d]);
+    for (int d = 0; d < WS - 1; ++d) {
+        row_sums(rb + d, r1[d], r2[d], rg[d]);
+        S1r += r1[d];
+        S2r += r2[d];
+    }
 #pragma unroll
     for (int k = 0; k < kGRows; ++k) {
         const int slot = (k + WS - 1) % WS;
         row_sums(rb + k + WS - 1, r1[slot], r2[slot], rg[slot]);
+        S1r += r1[slot];
+        S2r += r2[slot];
         const int x = x0 + c, y = y0 + rb + k;
         bool seg = false;
         if (x >= H + 1 && x < X - H && y >= H + 1 && y < Y - H) {
-            double S1 = 0.0, S2 = 0.0, Sg = 0.0;
+            double Sg = 0.0;
 #pragma unroll
-            for (int d = 0; d < WS; ++d) {
-                const int sl = (k + d) % WS; /* input row rb + k + d */
-                S1 += r1[sl];
-                S2 += r2[sl];
-                Sg += e[d] * rg[sl];
-            }
-            seg = glrt_test(P, S1, S2, Sg);
+            for (int d = 0; d < WS; ++d) Sg += e[d] * rg[(k + d) % WS]; /* input row rb + k + d */
+            seg = glrt_test(P, S1r, S2r, Sg);
         }
+        S1r -= r1[k % WS]; /* input row rb + k leaves the window */
+        S2r -= r2[k % WS];
         /* wave-aggregated append to the frame's list (the wave walks one row at a time) */
         const unsigned long long m = __ballot(seg);
         if (m) {
