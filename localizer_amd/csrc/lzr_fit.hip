@@ -230,7 +230,7 @@ struct RoundArgs {
 };
 
 /* W = minimum waves per SIMD the register allocation must allow (occupancy target). */
-template <int P, typename Acc, bool F64, int W>
+template <int P, typename Acc, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     extern __shared__ double lds_words[];
@@ -783,23 +783,23 @@ static void ensure(T *&ptr, size_t &cap, size_t need) {
     cap = n;
 }
 
-template <int P, typename Acc, bool F64, int W>
+template <int P, typename Acc, int W>
 static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream);
 
-template <int P, typename Acc, bool F64, int W>
+template <int P, typename Acc, int W>
 static int grid_for(LzrContext *ctx, size_t lds) {
     int per_cu = 0;
-    LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, F64, W>, 64, lds));
+    LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, W>, 64, lds));
     if (per_cu < 1) throw std::string("fitting kernel cannot be resident with this window size");
     if (ctx->waves_per_cu > 0) per_cu = std::min(per_cu, ctx->waves_per_cu);
     return per_cu * ctx->num_cus;
 }
 
 /* Parking buffers of a class (used when rounds > 1). */
-template <int P, typename Acc, bool F64, int W>
+template <int P, typename Acc, int W>
 static void ensure_park(LzrContext *ctx, int cls, size_t lds) {
     if (ctx->rounds <= 1) return;
-    const size_t cap = (size_t)grid_for<P, Acc, F64, W>(ctx, lds) * 64;
+    const size_t cap = (size_t)grid_for<P, Acc, W>(ctx, lds) * 64;
     for (int b = 0; b < 2; ++b) {
         if (ctx->park_cap[cls][b] < cap) {
             if (ctx->d_park[cls][b]) LZR_CHECK(hipFree(ctx->d_park[cls][b]));
@@ -811,10 +811,10 @@ static void ensure_park(LzrContext *ctx, int cls, size_t lds) {
     }
 }
 
-template <int P, typename Acc, bool F64, int W>
+template <int P, typename Acc, int W>
 static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
-    const int grid = grid_for<P, Acc, F64, W>(ctx, lds);
-    ensure_park<P, Acc, F64, W>(ctx, cls, lds);
+    const int grid = grid_for<P, Acc, W>(ctx, lds);
+    ensure_park<P, Acc, W>(ctx, cls, lds);
     unsigned *ctr = ctx->d_park_ctr + 4 * cls; /* buffer b: count ctr[2b], head ctr[2b+1] */
     for (int round = 0; round < ctx->rounds; ++round) {
         const int ob = round & 1, ib = ob ^ 1;
@@ -827,7 +827,7 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
         r.out = ctx->d_park[cls][ob];
         r.out_count = ctr + 2 * ob;
         r.threshold = round == ctx->rounds - 1 ? 0 : ctx->park_threshold;
-        hipLaunchKernelGGL((lm_persistent<P, Acc, F64, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
+        hipLaunchKernelGGL((lm_persistent<P, Acc, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
     }
     if (a.strag_trials > 0) {
         const int nmax = 2 * ctx->cfg.max_half_width + 1;
@@ -850,7 +850,6 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
 }
 
 typedef double AccT;           /* normal-equation accumulation type      */
-constexpr bool kF64Residual = true; /* current-point residual in FP64 */
 
 static size_t launch_lds(const LzrContext *ctx) {
     const int nmax = 2 * ctx->cfg.max_half_width + 1;
@@ -881,11 +880,11 @@ static void prepare(LzrContext *ctx, int ncands) {
     ensure_strag(ctx, ncands);
     const size_t lds = launch_lds(ctx);
     if (occupancy5(ctx) == 1)
-        ensure_park<5, AccT, kF64Residual, 1>(ctx, 0, lds);
+        ensure_park<5, AccT, 1>(ctx, 0, lds);
     else
-        ensure_park<5, AccT, kF64Residual, 2>(ctx, 0, lds);
-    ensure_park<7, AccT, kF64Residual, 1>(ctx, 1, lds);
-    ensure_park<4, AccT, kF64Residual, 2>(ctx, 2, lds);
+        ensure_park<5, AccT, 2>(ctx, 0, lds);
+    ensure_park<7, AccT, 1>(ctx, 1, lds);
+    ensure_park<4, AccT, 2>(ctx, 2, lds);
 }
 
 /* Enqueues one fit of ncands candidates (d_count, when not null, holds the live count
@@ -926,12 +925,12 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
      * another class's main kernel (fork/join by events; also inside a graph capture). */
     LZR_CHECK(hipEventRecord(ctx->fork, stream));
     for (int c = 0; c < kClasses - 1; ++c) LZR_CHECK(hipStreamWaitEvent(ctx->side[c], ctx->fork, 0));
-    run_class<7, AccT, kF64Residual, 1>(ctx, a, 1, lds, ctx->side[0]);
-    run_class<4, AccT, kF64Residual, 2>(ctx, a, 2, lds, ctx->side[1]);
+    run_class<7, AccT, 1>(ctx, a, 1, lds, ctx->side[0]);
+    run_class<4, AccT, 2>(ctx, a, 2, lds, ctx->side[1]);
     if (occupancy5(ctx) == 1)
-        run_class<5, AccT, kF64Residual, 1>(ctx, a, 0, lds, stream);
+        run_class<5, AccT, 1>(ctx, a, 0, lds, stream);
     else
-        run_class<5, AccT, kF64Residual, 2>(ctx, a, 0, lds, stream);
+        run_class<5, AccT, 2>(ctx, a, 0, lds, stream);
     for (int c = 0; c < kClasses - 1; ++c) {
         LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
         LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));

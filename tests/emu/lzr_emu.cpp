@@ -7,7 +7,7 @@
 
 using namespace lzr;
 
-template <int P, typename Acc, bool F64>
+template <int P>
 static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int ny, int nframes,
                     const LzrCandidate &c, int idx, int model, LzrFit &out) {
     LaneState<P> s;
@@ -19,7 +19,7 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) D[i + n * j] = (float)src[(size_t)j * nx + i];
     for (;;) {
-        PassAcc<P, Acc> a;
+        PassAcc<P, double> a;
         a.zero();
         struct {
             const float *D;
@@ -37,15 +37,15 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             lane_ell_in(s, cfg, q);
             double v[kEllSums];
             ell_pass(q, n, n, pix, xb, v);
-            ell_finish<Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
+            ell_finish<double>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
         } else {
             SepIn q;
             lane_sep_in<P>(s, cfg, q);
             double v[kSepSums];
             sep_pass(q, n, n, false, pix, xb, v);
-            sep_finish<P, Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
+            sep_finish<P, double>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
         }
-        if (!lane_after_pass<P, Acc>(s, cfg, a, model, reinterpret_cast<LzrRawFit *>(&out))) {
+        if (!lane_after_pass<P, double>(s, cfg, a, model, reinterpret_cast<LzrRawFit *>(&out))) {
             finish_any(out, cfg, c);
             return;
         }
@@ -53,9 +53,8 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
 }
 
 extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, int max_h, int diag_rows,
-                             int max_iter, double epsabs, double epsrel, int acc_double,
-                             const uint16_t *frames, int nx, int ny, int nframes,
-                             const LzrCandidate *cands, int ncands, LzrFit *out) {
+                             int max_iter, double epsabs, double epsrel, const uint16_t *frames, int nx, int ny,
+                             int nframes, const LzrCandidate *cands, int ncands, LzrFit *out) {
     LaneConfig cfg;
     cfg.initial_psf_width = w;
     cfg.sigma = sigma;
@@ -70,20 +69,12 @@ extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, i
     int accepted = 0;
     for (int i = 0; i < ncands; ++i) {
         const int m = cands[i].model >= 0 ? cands[i].model : model;
-        const bool f64 = acc_double & 2;
-        const bool dacc = acc_double & 1;
-#define LZR_EMU_CALL(PP, AA, FF) fit_one<PP, AA, FF>(cfg, frames, nx, ny, nframes, cands[i], i, m, out[i])
-        if (m == LZR_MODEL_SYMMETRIC) {
-            if (dacc) { if (f64) LZR_EMU_CALL(5, double, true); else LZR_EMU_CALL(5, double, false); }
-            else { if (f64) LZR_EMU_CALL(5, float, true); else LZR_EMU_CALL(5, float, false); }
-        } else if (m == LZR_MODEL_FIXED_WIDTH) {
-            if (dacc) { if (f64) LZR_EMU_CALL(4, double, true); else LZR_EMU_CALL(4, double, false); }
-            else { if (f64) LZR_EMU_CALL(4, float, true); else LZR_EMU_CALL(4, float, false); }
-        } else {
-            if (dacc) { if (f64) LZR_EMU_CALL(7, double, true); else LZR_EMU_CALL(7, double, false); }
-            else { if (f64) LZR_EMU_CALL(7, float, true); else LZR_EMU_CALL(7, float, false); }
-        }
-#undef LZR_EMU_CALL
+        if (m == LZR_MODEL_SYMMETRIC)
+            fit_one<5>(cfg, frames, nx, ny, nframes, cands[i], i, m, out[i]);
+        else if (m == LZR_MODEL_FIXED_WIDTH)
+            fit_one<4>(cfg, frames, nx, ny, nframes, cands[i], i, m, out[i]);
+        else
+            fit_one<7>(cfg, frames, nx, ny, nframes, cands[i], i, m, out[i]);
         accepted += out[i].status == LZR_ACCEPTED;
     }
     return accepted;

@@ -15,7 +15,7 @@ def lib():
     if _LIB is None:
         L = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblzr_emu.so"))
         L.emu_fit_batch.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_int, ctypes.c_void_p]
         L.emu_fit_batch.restype = ctypes.c_int
@@ -30,7 +30,7 @@ def lib():
 
 
 def fit_batch(frames, cands, model, w, sigma=1.0, default_half_width=0, max_half_width=0, diag_rows=-1,
-              max_iter=200, epsabs=1e-4, epsrel=1e-4, acc_double=True, f64_residual=True):
+              max_iter=200, epsabs=1e-4, epsrel=1e-4):
     frames = np.ascontiguousarray(frames, np.uint16)
     if frames.ndim == 2:
         frames = frames[None]
@@ -39,7 +39,7 @@ def fit_batch(frames, cands, model, w, sigma=1.0, default_half_width=0, max_half
     h = default_half_width if default_half_width > 0 else abi.default_half_width(w)
     out = np.zeros(cands.size, abi.FIT_DTYPE)
     lib().emu_fit_batch(int(model), float(w), float(sigma), int(h), int(max(max_half_width, h)), int(diag_rows),
-                        int(max_iter), float(epsabs), float(epsrel), int(bool(acc_double)) | (2 if f64_residual else 0), frames.ctypes.data,
+                        int(max_iter), float(epsabs), float(epsrel), frames.ctypes.data,
                         nx, ny, nf, cands.ctypes.data, cands.size, out.ctypes.data)
     return out
 
