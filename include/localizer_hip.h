@@ -111,10 +111,10 @@ typedef struct LzrContext LzrContext;
 #define LZR_OPT_DIAG_UPDATE_ROWS 4 /* rows used by the lmsder scale update: 0 = all, default p (GSL 1.16) */
 #define LZR_OPT_WAVES_PER_CU 5     /* persistent-kernel occupancy (autotuned when 0)                   */
 #define LZR_OPT_MAX_HALF_WIDTH 6   /* largest per-candidate window half-width accepted (default: the context's) */
-#define LZR_OPT_OCCUPANCY 7        /* register budget of the fitting kernel: 1 or 2 waves per SIMD; 0 (default) = 2 when the window tiles of 8 waves fit a CU's LDS */
+#define LZR_OPT_OCCUPANCY 7        /* register budget of the fitting kernel: 1 or 2 waves per SIMD; 0 (default) = 2 when the window tiles of 6 waves fit a CU's LDS */
 #define LZR_OPT_ROUNDS 8           /* fitting-kernel launches per model class; unfinished fits are parked between them (default 1) */
 #define LZR_OPT_PARK_THRESHOLD 9   /* a wave parks its fits once the queue is drained and at most this many lanes are busy (default 16) */
-#define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues on 16 lanes in the straggler kernel (default 48, 0 = off) */
+#define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues in the straggler kernel, on 1..16 lanes by the count of such fits (default 48, 0 = off) */
 
 /* Creates a fitting context on the current HIP device.
  * Replaces the constructors FitPositions_SymmetricGaussian(initialPSFWidth, sigma)
