@@ -157,6 +157,35 @@ def test_gpu_mixed_models_and_windows():
         assert st["status_agreement"] >= 0.99
 
 
+@pytest.fixture(scope="module")
+def ellipsoidal_100k(plain_100k):
+    frames, cands, _ = plain_100k
+    ref = po.fit_batch(frames, cands, abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4)
+    return frames, cands, ref
+
+
+@pytest.mark.parametrize("trials", [3, 8, 14, 23, 48])
+def test_gpu_ellipsoidal_straggler_group_sizes(ellipsoidal_100k, trials):
+    """The ellipsoidal straggler pass (rows of the window split over the group, moments added
+    by the butterfly) at every group size: after 3 / 8 / 14 / 23 / 48 trials about 100k /
+    42k / 25k / 13k / 3k of the 100k fits are handed off (groups of 1, 2, 4, 8, 16 lanes on
+    a 256-CU MI355X).  Against the oracle as the ellipsoidal parity test, repeat calls
+    bit-identical."""
+    frames, cands, ref = ellipsoidal_100k
+    _say(f"case: ellipsoidal 9x9, 100k fits, hand-off after {trials} trials")
+    with native.Context(abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4) as ctx:
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, trials)
+        got = ctx.fit(frames, cands)
+        again = ctx.fit(frames, cands)
+    st = compare(ref, got, abi.MODEL_ELLIPSOIDAL)
+    _say(format_stats(f"gpu vs oracle ellipsoidal, ~{int(np.sum(ref['trials'] > trials))} fits handed off", st))
+    assert st["status_agreement"] >= 0.995
+    ok = (ref["lm_status"] == 0) & (got["lm_status"] == 0) & (ref["status"] == 0) & (got["status"] == 0)
+    dxy = np.abs(ref["params"][ok][:, 3:5] - got["params"][ok][:, 3:5]).max(axis=1)
+    assert np.mean(dxy <= 1e-3) >= 0.995
+    assert np.array_equal(got.view(np.uint8), again.view(np.uint8))
+
+
 def test_gpu_mixed_windows_bit_identical():
     """Lanes with different window sizes share a wave (its loop runs to the largest window),
     and which fits share a wave depends on timing: repeated calls must be bit-identical for
