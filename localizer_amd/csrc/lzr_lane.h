@@ -510,6 +510,22 @@ struct GaussRec {
     }
 };
 
+/* Pixels per step of the separable pass's row loop (odd windows of 9 and 15 split evenly). */
+constexpr int kSepChunk = 3;
+
+/* Adds one pixel (value D, column factor ex at offset u) to a row's inner sums. */
+LZR_HD LZR_INLINE void sep_row_pixel(double D, double ex, double u, double &r1, double &ru, double &ru2,
+                                     double &rd, double &rdd) {
+    LZR_NO_CONTRACT
+    const double t = ex * D;
+    const double tu = t * u;
+    r1 += t;
+    ru += tu;
+    ru2 = dfma(tu, u, ru2);
+    rd += D;
+    rdd = dfma(D, D, rdd);
+}
+
 /* The separable pass of one lane over its n x n window (nloop >= n: the loop bound shared
  * by the wave; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
  * returns pixel (i, j) as stored, xb holds one FP64 value per column. */
@@ -538,41 +554,25 @@ LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, co
         ey = j < n ? ey : 0.0;
         axis_moments(ey, w, v + kMy0);
         double r1 = 0.0, ru = 0.0, ru2 = 0.0, rd = 0.0, rdd = 0.0;
-        /* the next pixel's LDS reads are issued before this pixel's arithmetic */
-        auto Dn = pix.raw(0, j);
-        double xn = xb.get(0);
-        if (guard) {
-            const bool row_in = j < n;
-            for (int i = 0; i < nloop; ++i) {
-                const double D = row_in && i < n ? (double)Dn : 0.0;
-                const double ex = xn;
-                const int inext = i + 1 < nloop ? i + 1 : i; /* unconditional, in-bounds read */
-                Dn = pix.raw(inext, j);
-                xn = xb.get(inext);
-                const double u = (double)i - q.x1;
-                const double t = ex * D;
-                const double tu = t * u;
-                r1 += t;
-                ru += tu;
-                ru2 = dfma(tu, u, ru2);
-                rd += D;
-                rdd = dfma(D, D, rdd);
+        const bool row_in = !guard || j < n;
+        /* kSepChunk pixels per step: all their reads are issued before their arithmetic, so
+         * one LDS latency covers the chunk.  Reads past the loop bound are clamped in bounds
+         * and skipped (nloop is uniform over the wave). */
+        for (int i0 = 0; i0 < nloop; i0 += kSepChunk) {
+            decltype(pix.raw(0, 0)) Dc[kSepChunk];
+            double Xc[kSepChunk];
+#pragma unroll
+            for (int k = 0; k < kSepChunk; ++k) {
+                const int i = i0 + k < nloop ? i0 + k : nloop - 1;
+                Dc[k] = pix.raw(i, j);
+                Xc[k] = xb.get(i);
             }
-        } else {
-            for (int i = 0; i < nloop; ++i) {
-                const double D = (double)Dn;
-                const double ex = xn;
-                const int inext = i + 1 < nloop ? i + 1 : i;
-                Dn = pix.raw(inext, j);
-                xn = xb.get(inext);
-                const double u = (double)i - q.x1;
-                const double t = ex * D;
-                const double tu = t * u;
-                r1 += t;
-                ru += tu;
-                ru2 = dfma(tu, u, ru2);
-                rd += D;
-                rdd = dfma(D, D, rdd);
+#pragma unroll
+            for (int k = 0; k < kSepChunk; ++k) {
+                const int i = i0 + k;
+                if (i >= nloop) break;
+                const double D = !guard || (row_in && i < n) ? (double)Dc[k] : 0.0;
+                sep_row_pixel(D, Xc[k], (double)i - q.x1, r1, ru, ru2, rd, rdd);
             }
         }
         row_sums(ey, w, r1, ru, ru2, rd, rdd, v);
@@ -782,21 +782,35 @@ LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, const Pix &pix
         }
         double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0}, W[3] = {0.0, 0.0, 0.0};
         double rd = 0.0, rdd = 0.0;
-        for (int i = 0; i < nloop; ++i) {
-            const bool in = row_in && i < n;
-            const double D = in ? (double)pix.raw(i, j) : 0.0;
-            const double u = (double)i - q.x1;
-            /* outside the lane's window the factors may overflow (the range check covers
-             * the window only): masked by a select, never by a product with 0 */
-            double e;
-            if (fac) {
-                const double ef = xb.get(i) * c;
-                c = c * g;
-                e = in ? ef : 0.0;
-            } else {
-                e = in ? dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * v + (q.qb * v) * v)) : 0.0;
+        /* kSepChunk pixels per step, reads first (clamped in bounds, skipped past nloop) */
+        for (int i0 = 0; i0 < nloop; i0 += kSepChunk) {
+            decltype(pix.raw(0, 0)) Dc[kSepChunk];
+            double Xc[kSepChunk];
+#pragma unroll
+            for (int k = 0; k < kSepChunk; ++k) {
+                const int i = i0 + k < nloop ? i0 + k : nloop - 1;
+                Dc[k] = pix.raw(i, j);
+                Xc[k] = xb.get(i);
             }
-            ell_pixel(e, u, D, P, R, W, rd, rdd);
+#pragma unroll
+            for (int k = 0; k < kSepChunk; ++k) {
+                const int i = i0 + k;
+                if (i >= nloop) break;
+                const bool in = row_in && i < n;
+                const double D = in ? (double)Dc[k] : 0.0;
+                const double u = (double)i - q.x1;
+                /* outside the lane's window the factors may overflow (the range check covers
+                 * the window only): masked by a select, never by a product with 0 */
+                double e;
+                if (fac) {
+                    const double ef = Xc[k] * c;
+                    c = c * g;
+                    e = in ? ef : 0.0;
+                } else {
+                    e = in ? dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * v + (q.qb * v) * v)) : 0.0;
+                }
+                ell_pixel(e, u, D, P, R, W, rd, rdd);
+            }
         }
         ell_row(row_in ? w : 0.0, v, P, R, W, rd, rdd, s);
     }
