@@ -373,7 +373,7 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             const uint16_t *t;
             int n;
             __device__ float operator()(int i, int j) const { return (float)t[(i + j * n) * 64]; }
-            __device__ uint16_t raw(int i, int j) const { return t[(i + j * n) * 64]; }
+            __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
         } pix{tile + lane, active ? n : nw};
         struct {
             double *c;
@@ -394,7 +394,7 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
                 make_ell_in(t0, d0, 1.0, q);
             }
             double v7[kEllSums];
-            ell_pass(q, n, nw, pix, xb, v7);
+            ell_pass(q, n, nw, mixed, pix, xb, v7);
             ell_finish<Acc>(q, n * n, v7, s.phase == kInit, s.fnorm * s.fnorm, acc);
         } else {
             SepIn q;
@@ -575,14 +575,14 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                 struct {
                     const uint16_t *t;
                     int n;
-                    __device__ uint16_t raw(int i, int j) const { return t[(i + j * n) * 64]; }
+                    __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
                 } pix{tile + lane, n};
                 struct {
                     double *c;
                     __device__ void set(int i, double x) { c[i * 64] = x; }
                     __device__ double get(int i) const { return c[i * 64]; }
                 } xb{xcol + lane};
-                ell_pass(q, n, nw, pix, xb, v7);
+                ell_pass(q, n, nw, __any(active && n != nw), pix, xb, v7);
             } else {
                 /* moment pass split over the group: lane r computes the column factors of
                  * columns r, r + k, ... and the rows r, r + k, ... of the window; the
@@ -642,7 +642,7 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                 struct {
                     const uint16_t *t;
                     int n;
-                    __device__ uint16_t raw(int i, int j) const { return t[(i + j * n) * 64]; }
+                    __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
                 } pix{tile + lane, n};
                 struct {
                     double *c;

@@ -526,6 +526,32 @@ LZR_HD LZR_INLINE void sep_row_pixel(double D, double ex, double u, double &r1, 
     rdd = dfma(D, D, rdd);
 }
 
+/* kSepChunk pixels of row j from column i0: all their reads are issued before their
+ * arithmetic, so one LDS latency covers the chunk (full chunks read at fixed offsets from the
+ * chunk's first pixel).  Tail: reads past nloop are clamped in bounds and skipped (nloop is
+ * uniform over the wave).  Guard: pixels beyond the lane's own n x n window are masked. */
+template <bool Tail, bool Guard, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void sep_chunk(const SepIn &q, int n, int nloop, bool row_in, const Pix &pix,
+                                 const XBuf &xb, int i0, int j, double &r1, double &ru, double &ru2, double &rd,
+                                 double &rdd) {
+    LZR_NO_CONTRACT
+    decltype(pix.raw(0, 0)) Dc[kSepChunk];
+    double Xc[kSepChunk];
+#pragma unroll
+    for (int k = 0; k < kSepChunk; ++k) {
+        const int i = Tail && i0 + k >= nloop ? nloop - 1 : i0 + k;
+        Dc[k] = pix.raw(i, j);
+        Xc[k] = xb.get(i);
+    }
+#pragma unroll
+    for (int k = 0; k < kSepChunk; ++k) {
+        const int i = i0 + k;
+        if (Tail && i >= nloop) break;
+        const double D = !Guard || (row_in && i < n) ? (double)Dc[k] : 0.0;
+        sep_row_pixel(D, Xc[k], (double)i - q.x1, r1, ru, ru2, rd, rdd);
+    }
+}
+
 /* The separable pass of one lane over its n x n window (nloop >= n: the loop bound shared
  * by the wave; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
  * returns pixel (i, j) as stored, xb holds one FP64 value per column. */
@@ -555,25 +581,15 @@ LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, co
         axis_moments(ey, w, v + kMy0);
         double r1 = 0.0, ru = 0.0, ru2 = 0.0, rd = 0.0, rdd = 0.0;
         const bool row_in = !guard || j < n;
-        /* kSepChunk pixels per step: all their reads are issued before their arithmetic, so
-         * one LDS latency covers the chunk.  Reads past the loop bound are clamped in bounds
-         * and skipped (nloop is uniform over the wave). */
-        for (int i0 = 0; i0 < nloop; i0 += kSepChunk) {
-            decltype(pix.raw(0, 0)) Dc[kSepChunk];
-            double Xc[kSepChunk];
-#pragma unroll
-            for (int k = 0; k < kSepChunk; ++k) {
-                const int i = i0 + k < nloop ? i0 + k : nloop - 1;
-                Dc[k] = pix.raw(i, j);
-                Xc[k] = xb.get(i);
-            }
-#pragma unroll
-            for (int k = 0; k < kSepChunk; ++k) {
-                const int i = i0 + k;
-                if (i >= nloop) break;
-                const double D = !guard || (row_in && i < n) ? (double)Dc[k] : 0.0;
-                sep_row_pixel(D, Xc[k], (double)i - q.x1, r1, ru, ru2, rd, rdd);
-            }
+        int i0 = 0;
+        if (guard) {
+            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
+                sep_chunk<false, true>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
+            if (i0 < nloop) sep_chunk<true, true>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
+        } else {
+            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
+                sep_chunk<false, false>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
+            if (i0 < nloop) sep_chunk<true, false>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
         }
         row_sums(ey, w, r1, ru, ru2, rd, rdd, v);
     }
@@ -739,11 +755,47 @@ LZR_HD LZR_INLINE void ell_pixel(double e, double u, double D, double (&P)[5], d
     rdd = dfma(D, D, rdd);
 }
 
+/* kSepChunk pixels of row j (offset v) from column i0 for the ellipsoidal pass, reads first
+ * (Tail and Guard as for sep_chunk); c is the row's cross factor, advanced by g per column. */
+template <bool Tail, bool Guard, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void ell_chunk(const EllIn &q, int n, int nloop, bool fac, bool row_in, double v, double g,
+                                 double &c, const Pix &pix, const XBuf &xb, int i0, int j, double (&P)[5],
+                                 double (&R)[3], double (&W)[3], double &rd, double &rdd) {
+    LZR_NO_CONTRACT
+    decltype(pix.raw(0, 0)) Dc[kSepChunk];
+    double Xc[kSepChunk];
+#pragma unroll
+    for (int k = 0; k < kSepChunk; ++k) {
+        const int i = Tail && i0 + k >= nloop ? nloop - 1 : i0 + k;
+        Dc[k] = pix.raw(i, j);
+        Xc[k] = xb.get(i);
+    }
+#pragma unroll
+    for (int k = 0; k < kSepChunk; ++k) {
+        const int i = i0 + k;
+        if (Tail && i >= nloop) break;
+        const bool in = !Guard || (row_in && i < n);
+        const double D = in ? (double)Dc[k] : 0.0;
+        const double u = (double)i - q.x1;
+        /* outside the lane's window the factors may overflow (the range check covers the
+         * window only): masked by a select, never by a product with 0 */
+        double e;
+        if (fac) {
+            const double ef = Xc[k] * c;
+            c = c * g;
+            e = in ? ef : 0.0;
+        } else {
+            e = in ? dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * v + (q.qb * v) * v)) : 0.0;
+        }
+        ell_pixel(e, u, D, P, R, W, rd, rdd);
+    }
+}
+
 /* The ellipsoidal moment pass of one lane over its n x n window (nloop >= n: the wave's
- * loop bound; pixels beyond n are masked).  pix.raw(i, j) returns pixel (i, j) as stored,
- * xb holds one FP64 value per column. */
+ * loop bound; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
+ * returns pixel (i, j) as stored, xb holds one FP64 value per column. */
 template <typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, const Pix &pix, XBuf &xb,
+LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
                                 double (&s)[kEllSums]) {
     LZR_NO_CONTRACT
 #pragma unroll
@@ -782,35 +834,16 @@ LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, const Pix &pix
         }
         double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0}, W[3] = {0.0, 0.0, 0.0};
         double rd = 0.0, rdd = 0.0;
-        /* kSepChunk pixels per step, reads first (clamped in bounds, skipped past nloop) */
-        for (int i0 = 0; i0 < nloop; i0 += kSepChunk) {
-            decltype(pix.raw(0, 0)) Dc[kSepChunk];
-            double Xc[kSepChunk];
-#pragma unroll
-            for (int k = 0; k < kSepChunk; ++k) {
-                const int i = i0 + k < nloop ? i0 + k : nloop - 1;
-                Dc[k] = pix.raw(i, j);
-                Xc[k] = xb.get(i);
-            }
-#pragma unroll
-            for (int k = 0; k < kSepChunk; ++k) {
-                const int i = i0 + k;
-                if (i >= nloop) break;
-                const bool in = row_in && i < n;
-                const double D = in ? (double)Dc[k] : 0.0;
-                const double u = (double)i - q.x1;
-                /* outside the lane's window the factors may overflow (the range check covers
-                 * the window only): masked by a select, never by a product with 0 */
-                double e;
-                if (fac) {
-                    const double ef = Xc[k] * c;
-                    c = c * g;
-                    e = in ? ef : 0.0;
-                } else {
-                    e = in ? dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * v + (q.qb * v) * v)) : 0.0;
-                }
-                ell_pixel(e, u, D, P, R, W, rd, rdd);
-            }
+        int i0 = 0;
+        if (guard) {
+            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
+                ell_chunk<false, true>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
+            if (i0 < nloop) ell_chunk<true, true>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
+        } else {
+            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
+                ell_chunk<false, false>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
+            if (i0 < nloop)
+                ell_chunk<true, false>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
         }
         ell_row(row_in ? w : 0.0, v, P, R, W, rd, rdd, s);
     }

@@ -36,7 +36,7 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             EllIn q;
             lane_ell_in(s, cfg, q);
             double v[kEllSums];
-            ell_pass(q, n, n, pix, xb, v);
+            ell_pass(q, n, n, false, pix, xb, v);
             ell_finish<double>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
         } else {
             SepIn q;
@@ -139,7 +139,7 @@ extern "C" int emu_ell_pass(const double *theta, const double *dx, double sigma,
         double get(int i) const { return v[i]; }
     } xb;
     double v[kEllSums];
-    ell_pass(q, n, nloop, pix, xb, v);
+    ell_pass(q, n, nloop, nloop != n, pix, xb, v);
     PassAcc<7, double> a;
     a.zero();
     ell_finish<double>(q, n * n, v, true, 0.0, a);
