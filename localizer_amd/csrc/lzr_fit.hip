@@ -692,7 +692,7 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
     }
 }
 
-/* Which group sizes a straggler launch runs: all, groups of 2..16 lanes, or one lane per
+/* Which group sizes a straggler launch runs: all, groups of 4..16 lanes, or one lane per
  * fit.  One lane per fit (many long fits) is throughput work and runs best at two waves
  * per SIMD; groups are latency work and run best with the full register file, so the
  * symmetric and fixed-width classes launch one build of each (the one that does not
@@ -708,6 +708,10 @@ lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     const Parked<P> *park = strag_region<P>(args, cls);
     int lk = kMaxGroup == 16 ? 4 : 3; /* log2 of the group size */
     while (lk > 0 && (unsigned long long)count << lk > 2ull * lanes_total) --lk; /* <= 2 fits per group */
+    /* The one-lane build runs two waves per SIMD and so holds four times the fits of 2-lane
+     * groups, whose split pass saves little of a trial (every lane repeats the LM logic):
+     * the classes that have that build use it instead of 2-lane groups. */
+    if (Role != kAllGroups && lk == 1) lk = 0;
     lk = __builtin_amdgcn_readfirstlane(lk);
     if constexpr (Role == kOneLane) {
         if (lk == 0) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
@@ -717,7 +721,9 @@ lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     case 4: straggler_groups<P, Acc, 4>(args, count, head, park, lds_words); break;
     case 3: straggler_groups<P, Acc, 3>(args, count, head, park, lds_words); break;
     case 2: straggler_groups<P, Acc, 2>(args, count, head, park, lds_words); break;
-    case 1: straggler_groups<P, Acc, 1>(args, count, head, park, lds_words); break;
+    case 1:
+        if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 1>(args, count, head, park, lds_words);
+        break;
     default:
         if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
         break;

@@ -125,7 +125,8 @@ def test_gpu_straggler_group_sizes(plain_100k, trials):
     """Fits still running after `trials` trials go to the straggler kernel, whose group size
     follows their count (at most two turns of the launch's 65536 lanes on a 256-CU MI355X):
     after 4 / 11 / 17 / 24 / 48 trials about 99k / 46k / 25k / 13k / 1.7k of the 100k fits
-    are handed off, which selects groups of 1 (the two-wave build), 2, 4, 8 and 16 lanes.
+    are handed off, which selects one lane per fit (the two-wave build; this model skips
+    2-lane groups) for the first two and groups of 4, 8 and 16 lanes for the others.
     Every group size must match the oracle, and repeat calls must be bit-identical."""
     frames, cands, ref = plain_100k
     _say(f"case: symmetric 9x9, 100k fits, hand-off after {trials} trials")
