@@ -484,6 +484,9 @@ __device__ __forceinline__ double group_sum(double v, int k) {
     return v;
 }
 
+/* Pixels per step of the group pass of the separable models (reads first). */
+constexpr int kGroupChunk = 4;
+
 /* The straggler loop for groups of 2^LK lanes. */
 template <int P, typename Acc, int LK>
 __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
@@ -605,18 +608,30 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                     }
                     double Pm[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Rm[3] = {0.0, 0.0, 0.0}, Wm[3] = {0.0, 0.0, 0.0};
                     double rd = 0.0, rdd = 0.0;
-                    for (int i = 0; i < n; ++i) {
-                        const int p = i + n * j;
-                        const double D = (double)tile[(p >> lk) * 64 + grp * kGroup + (p & kMask)];
-                        const double u = (double)i - q.x1;
-                        double e;
-                        if (fac) {
-                            e = gx[(i >> lk) * 64 + (i & kMask)] * c;
-                            c = c * g;
-                        } else {
-                            e = dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * vv + (q.qb * vv) * vv));
+                    /* kSepChunk pixels per step, reads first (clamped to the row) */
+                    for (int i0 = 0; i0 < n; i0 += kSepChunk) {
+                        double Dk[kSepChunk], Xk[kSepChunk];
+#pragma unroll
+                        for (int k = 0; k < kSepChunk; ++k) {
+                            const int i = i0 + k < n ? i0 + k : n - 1;
+                            const int p = i + n * j;
+                            Dk[k] = (double)tile[(p >> lk) * 64 + grp * kGroup + (p & kMask)];
+                            Xk[k] = gx[(i >> lk) * 64 + (i & kMask)];
                         }
-                        ell_pixel(e, u, D, Pm, Rm, Wm, rd, rdd);
+#pragma unroll
+                        for (int k = 0; k < kSepChunk; ++k) {
+                            const int i = i0 + k;
+                            if (i >= n) break;
+                            const double u = (double)i - q.x1;
+                            double e;
+                            if (fac) {
+                                e = Xk[k] * c;
+                                c = c * g;
+                            } else {
+                                e = dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * vv + (q.qb * vv) * vv));
+                            }
+                            ell_pixel(e, u, Dk[k], Pm, Rm, Wm, rd, rdd);
+                        }
                     }
                     ell_row(w, vv, Pm, Rm, Wm, rd, rdd, v7);
                 }
@@ -669,13 +684,34 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                     }
                 }
                 __syncthreads();
-                for (int m = 0; m < mmax; ++m) {
-                    const int pp = r + kGroup * m;
-                    if (pp < n2) {
+                /* kGroupChunk pixels per step: their tile and factor reads are issued before
+                 * the arithmetic.  A lane's slots past its window (pp >= n2) read pixel 0 and
+                 * add +0, which leaves every sum unchanged. */
+                for (int m0 = 0; m0 < mmax; m0 += kGroupChunk) {
+                    unsigned Dr[kGroupChunk];
+                    double ex[kGroupChunk], ey[kGroupChunk];
+                    int ii[kGroupChunk], jj[kGroupChunk];
+                    bool ok[kGroupChunk];
+#pragma unroll
+                    for (int k = 0; k < kGroupChunk; ++k) {
+                        const int m = m0 + k < mmax ? m0 + k : mmax - 1;
+                        const int pp0 = r + kGroup * (m0 + k);
+                        ok[k] = m0 + k < mmax && pp0 < n2;
+                        const int pp = ok[k] ? pp0 : 0;
                         const int j = (pp * inv) >> 16, i = pp - j * n;
                         const int ty = n + j;
-                        const double e = gx[(i >> lk) * 64 + (i & kMask)] * gx[(ty >> lk) * 64 + (ty & kMask)];
-                        sep_pixel(e, (double)i - q.x1, (double)j - q.y1, (double)tile[m * 64 + lane], v);
+                        ii[k] = i;
+                        jj[k] = j;
+                        Dr[k] = tile[m * 64 + lane];
+                        ex[k] = gx[(i >> lk) * 64 + (i & kMask)];
+                        ey[k] = gx[(ty >> lk) * 64 + (ty & kMask)];
+                    }
+#pragma unroll
+                    for (int k = 0; k < kGroupChunk; ++k) {
+                        if (m0 + k >= mmax) break;
+                        const double e = ok[k] ? ex[k] * ey[k] : 0.0;
+                        const double D = ok[k] ? (double)Dr[k] : 0.0;
+                        sep_pixel(e, (double)ii[k] - q.x1, (double)jj[k] - q.y1, D, v);
                     }
                 }
 #pragma unroll
