@@ -164,7 +164,7 @@ def _window_ell(rng, n, A, sx, sy, x0, y0, rho, b):
     return rng.poisson(b + A * np.exp(-Q / (2 * (1 - rho * rho)))).astype(np.uint16)
 
 
-@pytest.mark.parametrize("n", [5, 9, 15])
+@pytest.mark.parametrize("n", [5, 7, 9, 11, 15])
 def test_ellipsoidal_moment_pass_matches_pixel_sums(n):
     """The ellipsoidal moment pass (Ex Ey C factorisation, the cross factor by recurrence
     along rows) against the reference's columns summed pixel by pixel."""
@@ -192,6 +192,7 @@ def test_ellipsoidal_direct_fallback_and_masking():
                   [900, 30.0, 25.0, 5.0, 5.0, 0.0, 100]):  # nearly flat
         _check_ell(theta, np.zeros(7), 1.0, window)
         _check_ell(theta, np.zeros(7), 2.0, window, nloop=15)
+        _check_ell(theta, np.zeros(7), 2.0, window, nloop=13)
 
     # strong coupling (|qc| ~ 6.6): factored inside the 5 x 5 window, but the cross factor
     # overflows in the loop's columns and rows beyond it; those must contribute exactly 0
