@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -299,48 +300,59 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         if (!open && s.phase == kIdle) s.phase = kFinished;
 
         /* ---- gather the windows of newly loaded lanes ----
-         * kGatherLanes windows at a time, every load of the batch issued before the first
-         * LDS store, so the batch costs one global-memory latency. */
+         * GL windows at a time, every load of the batch issued before the first LDS store,
+         * so the batch costs one global-memory latency; each lane loads R pixels of each
+         * window per step (8 windows x 2 pixels when the windows have at most 128 pixels,
+         * else 4 x 4: the same 16 loads per lane). */
         unsigned long long gmask = __ballot(fresh);
-        while (gmask) {
-            constexpr int kGatherLanes = 4;
-            int Lb[kGatherLanes], nb[kGatherLanes], inv[kGatherLanes];
-            const uint16_t *src[kGatherLanes];
-            int most = 0;
+        const int gn = __builtin_amdgcn_readfirstlane(wave_max(fresh ? s.n : 0));
+        auto gather = [&](auto gl, auto rr) {
+            constexpr int kGatherLanes = decltype(gl)::value, R = decltype(rr)::value;
+            while (gmask) {
+                int Lb[kGatherLanes], nb[kGatherLanes], inv[kGatherLanes];
+                const uint16_t *src[kGatherLanes];
+                int most = 0;
 #pragma unroll
-            for (int b = 0; b < kGatherLanes; ++b) {
-                const int L = gmask ? __ffsll((long long)gmask) - 1 : -1;
-                if (gmask) gmask &= gmask - 1ull;
-                const int Ls = L < 0 ? 0 : L;
-                const int fL = __shfl(s.frame, Ls, 64);
-                const int xL = __shfl(s.xoff, Ls, 64);
-                const int yL = __shfl(s.yoff, Ls, 64);
-                const int nL = L < 0 ? 0 : __shfl(s.n, Ls, 64);
-                Lb[b] = Ls;
-                nb[b] = nL;
-                /* j = k / n as (k * inv) >> 16: exact for k < n^2 when n <= 31 (half-width <= 15) */
-                inv[b] = nL > 0 ? 65536 / nL + 1 : 0;
-                src[b] = args.frames + ((size_t)fL * args.ny + yL) * args.nx + xL;
-                most = max(most, nL * nL);
+                for (int b = 0; b < kGatherLanes; ++b) {
+                    const int L = gmask ? __ffsll((long long)gmask) - 1 : -1;
+                    if (gmask) gmask &= gmask - 1ull;
+                    const int Ls = L < 0 ? 0 : L;
+                    const int fL = __shfl(s.frame, Ls, 64);
+                    const int xL = __shfl(s.xoff, Ls, 64);
+                    const int yL = __shfl(s.yoff, Ls, 64);
+                    const int nL = L < 0 ? 0 : __shfl(s.n, Ls, 64);
+                    Lb[b] = Ls;
+                    nb[b] = nL;
+                    /* j = k / n as (k * inv) >> 16: exact for k < n^2 when n <= 31 (half-width <= 15) */
+                    inv[b] = nL > 0 ? 65536 / nL + 1 : 0;
+                    src[b] = args.frames + ((size_t)fL * args.ny + yL) * args.nx + xL;
+                    most = max(most, nL * nL);
+                }
+                for (int k0 = 0; k0 < most; k0 += R * 64) {
+                    uint16_t v[kGatherLanes][R];
+#pragma unroll
+                    for (int b = 0; b < kGatherLanes; ++b)
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int k = k0 + lane + 64 * r;
+                            const int j = (k * inv[b]) >> 16, i = k - j * nb[b];
+                            v[b][r] = k < nb[b] * nb[b] ? src[b][(size_t)j * args.nx + i] : (uint16_t)0;
+                        }
+#pragma unroll
+                    for (int b = 0; b < kGatherLanes; ++b)
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int k = k0 + lane + 64 * r;
+                            if (k < nb[b] * nb[b]) tile[k * 64 + Lb[b]] = v[b][r];
+                        }
+                }
             }
-            for (int k0 = 0; k0 < most; k0 += 4 * 64) {
-                uint16_t v[kGatherLanes][4];
-#pragma unroll
-                for (int b = 0; b < kGatherLanes; ++b)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int k = k0 + lane + 64 * r;
-                        const int j = (k * inv[b]) >> 16, i = k - j * nb[b];
-                        v[b][r] = k < nb[b] * nb[b] ? src[b][(size_t)j * args.nx + i] : (uint16_t)0;
-                    }
-#pragma unroll
-                for (int b = 0; b < kGatherLanes; ++b)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int k = k0 + lane + 64 * r;
-                        if (k < nb[b] * nb[b]) tile[k * 64 + Lb[b]] = v[b][r];
-                    }
-            }
+        };
+        if (gmask) {
+            if (gn * gn <= 128)
+                gather(std::integral_constant<int, 8>{}, std::integral_constant<int, 2>{});
+            else
+                gather(std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{});
         }
         fresh = false;
         __syncthreads();
