@@ -20,7 +20,8 @@
  * Pixel passes.  Symmetric and fixed-width models (separable exponential): the pass
  * at the trial point is a set of FP64 1-D moments over the window's columns and rows
  * plus five FP64 data sums over the pixels, from which J^T J, J^T f and |f|^2 follow
- * in closed form (sep_pass / sep_finish; about a dozen FP64 operations per pixel).
+ * in closed form (sep_pass / sep_finish: seven FP64 operations per pixel plus the pixel's
+ * conversion and column offset, pixels read from LDS three at a time).
  * Ellipsoidal model: its Jacobian columns are the exponential times polynomials of
  * degree <= 2 in the pixel offsets, so the pass accumulates FP64 moments of e1 and e1^2
  * up to degree 4 row by row (the cross factor exp(-2 qc u v) by recurrence along a row)
