@@ -178,13 +178,14 @@ __global__ void __launch_bounds__(kSortThreads) scatter_kernel(const LzrCandidat
         if (key[k] >= 0) order[base[key[k]] + local[k]] = first + k * kSortThreads;
 }
 
-/* Turns the raw end states into the reference records (post-checks, covariance,
- * 2x2 eigen-decomposition), one thread per candidate. */
-__global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, const int *d_count, LzrFit *out,
-                                LaneConfig cfg) {
+/* Turns the raw end states into the reference records (post-checks; for accepted fits J^T J
+ * rebuilt at the final point from the window and its FP64 covariance; 2x2
+ * eigen-decomposition), one thread per candidate. */
+__global__ void epilogue_kernel(const uint16_t *frames, int nx, int ny, const LzrCandidate *cands, int ncands,
+                                const int *d_count, LzrFit *out, LaneConfig cfg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= live_count(d_count, ncands)) return;
-    finish_any(out[i], cfg, cands[i]);
+    finish_any(out[i], cfg, cands[i], frames, nx, ny);
 }
 
 __global__ void reset_counters(unsigned *counters, int n) {
@@ -448,75 +449,48 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
 /* ---- straggler kernel ------------------------------------------------------------
  * Fits handed off after strag_trials trials are the long tail of a batch: one lane per
  * fit would leave the end of the launch to a few lanes running ~200 more passes each.
- * Here a fit is held by a group of k lanes: lane r of the group takes the window's
- * pixels p = r + k m (and, for the separable models, the column and row factors
- * t = r + k m), a butterfly over the group adds the partial sums (every lane of the
- * group ends with the same bits), and all lanes of the group run the same LM logic on
- * the same inputs.  k is the largest power of two <= kMaxGroup for which the handed-off
- * fits need at most two groups' turns on the launch's lanes: a function of the hand-off
- * count, which the fits alone determine, so a fit's result does not depend on which
- * wave or group runs it or on timing.  When many fits are long (dim data), k falls to
- * 1: one lane per fit, with the first kernel's pass.  (500k 9x9 symmetric fits, with the
- * earlier per-pixel pass: 7.90 ms without hand-off, 6.72 ms with 8-lane groups after 32 trials, 6.48 ms
- * with 16-lane groups after 48.) */
+ * Here a fit is held by a group of k lanes that computes exactly the one-lane pass: every
+ * lane evaluates the column factors and the row recurrence itself (a few operations per
+ * column and row), lane r sums the pixels of rows r, r + k, ... (the row's inner sums, in
+ * the one-lane order), and the rows' results are folded in row order by every lane after an
+ * exchange through LDS.  The sums, and so the fit, are bit-identical whatever k is, and
+ * identical to the fit never handed off: a fit's result does not depend on its batch, on
+ * the hand-off count that sets k, or on timing.  All lanes of a group then run the same LM
+ * logic on the same inputs.  k is the largest power of two <= kMaxGroup for which the
+ * handed-off fits need at most two groups' turns on the launch's lanes; with many long fits
+ * (dim data) k falls to 1: one lane per fit, the first kernel's pass. */
 constexpr int kMaxGroup = LZR_STRAGGLER_GROUP;
-static_assert(kMaxGroup == 8 || kMaxGroup == 16, "group sums are written for up to 16 lanes");
+static_assert(kMaxGroup == 8 || kMaxGroup == 16, "group sizes are 1..16 lanes");
 
-/* Sum over the k lanes of a group by DPP (k a power of two <= 16): quad swaps
- * (1 0 3 2) for 2 lanes, (2 3 0 1) for 4, the half-row mirror for 8, the row mirror
- * for 16; FP add is commutative, so every lane of the group holds identical sums. */
-__device__ __forceinline__ double group_sum(double v, int k) {
-    auto step = [](double x, int ctrl) {
-        const long long b = __double_as_longlong(x);
-        int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
-        switch (ctrl) {
-        case 0:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
-            break;
-        case 1:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
-            break;
-        case 2:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xF, 0xF, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xF, 0xF, false);
-            break;
-        default:
-            lo = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xF, 0xF, false);
-            hi = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xF, 0xF, false);
-            break;
-        }
-        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-    };
-    if (k >= 2) v += step(v, 0);
-    if (k >= 4) v += step(v, 1);
-    if (k >= 8) v += step(v, 2);
-    if (k >= 16) v += step(v, 3);
-    return v;
-}
-
-/* Pixels per step of the group pass of the separable models (reads first). */
-constexpr int kGroupChunk = 4;
+/* Row-exchange slots per lane in LDS: a row's factor and inner sums. */
+constexpr int kSepXchg = 6;  /* ey, r1, ru, ru2, rd, rdd */
+constexpr int kEllXchg = 15; /* w, P[5], R[3], W[3], rd, rdd, bad */
+__host__ __device__ constexpr int xchg_slots(int P) { return P == 7 ? kEllXchg : kSepXchg; }
 
 /* The straggler loop for groups of 2^LK lanes. */
 template <int P, typename Acc, int LK>
 __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
                                                  const Parked<P> *park, double *lds_words) {
-    /* LDS: per-group FP64 factors E_x(0..n-1), E_y(0..n-1) (slot t at row t/k, column
-     * k g + t%k; 2 nmax rows), then the windows: pixel p of a group at row p/k, column
-     * k g + p%k */
+    /* LDS: per-lane FP64 column factors [nmax][64], the row exchange [slots][64], then the
+     * windows: lane r of a group holds rows r, r + k, ...: pixel (i, r + k m) at
+     * [(i + n m) * 64 + lane] (for k = 1 the first kernel's layout) */
     const int nmax = 2 * args.cfg.max_half_width + 1;
     double *xcol = lds_words;
-    uint16_t *tile = reinterpret_cast<uint16_t *>(lds_words + 2 * nmax * 64);
+    double *xchg = lds_words + nmax * 64;
+    uint16_t *tile = reinterpret_cast<uint16_t *>(xchg + xchg_slots(P) * 64);
     constexpr int lk = LK, kGroup = 1 << LK, kMask = kGroup - 1;
-    const int lane = threadIdx.x, grp = lane >> lk, r = lane & kMask;
+    const int lane = threadIdx.x, grp = lane >> lk, r = lane & kMask, gbase = grp * kGroup;
 
     LaneState<P> s;
     s.phase = kIdle;
     s.n = 1;
     int model = 0;
     bool open = true;
+    struct {
+        double *c;
+        __device__ void set(int i, double x) { c[i * 64] = x; }
+        __device__ double get(int i) const { return c[i * 64]; }
+    } xb{xcol + lane};
     for (;;) {
         /* ---- refill idle groups (one queue entry per group) ---- */
         bool fresh = false;
@@ -529,8 +503,8 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                 if (lane == leader) base = atomicAdd(head, (unsigned)cnt);
                 base = __shfl(base, leader, 64);
                 if (base + (unsigned)cnt >= count) open = false;
-                if ((want >> (grp * kGroup)) & 1ull) {
-                    const unsigned q = base + (unsigned)__popcll(want & ((1ull << (grp * kGroup)) - 1ull));
+                if ((want >> gbase) & 1ull) {
+                    const unsigned q = base + (unsigned)__popcll(want & ((1ull << gbase) - 1ull));
                     if (q < count) {
                         s = park[q].s;
                         model = park[q].model;
@@ -543,23 +517,23 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
         }
         if (!open && s.phase == kIdle) s.phase = kFinished;
 
-        /* ---- gather the windows of new fits: lane r loads pixels r, r + 8, ... ---- */
+        /* ---- gather the rows r, r + k, ... of new fits' windows, 8 pixels per step ---- */
         if (fresh) {
-            const int n = s.n, n2 = n * n, inv = 65536 / n + 1;
-            const uint16_t *src = args.frames + ((size_t)s.frame * args.ny + s.yoff) * args.nx + s.xoff;
-            for (int m0 = 0; m0 * kGroup < n2; m0 += 8) {
+            const int n = s.n, nr = (n - r + kMask) >> lk; /* rows of this lane */
+            const int cnt = nr * n;
+            const int inv = 65536 / n + 1; /* m = t / n as (t * inv) >> 16, exact for t < n^2, n <= 31 */
+            const uint16_t *src = args.frames + ((size_t)s.frame * args.ny + s.yoff + r) * args.nx + s.xoff;
+            for (int t0 = 0; t0 < cnt; t0 += 8) {
                 uint16_t v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    const int pp = r + kGroup * (m0 + u);
-                    const int j = (pp * inv) >> 16, i = pp - j * n;
-                    v[u] = pp < n2 ? src[(size_t)j * args.nx + i] : (uint16_t)0;
+                    const int t = t0 + u;
+                    const int m = (t * inv) >> 16, i = t - m * n;
+                    v[u] = t < cnt ? src[(size_t)(m * kGroup) * args.nx + i] : (uint16_t)0;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int pp = r + kGroup * (m0 + u);
-                    if (pp < n2) tile[(m0 + u) * 64 + lane] = v[u];
-                }
+                for (int u = 0; u < 8; ++u)
+                    if (t0 + u < cnt) tile[(t0 + u) * 64 + lane] = v[u];
             }
         }
         __syncthreads();
@@ -570,12 +544,13 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
         PassAcc<P, Acc> acc;
         acc.zero();
         const int n = active ? s.n : 1;
-        const int n2 = n * n, inv = 65536 / n + 1;
-        const int mmax = __builtin_amdgcn_readfirstlane(wave_max((n2 + kMask) >> lk));
-        const double *gx = xcol + grp * kGroup;
-        double z[P];
-#pragma unroll
-        for (int k = 0; k < P; ++k) z[k] = 1.0;
+        const int nw = __builtin_amdgcn_readfirstlane(wave_max(n));
+        const int rounds = (nw + kMask) >> lk; /* wave-uniform: the exchange's barriers */
+        struct {
+            const uint16_t *t;
+            int n;
+            __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
+        } pix{tile + lane, n};
         if constexpr (P == 7) {
             EllIn q;
             if (active) {
@@ -586,150 +561,120 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
             }
             double v7[kEllSums];
             if constexpr (kGroup == 1) {
-                const int nw = __builtin_amdgcn_readfirstlane(wave_max(n));
-                struct {
-                    const uint16_t *t;
-                    int n;
-                    __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
-                } pix{tile + lane, n};
-                struct {
-                    double *c;
-                    __device__ void set(int i, double x) { c[i * 64] = x; }
-                    __device__ double get(int i) const { return c[i * 64]; }
-                } xb{xcol + lane};
                 ell_pass(q, n, nw, __any(active && n != nw), pix, xb, v7);
             } else {
-                /* moment pass split over the group: lane r computes the column factors of
-                 * columns r, r + k, ... and the rows r, r + k, ... of the window; the
-                 * butterfly adds the moments */
-#pragma unroll
-                for (int k = 0; k < kEllSums; ++k) v7[k] = 0.0;
-                for (int t = r; t < n; t += kGroup) {
-                    const double u = (double)t - q.x1;
-                    xcol[(t >> lk) * 64 + lane] = dexp(-(q.qa * u) * u);
-                }
-                __syncthreads();
-                const bool fac = ell_factored(q, n);
-                for (int j = r; j < n; j += kGroup) {
-                    const double vv = (double)j - q.y1;
-                    double w = 1.0, g = 1.0, c = 1.0;
-                    if (fac) {
-                        w = dexp(-(q.qb * vv) * vv);
-                        g = dexp(-2.0 * q.qc * vv);
-                        c = dexp(2.0 * q.qc * q.x1 * vv);
-                    }
-                    double Pm[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Rm[3] = {0.0, 0.0, 0.0}, Wm[3] = {0.0, 0.0, 0.0};
-                    double rd = 0.0, rdd = 0.0;
-                    /* kSepChunk pixels per step, reads first (clamped to the row) */
-                    for (int i0 = 0; i0 < n; i0 += kSepChunk) {
-                        double Dk[kSepChunk], Xk[kSepChunk];
-#pragma unroll
-                        for (int k = 0; k < kSepChunk; ++k) {
-                            const int i = i0 + k < n ? i0 + k : n - 1;
-                            const int p = i + n * j;
-                            Dk[k] = (double)tile[(p >> lk) * 64 + grp * kGroup + (p & kMask)];
-                            Xk[k] = gx[(i >> lk) * 64 + (i & kMask)];
-                        }
-#pragma unroll
-                        for (int k = 0; k < kSepChunk; ++k) {
-                            const int i = i0 + k;
-                            if (i >= n) break;
-                            const double u = (double)i - q.x1;
-                            double e;
-                            if (fac) {
-                                e = Xk[k] * c;
-                                c = c * g;
-                            } else {
-                                e = dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * vv + (q.qb * vv) * vv));
+                EllRows rw;
+                ell_columns(q, n, n, xb, v7, rw);
+                for (int m = 0; m < rounds; ++m) {
+                    /* the round's rows: every lane steps the row recurrences; the owner keeps its
+                     * row's weight and cross factors and sums the row */
+                    double wo = 0.0, go = 1.0, co = 1.0;
+                    for (int rr = 0; rr < kGroup; ++rr) {
+                        const int j = m * kGroup + rr;
+                        if (j < n) {
+                            double w, g, c;
+                            ell_row_factor(q, rw, j, w, g, c);
+                            if (rr == r) {
+                                wo = w;
+                                go = g;
+                                co = c;
                             }
-                            ell_pixel(e, u, Dk[k], Pm, Rm, Wm, rd, rdd);
                         }
                     }
-                    ell_row(w, vv, Pm, Rm, Wm, rd, rdd, v7);
-                }
+                    const int jo = m * kGroup + r;
+                    EllRow ro{{0.0, 0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
+                    if (jo < n) {
+                        struct {
+                            const uint16_t *t;
+                            __device__ unsigned raw(int i, int) const { return t[i * 64]; }
+                        } prow{tile + (m * n) * 64 + lane};
+                        ro = ell_row_sums<false>(q, n, n, rw.fac, true, go, co, prow, xb, jo);
+                    }
+                    double *x = xchg + lane;
+                    x[0] = wo;
 #pragma unroll
-                for (int k = 0; k < kEllSums; ++k) v7[k] = group_sum(v7[k], kGroup);
+                    for (int a = 0; a < 5; ++a) x[(1 + a) * 64] = ro.P[a];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        x[(6 + a) * 64] = ro.R[a];
+                        x[(9 + a) * 64] = ro.W[a];
+                    }
+                    x[12 * 64] = ro.rd;
+                    x[13 * 64] = ro.rdd;
+                    x[14 * 64] = ro.bad;
+                    __syncthreads();
+                    for (int rr = 0; rr < kGroup; ++rr) {
+                        const int j = m * kGroup + rr;
+                        if (j < n) {
+                            const double *y = xchg + gbase + rr;
+                            double Pm[5], Rm[3], Wm[3];
+#pragma unroll
+                            for (int a = 0; a < 5; ++a) Pm[a] = y[(1 + a) * 64];
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                Rm[a] = y[(6 + a) * 64];
+                                Wm[a] = y[(9 + a) * 64];
+                            }
+                            ell_row(y[0], (double)j - q.y1, Pm, Rm, Wm, y[12 * 64], y[13 * 64], y[14 * 64], v7);
+                        }
+                    }
+                    __syncthreads();
+                }
             }
-            ell_finish<Acc>(q, n2, v7, s.phase == kInit, s.fnorm * s.fnorm, acc);
+            ell_finish<Acc>(q, n * n, v7, s.phase == kInit, s.fnorm * s.fnorm, acc);
         } else {
-            /* separable pass split over the group: lane r computes the factors of slots
-             * t = r, r + k, ... (t < n: column t, else row t - n) with their 1-D moments and
-             * the data sums of pixels r, r + k, ...; the butterfly adds the partial sums */
             SepIn q;
+            double z[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) z[k] = 1.0;
             if (active)
                 lane_sep_in<P>(s, args.cfg, q);
             else
                 make_sep_in(z, z, 1.0, 1.0, q);
             double v[kSepSums];
             if constexpr (kGroup == 1) {
-                /* one lane per fit: the first kernel's pass (factor recurrence, row sums) over
-                 * the same tile and column layout */
-                const int nw = __builtin_amdgcn_readfirstlane(wave_max(n));
-                const bool mixed = __any(active && n != nw);
-                struct {
-                    const uint16_t *t;
-                    int n;
-                    __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
-                } pix{tile + lane, n};
-                struct {
-                    double *c;
-                    __device__ void set(int i, double x) { c[i * 64] = x; }
-                    __device__ double get(int i) const { return c[i * 64]; }
-                } xb{xcol + lane};
-                sep_pass(q, n, nw, mixed, pix, xb, v);
+                /* one lane per fit: the first kernel's pass over the same layout */
+                sep_pass(q, n, nw, __any(active && n != nw), pix, xb, v);
             } else {
-#pragma unroll
-                for (int k = 0; k < kSepSums; ++k) v[k] = 0.0;
-                for (int t = r; t < 2 * n; t += kGroup) {
-                    const bool isx = t < n;
-                    const double u = (double)(isx ? t : t - n) - (isx ? q.x1 : q.y1);
-                    const double e = dexp(-(q.c1 * u) * u);
-                    xcol[(t >> lk) * 64 + lane] = e;
-                    double m8[8];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) m8[k] = 0.0;
-                    axis_moments(e, u, m8);
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        v[kMx0 + k] += isx ? m8[k] : 0.0;
-                        v[kMy0 + k] += isx ? 0.0 : m8[k];
+                GaussRec gy;
+                sep_columns(q, n, n, xb, v, gy);
+                for (int m = 0; m < rounds; ++m) {
+                    double eo = 0.0;
+                    for (int rr = 0; rr < kGroup; ++rr) {
+                        const int j = m * kGroup + rr;
+                        if (j < n) {
+                            const double ey = sep_row_factor(q, gy, j, n, v);
+                            if (rr == r) eo = ey;
+                        }
                     }
+                    const int jo = m * kGroup + r;
+                    SepRow ro{0.0, 0.0, 0.0, 0.0, 0.0};
+                    if (jo < n) {
+                        struct {
+                            const uint16_t *t;
+                            __device__ unsigned raw(int i, int) const { return t[i * 64]; }
+                        } prow{tile + (m * n) * 64 + lane};
+                        ro = sep_row<false>(q, n, n, true, prow, xb, jo);
+                    }
+                    double *x = xchg + lane;
+                    x[0] = eo;
+                    x[64] = ro.r1;
+                    x[128] = ro.ru;
+                    x[192] = ro.ru2;
+                    x[256] = ro.rd;
+                    x[320] = ro.rdd;
+                    __syncthreads();
+                    for (int rr = 0; rr < kGroup; ++rr) {
+                        const int j = m * kGroup + rr;
+                        if (j < n) {
+                            const double *y = xchg + gbase + rr;
+                            row_sums(y[0], (double)j - q.y1, y[64], y[128], y[192], y[256], y[320], v);
+                        }
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-                /* kGroupChunk pixels per step: their tile and factor reads are issued before
-                 * the arithmetic.  A lane's slots past its window (pp >= n2) read pixel 0 and
-                 * add +0, which leaves every sum unchanged. */
-                for (int m0 = 0; m0 < mmax; m0 += kGroupChunk) {
-                    unsigned Dr[kGroupChunk];
-                    double ex[kGroupChunk], ey[kGroupChunk];
-                    int ii[kGroupChunk], jj[kGroupChunk];
-                    bool ok[kGroupChunk];
-#pragma unroll
-                    for (int k = 0; k < kGroupChunk; ++k) {
-                        const int m = m0 + k < mmax ? m0 + k : mmax - 1;
-                        const int pp0 = r + kGroup * (m0 + k);
-                        ok[k] = m0 + k < mmax && pp0 < n2;
-                        const int pp = ok[k] ? pp0 : 0;
-                        const int j = (pp * inv) >> 16, i = pp - j * n;
-                        const int ty = n + j;
-                        ii[k] = i;
-                        jj[k] = j;
-                        Dr[k] = tile[m * 64 + lane];
-                        ex[k] = gx[(i >> lk) * 64 + (i & kMask)];
-                        ey[k] = gx[(ty >> lk) * 64 + (ty & kMask)];
-                    }
-#pragma unroll
-                    for (int k = 0; k < kGroupChunk; ++k) {
-                        if (m0 + k >= mmax) break;
-                        const double e = ok[k] ? ex[k] * ey[k] : 0.0;
-                        const double D = ok[k] ? (double)Dr[k] : 0.0;
-                        sep_pixel(e, (double)ii[k] - q.x1, (double)jj[k] - q.y1, D, v);
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < kSepSums; ++k) v[k] = group_sum(v[k], kGroup);
             }
-            sep_finish<P, Acc>(q, n2, v, s.phase == kInit, s.fnorm * s.fnorm, acc);
+            sep_finish<P, Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, acc);
         }
 
         if (active) {
@@ -884,8 +829,7 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
         hipLaunchKernelGGL((lm_persistent<P, Acc, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
     }
     if (a.strag_trials > 0) {
-        const int nmax = 2 * ctx->cfg.max_half_width + 1;
-        const size_t slds = lds + (size_t)nmax * 64 * sizeof(double); /* 2 nmax factor rows */
+        const size_t slds = lds + (size_t)xchg_slots(P) * 64 * sizeof(double); /* + the row exchange */
         constexpr int kRole = P == 7 ? kAllGroups : kMultiLane;
         int per_cu = 0;
         LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1, kRole>, 64, slds));
@@ -989,8 +933,8 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
         LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
         LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
     }
-    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
-                       ctx->cfg);
+    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, frames, nx, ny, cands, ncands,
+                       d_count, out, ctx->cfg);
     LZR_CHECK(hipGetLastError());
     LZR_CHECK(ctx->order.end(stream));
 }

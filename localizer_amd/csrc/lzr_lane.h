@@ -81,16 +81,18 @@ struct NP {
 
 /* Raw end state of one fit, written by the fitting kernel into the caller's LzrFit
  * slot (same size and layout of the trailing fields) and turned into the reference
- * record by the epilogue (finish_record).  th is in absolute coordinates. */
+ * record by the epilogue (finish_record).  th is in window coordinates (x0, y0 relative to
+ * the window's corner); the epilogue rebuilds J^T J at th from the window. */
 struct LzrRawFit {
     double th[LZR_MAX_PARAMS];
-    float H[28]; /* J^T J at th, packed (P(P+1)/2 used) */
+    double unused[2 * LZR_MAX_PARAMS];
     double chi;
     int32_t status; /* kPending */
     int32_t lm_status;
     int16_t iterations, trials, accepted_steps, model;
 };
 static_assert(sizeof(LzrRawFit) == sizeof(LzrFit), "raw record must overlay LzrFit");
+static_assert(__builtin_offsetof(LzrRawFit, status) == __builtin_offsetof(LzrFit, status), "status overlays status");
 constexpr int32_t kPending = -1;
 
 /* Per-lane LM state.  x0/y0 are kept window-local (exact integer offsets xoff/yoff);
@@ -101,8 +103,8 @@ struct LaneState {
     double th[P];   /* current parameters                           */
     double dx[P];   /* last step (GSL s->dx)                        */
     double diag[P]; /* column scaling D                             */
-    float H[NP<P>::S]; /* J^T J at th (packed)                      */
-    float g[P];        /* J^T f at th                               */
+    double H[NP<P>::S]; /* J^T J at th (packed)                     */
+    double g[P];        /* J^T f at th                              */
     double delta, par, fnorm, xnorm;
     double pnorm;   /* |D dx| of the proposed step (lmpar's dxnorm) */
     int xoff, yoff, n, frame;
@@ -124,7 +126,9 @@ struct PassAcc {
     Acc sf;            /* sum f1                 */
     Acc sse0;          /* sum f0^2               */
     Acc dsse;          /* sum (f1-f0)(f1+f0)     */
+    int bad;           /* residual vector at the pass point: 0 finite, 1 one +-inf entry, 2 NaN norm */
     LZR_HD LZR_INLINE void zero() {
+        bad = 0;
 #pragma unroll
         for (int i = 0; i < NP<P>::SQ; ++i) hq[i] = Acc(0);
 #pragma unroll
@@ -183,6 +187,7 @@ LZR_HD LZR_INLINE double dsqrt(double v) {
  * (r <- r + r (1/2 - v r^2 / 2)), within ~1 ulp; one quarter-rate instruction and six
  * FMAs instead of a correctly rounded square root followed by a division. */
 LZR_HD LZR_INLINE double drsqrt(double v) {
+    LZR_NO_CONTRACT
 #if defined(__HIP_DEVICE_COMPILE__)
     double r = __builtin_amdgcn_rsq(v);
     const double h = 0.5 * v;
@@ -201,6 +206,7 @@ LZR_HD LZR_INLINE double drsqrt(double v) {
  * LM logic's divisions, where a last-bit difference from a correctly rounded quotient
  * is far below every tolerance the reference tests. */
 LZR_HD LZR_INLINE double drcp(double v) {
+    LZR_NO_CONTRACT
 #if defined(__HIP_DEVICE_COMPILE__)
     double r = __builtin_amdgcn_rcp(v);
     if (__builtin_isfinite(r) && r != 0.0) {
@@ -214,7 +220,10 @@ LZR_HD LZR_INLINE double drcp(double v) {
     return 1.0 / v;
 #endif
 }
-LZR_HD LZR_INLINE double ddiv(double a, double b) { return a * drcp(b); }
+LZR_HD LZR_INLINE double ddiv(double a, double b) {
+    LZR_NO_CONTRACT
+    return a * drcp(b);
+}
 LZR_HD LZR_INLINE double dabs(double v) { return v < 0 ? -v : v; }
 LZR_HD LZR_INLINE double dmax(double a, double b) { return a > b ? a : b; }
 LZR_HD LZR_INLINE double dmin(double a, double b) { return a < b ? a : b; }
@@ -222,11 +231,12 @@ LZR_HD LZR_INLINE double dmin(double a, double b) { return a < b ? a : b; }
 /* Cholesky M = L L^T (packed); false on a non-positive pivot. */
 template <int P>
 LZR_HD LZR_INLINE bool cholesky(const double (&M)[NP<P>::S], double (&L)[NP<P>::S], double (&inv_diag)[P]) {
+    LZR_NO_CONTRACT
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         double s = M[tri(j, j)];
 #pragma unroll
-        for (int k = 0; k < j; ++k) s -= L[tri(j, k)] * L[tri(j, k)];
+        for (int k = 0; k < j; ++k) s = dfma(-L[tri(j, k)], L[tri(j, k)], s);
         if (!(s > 0.0)) return false;
         const double inv = drsqrt(s);
         L[tri(j, j)] = s * inv;
@@ -235,7 +245,7 @@ LZR_HD LZR_INLINE bool cholesky(const double (&M)[NP<P>::S], double (&L)[NP<P>::
         for (int i = j + 1; i < P; ++i) {
             double t = M[tri(i, j)];
 #pragma unroll
-            for (int k = 0; k < j; ++k) t -= L[tri(i, k)] * L[tri(j, k)];
+            for (int k = 0; k < j; ++k) t = dfma(-L[tri(i, k)], L[tri(j, k)], t);
             L[tri(i, j)] = t * inv;
         }
     }
@@ -246,11 +256,12 @@ LZR_HD LZR_INLINE bool cholesky(const double (&M)[NP<P>::S], double (&L)[NP<P>::
 template <int P>
 LZR_HD LZR_INLINE void forward(const double (&L)[NP<P>::S], const double (&inv_diag)[P], const double (&b)[P],
                                double (&y)[P]) {
+    LZR_NO_CONTRACT
 #pragma unroll
     for (int i = 0; i < P; ++i) {
         double s = b[i];
 #pragma unroll
-        for (int k = 0; k < i; ++k) s -= L[tri(i, k)] * y[k];
+        for (int k = 0; k < i; ++k) s = dfma(-L[tri(i, k)], y[k], s);
         y[i] = s * inv_diag[i];
     }
 }
@@ -259,33 +270,40 @@ LZR_HD LZR_INLINE void forward(const double (&L)[NP<P>::S], const double (&inv_d
 template <int P>
 LZR_HD LZR_INLINE void backward(const double (&L)[NP<P>::S], const double (&inv_diag)[P], const double (&y)[P],
                                 double (&x)[P]) {
+    LZR_NO_CONTRACT
 #pragma unroll
     for (int i = P - 1; i >= 0; --i) {
         double s = y[i];
 #pragma unroll
-        for (int k = i + 1; k < P; ++k) s -= L[tri(k, i)] * x[k];
+        for (int k = i + 1; k < P; ++k) s = dfma(-L[tri(k, i)], x[k], s);
         x[i] = s * inv_diag[i];
     }
 }
 
 template <int P>
 LZR_HD LZR_INLINE double scaled_norm(const double (&d)[P], const double (&v)[P]) {
+    LZR_NO_CONTRACT
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < P; ++i) s += (d[i] * v[i]) * (d[i] * v[i]);
+    for (int i = 0; i < P; ++i) {
+        const double t = d[i] * v[i];
+        s = dfma(t, t, s);
+    }
     return dsqrt(s);
 }
 
 template <int P>
 LZR_HD LZR_INLINE double plain_norm2(const double (&v)[P]) {
+    LZR_NO_CONTRACT
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < P; ++i) s += v[i] * v[i];
+    for (int i = 0; i < P; ++i) s = dfma(v[i], v[i], s);
     return s;
 }
 
 template <int P>
 LZR_HD LZR_INLINE double plain_norm(const double (&v)[P]) {
+    LZR_NO_CONTRACT
     return dsqrt(plain_norm2<P>(v));
 }
 
@@ -294,6 +312,7 @@ LZR_HD LZR_INLINE double plain_norm(const double (&v)[P]) {
 template <int P>
 LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], const double (&diag)[P],
                              double delta, double &par_io, double (&x)[P], double &dxnorm_out) {
+    LZR_NO_CONTRACT
     constexpr int S = NP<P>::S;
     double L[S], id[P], y[P], u[P];
     /* Gauss-Newton step.  A Cholesky breakdown is MINPACK's rank-deficient case: the
@@ -344,7 +363,7 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
 #pragma unroll
         for (int i = 0; i < S; ++i) M[i] = H[i];
 #pragma unroll
-        for (int i = 0; i < P; ++i) M[tri(i, i)] += par * diag[i] * diag[i];
+        for (int i = 0; i < P; ++i) M[tri(i, i)] = dfma(par * diag[i], diag[i], M[tri(i, i)]);
         if (!cholesky<P>(M, L, id)) {
             /* (H + par D^2) lost definiteness in rounding: keep the last x */
             break;
@@ -383,6 +402,7 @@ LZR_HD LZR_INLINE int x_index() { return P == 4 ? 1 : (P == 5 ? 2 : 3); }
 /* Absolute-coordinate parameter vector (for test_delta and |D x|). */
 template <int P>
 LZR_HD LZR_INLINE void absolute(const LaneState<P> &s, const double (&t)[P], double (&a)[P]) {
+    LZR_NO_CONTRACT
 #pragma unroll
     for (int i = 0; i < P; ++i) a[i] = t[i];
     a[x_index<P>()] += (double)s.xoff;
@@ -390,6 +410,7 @@ LZR_HD LZR_INLINE void absolute(const LaneState<P> &s, const double (&t)[P], dou
 }
 
 LZR_HD LZR_INLINE void quad_coeffs(double sx, double sy, double rho, double &qa, double &qc, double &qb) {
+    LZR_NO_CONTRACT
     const double pre = 0.5 * drcp(1.0 - rho * rho);
     const double isx = drcp(sx), isy = drcp(sy);
     qa = pre * isx * isx;
@@ -432,6 +453,7 @@ struct SepIn {
  * d f/d r = A e d^2 / (sqrt(2) r^3 sigma), d f/d x0 = A e (x - x0) / (r^2 sigma)
  * (source/Localization.cpp:1151-1166); fixed width: (A, x0, y0, b) with r = w. */
 LZR_HD LZR_INLINE void make_sep_in(const double (&t0)[5], const double (&dx)[5], double is, double, SepIn &q) {
+    LZR_NO_CONTRACT
     const double ir1 = drcp(t0[1] + dx[1]);
     q.A1 = t0[0] + dx[0];
     q.x1 = t0[2] + dx[2];
@@ -443,6 +465,7 @@ LZR_HD LZR_INLINE void make_sep_in(const double (&t0)[5], const double (&dx)[5],
     q.kx = ir1 * ir1 * is;
 }
 LZR_HD LZR_INLINE void make_sep_in(const double (&t0)[4], const double (&dx)[4], double is, double w, SepIn &q) {
+    LZR_NO_CONTRACT
     const double iw = drcp(w);
     q.A1 = t0[0] + dx[0];
     q.x1 = t0[1] + dx[1];
@@ -527,14 +550,19 @@ LZR_HD LZR_INLINE void sep_row_pixel(double D, double ex, double u, double &r1, 
     rdd = dfma(D, D, rdd);
 }
 
+/* Inner sums of one pixel row of a separable pass: sum_i Ex D, sum_i Ex u D, sum_i Ex u^2 D,
+ * sum_i D, sum_i D^2. */
+struct SepRow {
+    double r1, ru, ru2, rd, rdd;
+};
+
 /* kSepChunk pixels of row j from column i0: all their reads are issued before their
  * arithmetic, so one LDS latency covers the chunk (full chunks read at fixed offsets from the
  * chunk's first pixel).  Tail: reads past nloop are clamped in bounds and skipped (nloop is
  * uniform over the wave).  Guard: pixels beyond the lane's own n x n window are masked. */
 template <bool Tail, bool Guard, typename Pix, typename XBuf>
 LZR_HD LZR_INLINE void sep_chunk(const SepIn &q, int n, int nloop, bool row_in, const Pix &pix,
-                                 const XBuf &xb, int i0, int j, double &r1, double &ru, double &ru2, double &rd,
-                                 double &rdd) {
+                                 const XBuf &xb, int i0, int j, SepRow &r) {
     LZR_NO_CONTRACT
     decltype(pix.raw(0, 0)) Dc[kSepChunk];
     double Xc[kSepChunk];
@@ -549,21 +577,19 @@ LZR_HD LZR_INLINE void sep_chunk(const SepIn &q, int n, int nloop, bool row_in, 
         const int i = i0 + k;
         if (Tail && i >= nloop) break;
         const double D = !Guard || (row_in && i < n) ? (double)Dc[k] : 0.0;
-        sep_row_pixel(D, Xc[k], (double)i - q.x1, r1, ru, ru2, rd, rdd);
+        sep_row_pixel(D, Xc[k], (double)i - q.x1, r.r1, r.ru, r.ru2, r.rd, r.rdd);
     }
 }
 
-/* The separable pass of one lane over its n x n window (nloop >= n: the loop bound shared
- * by the wave; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
- * returns pixel (i, j) as stored, xb holds one FP64 value per column. */
-template <typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
-                                double (&v)[kSepSums]) {
+/* Column factors Ex(i) (into xb) and their moments, the first step of a separable pass over
+ * columns 0..nloop-1 (factors beyond n are 0); gy is set up for the rows. */
+template <typename XBuf>
+LZR_HD LZR_INLINE void sep_columns(const SepIn &q, int n, int nloop, XBuf &xb, double (&v)[kSepSums], GaussRec &gy) {
     LZR_NO_CONTRACT
 #pragma unroll
     for (int k = 0; k < kSepSums; ++k) v[k] = 0.0;
     const double qexp = dexp(-2.0 * q.c1);
-    GaussRec gx, gy;
+    GaussRec gx;
     gx.start(q.c1, q.x1, n, qexp);
     gy.start(q.c1, q.y1, n, qexp);
     for (int i = 0; i < nloop; ++i) {
@@ -574,25 +600,49 @@ LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, co
         xb.set(i, e);
         axis_moments(e, u, v + kMx0);
     }
+}
+
+/* Row factor Ey(j) (the next step of the row recurrence; 0 beyond n), added to the row
+ * moments.  Rows must be taken in order j = 0, 1, ... */
+LZR_HD LZR_INLINE double sep_row_factor(const SepIn &q, GaussRec &gy, int j, int n, double (&v)[kSepSums]) {
+    LZR_NO_CONTRACT
+    const double w = (double)j - q.y1;
+    double ey = gy.next();
+    if (!gy.ok) ey = dexp(-(q.c1 * w) * w);
+    ey = j < n ? ey : 0.0;
+    axis_moments(ey, w, v + kMy0);
+    return ey;
+}
+
+/* Inner sums of row j over columns 0..nloop-1 (Guard: pixels outside the lane's own window
+ * masked; row_in false masks the whole row). */
+template <bool Guard, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE SepRow sep_row(const SepIn &q, int n, int nloop, bool row_in, const Pix &pix, const XBuf &xb,
+                                 int j) {
+    SepRow r{0.0, 0.0, 0.0, 0.0, 0.0};
+    int i0 = 0;
+    for (; i0 + kSepChunk <= nloop; i0 += kSepChunk) sep_chunk<false, Guard>(q, n, nloop, row_in, pix, xb, i0, j, r);
+    if (i0 < nloop) sep_chunk<true, Guard>(q, n, nloop, row_in, pix, xb, i0, j, r);
+    return r;
+}
+
+/* The separable pass of one lane over its n x n window (nloop >= n: the loop bound shared
+ * by the wave; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
+ * returns pixel (i, j) as stored, xb holds one FP64 value per column.  The pieces (columns,
+ * row factor, row inner sums, row fold) are also what a group of lanes sharing one fit runs:
+ * the same operations in the same order, so a fit's sums do not depend on who computes them. */
+template <typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
+                                double (&v)[kSepSums]) {
+    LZR_NO_CONTRACT
+    GaussRec gy;
+    sep_columns(q, n, nloop, xb, v, gy);
     for (int j = 0; j < nloop; ++j) {
-        const double w = (double)j - q.y1;
-        double ey = gy.next();
-        if (!gy.ok) ey = dexp(-(q.c1 * w) * w);
-        ey = j < n ? ey : 0.0;
-        axis_moments(ey, w, v + kMy0);
-        double r1 = 0.0, ru = 0.0, ru2 = 0.0, rd = 0.0, rdd = 0.0;
+        const double ey = sep_row_factor(q, gy, j, n, v);
         const bool row_in = !guard || j < n;
-        int i0 = 0;
-        if (guard) {
-            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
-                sep_chunk<false, true>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
-            if (i0 < nloop) sep_chunk<true, true>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
-        } else {
-            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
-                sep_chunk<false, false>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
-            if (i0 < nloop) sep_chunk<true, false>(q, n, nloop, row_in, pix, xb, i0, j, r1, ru, ru2, rd, rdd);
-        }
-        row_sums(ey, w, r1, ru, ru2, rd, rdd, v);
+        const SepRow r = guard ? sep_row<true>(q, n, nloop, row_in, pix, xb, j)
+                               : sep_row<false>(q, n, nloop, row_in, pix, xb, j);
+        row_sums(ey, (double)j - q.y1, r.r1, r.ru, r.ru2, r.rd, r.rdd, v);
     }
 }
 
@@ -664,7 +714,8 @@ enum EllSum : int {
     kEU = 21,    /* U_ab: 6 slots */
     kESD = 27,
     kESDD = 28,
-    kEllSums = 29
+    kEBad = 29, /* pixels whose residual is +-inf (1 each) or NaN (2 each), see ell_chunk */
+    kEllSums = 30
 };
 LZR_HD constexpr int ell_s(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
 LZR_HD constexpr int ell_t(int a, int b) { return (a + b) * (a + b + 1) / 2 + b; }
@@ -679,6 +730,7 @@ struct EllIn {
 /* Trial point t0 + dx of (A, sx, sy, x0, y0, rho, b); columns with den = 2(1-rho^2) sigma,
  * k = 2/(den ...) = 1/(omr sigma ...) as in source/Localization.cpp:1275-1280. */
 LZR_HD LZR_INLINE void make_ell_in(const double (&t0)[7], const double (&dx)[7], double is, EllIn &q) {
+    LZR_NO_CONTRACT
     const double sx = t0[1] + dx[1], sy = t0[2] + dx[2], rho = t0[5] + dx[5];
     quad_coeffs(sx, sy, rho, q.qa, q.qc, q.qb);
     q.A1 = t0[0] + dx[0];
@@ -703,6 +755,7 @@ LZR_HD LZR_INLINE void make_ell_in(const double (&t0)[7], const double (&dx)[7],
 
 /* True when the three factors Ex, Ey, C stay within exp(+-600) over an n x n window. */
 LZR_HD LZR_INLINE bool ell_factored(const EllIn &q, int n) {
+    LZR_NO_CONTRACT
     const double um = dmax(dabs(q.x1), dabs((double)(n - 1) - q.x1));
     const double vm = dmax(dabs(q.y1), dabs((double)(n - 1) - q.y1));
     return q.qa >= 0.0 && q.qb >= 0.0 && q.qa * um * um <= 600.0 && q.qb * vm * vm <= 600.0 &&
@@ -712,7 +765,7 @@ LZR_HD LZR_INLINE bool ell_factored(const EllIn &q, int n) {
 /* Row-end accumulation: inner sums P_a = sum e'^2 u^a (a <= 4), R_a = sum e' u^a,
  * W_a = sum e' D u^a (a <= 2) of row v with row weight w (e1 = w e'). */
 LZR_HD LZR_INLINE void ell_row(double w, double v, const double (&P)[5], const double (&R)[3], const double (&W)[3],
-                               double rd, double rdd, double (&s)[kEllSums]) {
+                               double rd, double rdd, double bad, double (&s)[kEllSums]) {
     LZR_NO_CONTRACT
     const double w2 = w * w;
     double vb[5];
@@ -732,6 +785,7 @@ LZR_HD LZR_INLINE void ell_row(double w, double v, const double (&P)[5], const d
         }
     s[kESD] += rd;
     s[kESDD] += rdd;
+    s[kEBad] += bad;
 }
 
 /* Inner sums of one pixel with factor e (e1 / row weight), offset u and data D. */
@@ -756,12 +810,23 @@ LZR_HD LZR_INLINE void ell_pixel(double e, double u, double D, double (&P)[5], d
     rdd = dfma(D, D, rdd);
 }
 
+/* Inner sums of one pixel row of the ellipsoidal pass (factors e' = e1 / row weight):
+ * P_a = sum e'^2 u^a, R_a = sum e' u^a, W_a = sum e' D u^a, sum D, sum D^2, and the count of
+ * pixels whose residual is not finite. */
+struct EllRow {
+    double P[5], R[3], W[3], rd, rdd, bad;
+};
+
 /* kSepChunk pixels of row j (offset v) from column i0 for the ellipsoidal pass, reads first
- * (Tail and Guard as for sep_chunk); c is the row's cross factor, advanced by g per column. */
+ * (Tail and Guard as for sep_chunk); c is the row's cross factor, advanced by g per column.
+ * Per-pixel exponentials (fac false) can overflow where |rho| >= 1: the reference's residual
+ * vector then holds +-inf (A e overflows) or NaN entries, and GSL's scaled 2-norm is +inf for
+ * one infinite entry and NaN for two or more (or any NaN), which the LM step treats
+ * differently; `bad` counts such pixels (1 per infinite, 2 per NaN residual).  In the
+ * factored path every factor stays within exp(+-600) and the count is 0. */
 template <bool Tail, bool Guard, typename Pix, typename XBuf>
 LZR_HD LZR_INLINE void ell_chunk(const EllIn &q, int n, int nloop, bool fac, bool row_in, double v, double g,
-                                 double &c, const Pix &pix, const XBuf &xb, int i0, int j, double (&P)[5],
-                                 double (&R)[3], double (&W)[3], double &rd, double &rdd) {
+                                 double &c, const Pix &pix, const XBuf &xb, int i0, int j, EllRow &r) {
     LZR_NO_CONTRACT
     decltype(pix.raw(0, 0)) Dc[kSepChunk];
     double Xc[kSepChunk];
@@ -787,24 +852,32 @@ LZR_HD LZR_INLINE void ell_chunk(const EllIn &q, int n, int nloop, bool fac, boo
             e = in ? ef : 0.0;
         } else {
             e = in ? dexp(-((q.qa * u) * u + (2.0 * q.qc * u) * v + (q.qb * v) * v)) : 0.0;
+            const double t = q.A1 * e;
+            r.bad += t - t != 0.0 ? (t == t ? 1.0 : 2.0) : 0.0; /* +-inf: inf - inf is NaN */
         }
-        ell_pixel(e, u, D, P, R, W, rd, rdd);
+        ell_pixel(e, u, D, r.P, r.R, r.W, r.rd, r.rdd);
     }
 }
 
-/* The ellipsoidal moment pass of one lane over its n x n window (nloop >= n: the wave's
- * loop bound; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
- * returns pixel (i, j) as stored, xb holds one FP64 value per column. */
-template <typename Pix, typename XBuf>
-LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
-                                double (&s)[kEllSums]) {
+/* Row recurrences of the ellipsoidal pass: Ey(j) and, for the factored form, the cross
+ * factor's row start C(0, j) = exp(2 qc x1 v_j) and step g_j = exp(-2 qc v_j). */
+struct EllRows {
+    GaussRec gy;
+    double gj, cj, gr, cr;
+    bool fac;
+};
+
+/* Column factors Ex(i) (into xb) of the ellipsoidal pass over columns 0..nloop-1 and the set-up
+ * of the row recurrences. */
+template <typename XBuf>
+LZR_HD LZR_INLINE void ell_columns(const EllIn &q, int n, int nloop, XBuf &xb, double (&s)[kEllSums], EllRows &rw) {
     LZR_NO_CONTRACT
 #pragma unroll
     for (int k = 0; k < kEllSums; ++k) s[k] = 0.0;
-    const bool fac = ell_factored(q, n);
-    GaussRec gx, gy;
+    rw.fac = ell_factored(q, n);
+    GaussRec gx;
     gx.start(q.qa, q.x1, n, dexp(-2.0 * q.qa));
-    gy.start(q.qb, q.y1, n, dexp(-2.0 * q.qb));
+    rw.gy.start(q.qb, q.y1, n, dexp(-2.0 * q.qb));
     for (int i = 0; i < nloop; ++i) {
         const double u = (double)i - q.x1;
         double e = gx.next();
@@ -813,40 +886,64 @@ LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, bool guard, co
     }
     /* row quantities by recurrence over rows: g_j = C(i + 1, j) / C(i, j) = exp(-2 qc v_j),
      * c_j = C(0, j) = exp(2 qc x1 v_j); within the window both stay inside exp(+-600) */
-    double gj = 1.0, cj = 1.0, gr = 1.0, cr = 1.0;
-    if (fac) {
-        gj = dexp(2.0 * q.qc * q.y1);
-        cj = dexp(-2.0 * q.qc * q.x1 * q.y1);
-        gr = dexp(-2.0 * q.qc);
-        cr = dexp(2.0 * q.qc * q.x1);
+    rw.gj = rw.cj = rw.gr = rw.cr = 1.0;
+    if (rw.fac) {
+        rw.gj = dexp(2.0 * q.qc * q.y1);
+        rw.cj = dexp(-2.0 * q.qc * q.x1 * q.y1);
+        rw.gr = dexp(-2.0 * q.qc);
+        rw.cr = dexp(2.0 * q.qc * q.x1);
     }
+}
+
+/* The next row's weight w (= Ey(j) in the factored form, else 1), cross-factor step g and start
+ * c.  Rows must be taken in order j = 0, 1, ... */
+LZR_HD LZR_INLINE void ell_row_factor(const EllIn &q, EllRows &rw, int j, double &w, double &g, double &c) {
+    LZR_NO_CONTRACT
+    const double v = (double)j - q.y1;
+    double ey = rw.gy.next();
+    if (!rw.gy.ok) ey = dexp(-(q.qb * v) * v);
+    w = 1.0;
+    g = 1.0;
+    c = 1.0;
+    if (rw.fac) {
+        w = ey;
+        g = rw.gj;
+        c = rw.cj;
+        rw.gj = rw.gj * rw.gr;
+        rw.cj = rw.cj * rw.cr;
+    }
+}
+
+/* Inner sums of row j over columns 0..nloop-1 (Guard and row_in as for sep_row). */
+template <bool Guard, typename Pix, typename XBuf>
+LZR_HD LZR_INLINE EllRow ell_row_sums(const EllIn &q, int n, int nloop, bool fac, bool row_in, double g, double c,
+                                      const Pix &pix, const XBuf &xb, int j) {
+    EllRow r{{0.0, 0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
+    const double v = (double)j - q.y1;
+    int i0 = 0;
+    for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
+        ell_chunk<false, Guard>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, r);
+    if (i0 < nloop) ell_chunk<true, Guard>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, r);
+    return r;
+}
+
+/* The ellipsoidal moment pass of one lane over its n x n window (nloop >= n: the wave's
+ * loop bound; with `guard`, pixels beyond the lane's own n are masked).  pix.raw(i, j)
+ * returns pixel (i, j) as stored, xb holds one FP64 value per column.  Built from the same
+ * pieces a group of lanes sharing one fit runs (see sep_pass). */
+template <typename Pix, typename XBuf>
+LZR_HD LZR_INLINE void ell_pass(const EllIn &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
+                                double (&s)[kEllSums]) {
+    LZR_NO_CONTRACT
+    EllRows rw;
+    ell_columns(q, n, nloop, xb, s, rw);
     for (int j = 0; j < nloop; ++j) {
-        const double v = (double)j - q.y1;
-        double ey = gy.next();
-        if (!gy.ok) ey = dexp(-(q.qb * v) * v);
+        double w, g, c;
+        ell_row_factor(q, rw, j, w, g, c);
         const bool row_in = j < n;
-        double w = 1.0, g = 1.0, c = 1.0;
-        if (fac) {
-            w = ey;
-            g = gj;
-            c = cj;
-            gj = gj * gr;
-            cj = cj * cr;
-        }
-        double P[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, R[3] = {0.0, 0.0, 0.0}, W[3] = {0.0, 0.0, 0.0};
-        double rd = 0.0, rdd = 0.0;
-        int i0 = 0;
-        if (guard) {
-            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
-                ell_chunk<false, true>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
-            if (i0 < nloop) ell_chunk<true, true>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
-        } else {
-            for (; i0 + kSepChunk <= nloop; i0 += kSepChunk)
-                ell_chunk<false, false>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
-            if (i0 < nloop)
-                ell_chunk<true, false>(q, n, nloop, fac, row_in, v, g, c, pix, xb, i0, j, P, R, W, rd, rdd);
-        }
-        ell_row(row_in ? w : 0.0, v, P, R, W, rd, rdd, s);
+        const EllRow r = guard ? ell_row_sums<true>(q, n, nloop, rw.fac, row_in, g, c, pix, xb, j)
+                               : ell_row_sums<false>(q, n, nloop, rw.fac, row_in, g, c, pix, xb, j);
+        ell_row(row_in ? w : 0.0, (double)j - q.y1, r.P, r.R, r.W, r.rd, r.rdd, r.bad, s);
     }
 }
 
@@ -913,6 +1010,7 @@ LZR_HD LZR_INLINE void ell_finish(const EllIn &q, int npix, const double (&s)[kE
         is * is * ((A1 * ((A1 * S00 + 2.0 * (b1 * T00)) - 2.0 * U00) + b1 * (N * b1 - 2.0 * SD)) + s[kESDD]);
     a.sse0 = Acc(init ? sse1 : 0.0);
     a.dsse = Acc(init ? 0.0 : sse1 - fn2);
+    a.bad = s[kEBad] == 0.0 ? 0 : (s[kEBad] == 1.0 ? 1 : 2);
 }
 
 /* Column norms of the first `rows` Jacobian rows at t (GSL 1.16 update_diag reads the
@@ -920,6 +1018,7 @@ LZR_HD LZR_INLINE void ell_finish(const EllIn &q, int npix, const double (&s)[kE
 template <int P>
 LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double is, double w, int rows,
                                             double (&cn)[P]) {
+    LZR_NO_CONTRACT
 #pragma unroll
     for (int c = 0; c < P; ++c) cn[c] = 0.0;
     /* reciprocals once per call (is = 1/sigma); the rows then take products only */
@@ -940,7 +1039,7 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
         if constexpr (P == 4) {
             const double A = t[0];
             const double ux = i - t[1], uy = j - t[2];
-            const double e = (double)fexp((float)(-(ux * ux + uy * uy) * (0.5 * r0)));
+            const double e = dexp(-(ux * ux + uy * uy) * (0.5 * r0));
             const double ae = A * e * r0 * is;
             row[0] = e * is;
             row[1] = ae * ux;
@@ -950,7 +1049,7 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
             const double A = t[0];
             const double ux = i - t[2], uy = j - t[3];
             const double d2 = ux * ux + uy * uy;
-            const double e = (double)fexp((float)(-d2 * (0.5 * r1)));
+            const double e = dexp(-d2 * (0.5 * r1));
             const double ae = A * e * r1 * is;
             row[0] = e * is;
             row[1] = ae * d2 * r0 * (1.0 / kSqrt2);
@@ -962,7 +1061,7 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
             const double ux = i - t[3], uy = j - t[4];
             const double vx = ux * r0, vy = uy * r1; /* ux/sx, uy/sy */
             const double q = vx * vx + vy * vy - 2.0 * rho * vx * vy;
-            const double e = (double)fexp((float)(-q * (0.5 * r2)));
+            const double e = dexp(-q * (0.5 * r2));
             const double aed = A * e * r3;
             row[0] = e * is;
             row[1] = -aed * (2.0 * rho * vx * vy - 2.0 * vx * vx) * r0;
@@ -976,7 +1075,7 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
             row[6] = is;
         }
 #pragma unroll
-        for (int c = 0; c < P; ++c) cn[c] += row[c] * row[c];
+        for (int c = 0; c < P; ++c) cn[c] = dfma(row[c], row[c], cn[c]);
         /* pixel k + 1 = i + n j, without a division */
         if (++i == n) {
             i = 0;
@@ -989,20 +1088,22 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
 
 /* Full normal equations from the pass accumulators. */
 template <int P, typename Acc>
-LZR_HD LZR_INLINE void assemble(const PassAcc<P, Acc> &a, int npix, double is, float (&H)[NP<P>::S], float (&g)[P]) {
+LZR_HD LZR_INLINE void assemble(const PassAcc<P, Acc> &a, int npix, double is, double (&H)[NP<P>::S], double (&g)[P]) {
+    LZR_NO_CONTRACT
 #pragma unroll
     for (int i = 0; i < P - 1; ++i) {
 #pragma unroll
-        for (int j = 0; j <= i; ++j) H[tri(i, j)] = (float)a.hq[tri(i, j)];
-        H[tri(P - 1, i)] = (float)((double)a.sj[i] * is);
-        g[i] = (float)a.gq[i];
+        for (int j = 0; j <= i; ++j) H[tri(i, j)] = (double)a.hq[tri(i, j)];
+        H[tri(P - 1, i)] = (double)a.sj[i] * is;
+        g[i] = (double)a.gq[i];
     }
-    H[tri(P - 1, P - 1)] = (float)((double)npix * is * is);
-    g[P - 1] = (float)((double)a.sf * is);
+    H[tri(P - 1, P - 1)] = (double)npix * is * is;
+    g[P - 1] = (double)a.sf * is;
 }
 
 template <int P>
-LZR_HD LZR_INLINE void to_double(const float (&Hf)[NP<P>::S], const float (&gf)[P], double (&H)[NP<P>::S], double (&g)[P]) {
+LZR_HD LZR_INLINE void to_double(const double (&Hf)[NP<P>::S], const double (&gf)[P], double (&H)[NP<P>::S], double (&g)[P]) {
+    LZR_NO_CONTRACT
 #pragma unroll
     for (int i = 0; i < NP<P>::S; ++i) H[i] = (double)Hf[i];
 #pragma unroll
@@ -1033,6 +1134,7 @@ LZR_HD LZR_INLINE void clear_record(LzrFit &o) {
 template <int P>
 LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const LzrCandidate &c, int roi, int nx,
                                   int ny, int nframes, int model, LzrFit &out, const LzrStart *st = nullptr) {
+    LZR_NO_CONTRACT
     s.roi = roi;
     const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
     if (h < 1 || h > cfg.max_half_width || c.frame < 0 || c.frame >= nframes) {
@@ -1043,9 +1145,10 @@ LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const 
     }
     const int n = 2 * h + 1;
     /* x_offset = floor(x - cutoff_radius) (source/Localization.cpp:112-124) */
+    /* floor(fx) + n - 1 <= nx - 1  <=>  fx < nx - n + 1 for fx >= 0, compared in FP64 so that no
+     * candidate coordinate can overflow an int (NaN coordinates fail every test) */
     const double fx = c.x - (double)h, fy = c.y - (double)h;
-    const bool inside = (fx >= 0.0) && (fy >= 0.0) && (fx < 2147483647.0) && (fy < 2147483647.0) &&
-                        ((int)fx + n - 1 <= nx - 1) && ((int)fy + n - 1 <= ny - 1);
+    const bool inside = (fx >= 0.0) && (fy >= 0.0) && (fx < (double)(nx - n + 1)) && (fy < (double)(ny - n + 1));
     if (!inside) {
         clear_record(out);
         out.model = (int16_t)model;
@@ -1115,6 +1218,7 @@ LZR_HD LZR_INLINE bool lane_start(LaneState<P> &s, const LaneConfig &cfg, const 
 /* Constants of a separable pass at the lane's trial point (the start point at kInit). */
 template <int P>
 LZR_HD LZR_INLINE void lane_sep_in(const LaneState<P> &s, const LaneConfig &cfg, SepIn &q) {
+    LZR_NO_CONTRACT
     double step[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
@@ -1123,6 +1227,7 @@ LZR_HD LZR_INLINE void lane_sep_in(const LaneState<P> &s, const LaneConfig &cfg,
 
 /* Constants of an ellipsoidal moment pass at the lane's trial point. */
 LZR_HD LZR_INLINE void lane_ell_in(const LaneState<7> &s, const LaneConfig &cfg, EllIn &q) {
+    LZR_NO_CONTRACT
     double step[7];
 #pragma unroll
     for (int i = 0; i < 7; ++i) step[i] = s.phase == kTrial ? s.dx[i] : 0.0;
@@ -1143,6 +1248,7 @@ LZR_HD LZR_INLINE void sep_pixel(double e, double u, double w, double D, double 
 
 template <int P>
 LZR_HD LZR_INLINE int test_delta(const LaneState<P> &s, const LaneConfig &cfg) {
+    LZR_NO_CONTRACT
     double a[P];
     absolute<P>(s, s.th, a);
 #pragma unroll
@@ -1156,6 +1262,7 @@ LZR_HD LZR_INLINE int test_delta(const LaneState<P> &s, const LaneConfig &cfg) {
 /* Proposes the next trial (one lmpar + the first-iteration radius clamp). */
 template <int P>
 LZR_HD LZR_INLINE void propose(LaneState<P> &s) {
+    LZR_NO_CONTRACT
     double H[NP<P>::S], g[P], x[P];
     to_double<P>(s.H, s.g, H, g);
     s.inner++;
@@ -1173,6 +1280,7 @@ LZR_HD LZR_INLINE void propose(LaneState<P> &s) {
  * ETOLG test reads it gives the same value. */
 template <int P>
 LZR_HD LZR_INLINE double gradient_norm(const LaneState<P> &s) {
+    LZR_NO_CONTRACT
     double gmax = 0.0;
 #pragma unroll
     for (int i = 0; i < P; ++i) gmax = dmax(gmax, dabs((double)s.g[i] / s.diag[i]));
@@ -1184,6 +1292,7 @@ LZR_HD LZR_INLINE double gradient_norm(const LaneState<P> &s) {
  * false when the loop ended (status in *st). */
 template <int P>
 LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int *st) {
+    LZR_NO_CONTRACT
     for (;;) {
         s.iterations++;
         if (s.fnorm != 0.0) break;
@@ -1198,17 +1307,13 @@ LZR_HD LZR_INLINE bool begin_iterate(LaneState<P> &s, const LaneConfig &cfg, int
     return true;
 }
 
-/* Stores the raw end state of the lane's fit (the reference's x, J^T J, |f| and loop
- * counters) for the epilogue. */
+/* Stores the raw end state of the lane's fit (the reference's x, |f| and loop counters) for
+ * the epilogue. */
 template <int P>
 LZR_HD LZR_INLINE void store_final(LaneState<P> &s, int lm_status, int model, LzrRawFit *o) {
     if (o) { /* null: another lane holding the same fit writes the record */
-        double x[P];
-        absolute<P>(s, s.th, x);
 #pragma unroll
-        for (int i = 0; i < P; ++i) o->th[i] = x[i];
-#pragma unroll
-        for (int i = 0; i < NP<P>::S; ++i) o->H[i] = s.H[i];
+        for (int i = 0; i < P; ++i) o->th[i] = s.th[i];
         o->chi = s.fnorm;
         o->status = kPending;
         o->lm_status = lm_status;
@@ -1224,6 +1329,7 @@ LZR_HD LZR_INLINE void store_final(LaneState<P> &s, int lm_status, int model, Lz
  * with the fit's final status in *st_out. */
 template <int P, typename Acc>
 LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int *st_out) {
+    LZR_NO_CONTRACT
     const int npix = s.n * s.n;
     if (s.phase == kInit) {
         /* lmsder set(): f, J at the start point */
@@ -1247,12 +1353,25 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
     }
     /* trial evaluated at th + dx */
     s.trials++;
+    /* a zero width makes the reference's model function fail (source/Localization.cpp:1029-1031,
+     * :1098-1100): iterate() returns that status and the driver leaves its loop */
+    bool zero_width = false;
+    if constexpr (P == 5) zero_width = s.th[1] + s.dx[1] == 0.0;
+    if constexpr (P == 7) zero_width = s.th[1] + s.dx[1] == 0.0 || s.th[2] + s.dx[2] == 0.0;
+    if (zero_width) {
+        *st_out = LZR_LM_FAILURE;
+        return false;
+    }
     const double fn2 = s.fnorm * s.fnorm;
-    /* FP32 overflow (a trial far outside the model's domain, e.g. |rho| > 1) stands for
-     * a residual the FP64 reference finds astronomically large: actred = -1 there. */
+    /* |f1| as GSL's scaled 2-norm sees the trial residual vector: NaN when it holds a NaN or
+     * two infinite entries, +inf for exactly one (a step to |rho| >= 1 overflows the model).
+     * Sums that overflow while every entry is finite stand for an astronomically large but
+     * finite norm, which the tests below treat like +inf. */
     const double dsse = (double)a.dsse;
     const bool finite = dsse == dsse && dabs(dsse) <= 1.0e300;
-    const double fnorm1 = finite ? dsqrt(dmax(fn2 + dsse, 0.0)) : 1.0e300;
+    double fnorm1 = finite ? dsqrt(dmax(fn2 + dsse, 0.0)) : 1.0e300;
+    if (a.bad == 1) fnorm1 = __builtin_inf();
+    if (a.bad == 2) fnorm1 = __builtin_nan("");
     const double inv_fn2 = drcp(fn2);
     double actred = -1.0;
     if (0.1 * fnorm1 < s.fnorm) actred = -dsse * inv_fn2; /* = 1 - (fnorm1/fnorm)^2 */
@@ -1265,8 +1384,8 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
     for (int i = 0; i < P; ++i) {
         double row = 0.0;
 #pragma unroll
-        for (int j = 0; j < P; ++j) row += H[tri(i, j)] * s.dx[j];
-        jdx2 += s.dx[i] * row;
+        for (int j = 0; j < P; ++j) row = dfma(H[tri(i, j)], s.dx[j], row);
+        jdx2 = dfma(s.dx[i], row, jdx2);
     }
     const double pnorm = s.pnorm; /* D is unchanged since the step was proposed */
     const double t1sq = dmax(jdx2, 0.0) * inv_fn2;
@@ -1353,6 +1472,7 @@ LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, c
 /* 2x2 eigen-decomposition of the ellipsoidal covariance (source/Localization.cpp:525-547):
  * ascending eigenvalues, theta = atan(v0/v1) of the first eigenvector (0 if v1 == 0). */
 LZR_HD LZR_INLINE bool eigen2(double sx, double sy, double rho, double &sd1, double &sd2, double &theta) {
+    LZR_NO_CONTRACT
     const double a = sx * sx, c = sy * sy, b = rho * sx * sy;
     const double lim = 1.79769313486231570815e+308;
     if (!(dabs(a) <= lim) || !(dabs(b) <= lim) || !(dabs(c) <= lim)) return false;
@@ -1388,16 +1508,17 @@ LZR_HD LZR_INLINE double dsin(double v) {
 #endif
 }
 
-/* Diagonal of (J^T J)^-1 at the final point (gsl_multifit_covar(J, 0.0, C) diagonal).
- * Returns false if J^T J is singular. */
+/* Diagonal of (J^T J)^-1 at the final point (gsl_multifit_covar(J, 0.0, C) diagonal), by
+ * Cholesky in FP64.  Returns false (and +inf) if J^T J is numerically singular. */
 template <int P>
-LZR_HD LZR_INLINE bool covariance_diagonal(const float (&Hf)[NP<P>::S], double (&cdiag)[P]) {
-    double H[NP<P>::S], L[NP<P>::S], id[P];
-#pragma unroll
-    for (int i = 0; i < NP<P>::S; ++i) H[i] = (double)Hf[i];
+LZR_HD LZR_INLINE bool covariance_diagonal(const double (&H)[NP<P>::S], double (&cdiag)[P]) {
+    LZR_NO_CONTRACT
+    double L[NP<P>::S], id[P];
     if (!cholesky<P>(H, L, id)) {
+        /* numerically singular: the parameters are undetermined (GSL's QRPT covariance gives
+         * rounding-noise values of 1e10 and more there) */
 #pragma unroll
-        for (int i = 0; i < P; ++i) cdiag[i] = 0.0;
+        for (int i = 0; i < P; ++i) cdiag[i] = __builtin_inf();
         return false;
     }
     /* C = L^-T L^-1; C_kk = sum_i (L^-1)_{ik}^2 */
@@ -1409,31 +1530,68 @@ LZR_HD LZR_INLINE bool covariance_diagonal(const float (&Hf)[NP<P>::S], double (
         forward<P>(L, id, e, y);
         double s2 = 0.0;
 #pragma unroll
-        for (int i = 0; i < P; ++i) s2 += y[i] * y[i];
+        for (int i = 0; i < P; ++i) s2 = dfma(y[i], y[i], s2);
         cdiag[k] = s2;
     }
     return true;
 }
 
+/* J^T J at the window-coordinate point t (the same pass, in the same order, as the lane's
+ * evaluation at that point: a fit's final J^T J, bit for bit).  pix.raw(i, j) reads pixel (i, j)
+ * of the n x n window. */
+template <int P, typename Pix>
+LZR_HD LZR_INLINE void normal_matrix(const double (&t)[P], const LaneConfig &cfg, int n, const Pix &pix,
+                                     double (&H)[NP<P>::S]) {
+    LZR_NO_CONTRACT
+    double zero[P], g[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) zero[i] = 0.0;
+    struct {
+        double v[2 * LZR_MAX_HALF_WIDTH + 1];
+        LZR_HD void set(int i, double x) { v[i] = x; }
+        LZR_HD double get(int i) const { return v[i]; }
+    } xb;
+    PassAcc<P, double> a;
+    a.zero();
+    if constexpr (P == 7) {
+        EllIn q;
+        make_ell_in(t, zero, cfg.inv_sigma, q);
+        double v[kEllSums];
+        ell_pass(q, n, n, false, pix, xb, v);
+        ell_finish<double>(q, n * n, v, true, 0.0, a);
+    } else {
+        SepIn q;
+        make_sep_in(t, zero, cfg.inv_sigma, cfg.initial_psf_width, q);
+        double v[kSepSums];
+        sep_pass(q, n, n, false, pix, xb, v);
+        sep_finish<P, double>(q, n * n, v, true, 0.0, a);
+    }
+    assemble<P, double>(a, n * n, cfg.inv_sigma, H, g);
+}
+
 /* Post-checks and output record of the reference drivers
  * (source/Localization.cpp:157-211 symmetric, :336-378 fixed width, :506-596 ellipsoidal,
- * :606-631 filter),
- * from the raw end state r of candidate c. */
-template <int P>
-LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, LzrFit &out) {
+ * :606-631 filter), from the raw end state r of candidate c; pix reads the candidate's window
+ * (for the covariance of accepted fits: gsl_multifit_covar(J, 0.0, C), :185, :556, from
+ * J^T J in FP64). */
+template <int P, typename Pix>
+LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, const Pix &pix,
+                                     LzrFit &out) {
+    LZR_NO_CONTRACT
     const int model = r.model;
     const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
     const int n = 2 * h + 1;
     const double xo = (double)(int)(c.x - (double)h), yo = (double)(int)(c.y - (double)h);
     const double xm = xo + n - 1, ym = yo + n - 1;
     const double w = cfg.initial_psf_width;
-    double x[P];
-    float Hf[NP<P>::S];
+    double t[P], x[P];
 #pragma unroll
-    for (int i = 0; i < P; ++i) x[i] = r.th[i];
-#pragma unroll
-    for (int i = 0; i < NP<P>::S; ++i) Hf[i] = r.H[i];
+    for (int i = 0; i < P; ++i) t[i] = x[i] = r.th[i];
+    /* absolute coordinates as the lane forms them (window offset + local) */
+    x[x_index<P>()] += xo;
+    x[x_index<P>() + 1] += yo;
     const double chi = r.chi;
+    double H[NP<P>::S];
     clear_record(out);
     out.iterations = r.iterations;
     out.lm_status = r.lm_status;
@@ -1458,7 +1616,8 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance_diagonal<P>(Hf, cd);
+        normal_matrix<P>(t, cfg, n, pix, H);
+        covariance_diagonal<P>(H, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 4);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1488,7 +1647,8 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance_diagonal<P>(Hf, cd);
+        normal_matrix<P>(t, cfg, n, pix, H);
+        covariance_diagonal<P>(H, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 5);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1527,7 +1687,8 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance_diagonal<P>(Hf, cd);
+        normal_matrix<P>(t, cfg, n, pix, H);
+        covariance_diagonal<P>(H, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1559,17 +1720,26 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
 }
 
 /* Epilogue entry: finishes a record written by store_final (no-op for records already
- * final, i.e. rejected before fitting). */
-LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c) {
+ * final, i.e. rejected before fitting).  frames: the stack (nx, ny) the fit read. */
+template <typename FrameT>
+LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c, const FrameT *frames,
+                                  int nx, int ny) {
     LzrRawFit r;
     __builtin_memcpy(&r, &rec, sizeof r);
     if (r.status != kPending) return;
+    const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
+    const int xoff = (int)(c.x - (double)h), yoff = (int)(c.y - (double)h);
+    struct {
+        const FrameT *src;
+        int nx;
+        LZR_HD FrameT raw(int i, int j) const { return src[(size_t)j * nx + i]; }
+    } pix{frames + ((size_t)c.frame * ny + yoff) * nx + xoff, nx};
     if (r.model == LZR_MODEL_SYMMETRIC)
-        finish_record<5>(r, cfg, c, rec);
+        finish_record<5>(r, cfg, c, pix, rec);
     else if (r.model == LZR_MODEL_FIXED_WIDTH)
-        finish_record<4>(r, cfg, c, rec);
+        finish_record<4>(r, cfg, c, pix, rec);
     else
-        finish_record<7>(r, cfg, c, rec);
+        finish_record<7>(r, cfg, c, pix, rec);
 }
 
 }  // namespace lzr

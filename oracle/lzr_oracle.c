@@ -481,6 +481,7 @@ typedef struct Lm {
     int iter;
     int trials, accepted;
     double *trace;
+    double *trial_trace;
     int max_trace;
 } Lm;
 
@@ -554,6 +555,10 @@ static int lm_iterate(Lm *s) {
 
         s->trials++;
         int st = eval_f(s, s->x_trial, s->f_trial);
+        if (s->trial_trace && s->trials <= s->max_trace) {
+            memcpy(s->trial_trace + (long)(s->trials - 1) * p, s->x_trial, sizeof(double) * p);
+            s->trial_trace[(long)s->max_trace * p + s->trials - 1] = st ? NAN : orc_nrm2(N, s->f_trial, 1);
+        }
         if (st) {
             record_trace(s);
             return st;
@@ -665,7 +670,7 @@ void orc_covar(int N, int p, const double *J, double *C) {
 int orc_lm_run(int model, const double *data, int n, double xoff, double yoff, double sigma,
                const double *theta0, const OrcOptions *opt, double *theta, double *f_out,
                double *J_out, int *iterations, int *lm_status, int *accepted, double *trace,
-               int max_trace, double fixed_width) {
+               int max_trace, double fixed_width, double *trial_trace) {
     Lm *s = (Lm *)calloc(1, sizeof(Lm));
     s->model = (model == LZR_MODEL_SYMMETRIC || model == LZR_MODEL_FIXED_WIDTH) ? model : LZR_MODEL_ELLIPSOIDAL;
     s->p = s->model == LZR_MODEL_SYMMETRIC ? 5 : (s->model == LZR_MODEL_FIXED_WIDTH ? 4 : 7);
@@ -678,6 +683,7 @@ int orc_lm_run(int model, const double *data, int n, double xoff, double yoff, d
     s->sigma = sigma;
     s->opt = *opt;
     s->trace = trace;
+    s->trial_trace = trial_trace;
     s->max_trace = max_trace;
     memcpy(s->x, theta0, sizeof(double) * s->p);
     int status = lm_set(s);

@@ -86,13 +86,15 @@ void orc_covar(int N, int p, const double *J, double *C);
 
 /* Runs the LM loop of the reference driver on one window (model, data n x n at
  * (xoff, yoff)) from theta0.  Writes the final theta, and if trace != NULL the accepted
- * point after every trial evaluation (trace[t*p + i], t < max_trace).  Returns the
- * number of trials; *iterations, *lm_status, *accepted receive the loop counters.
- * fixed_width is the width of LZR_MODEL_FIXED_WIDTH (ignored by the other models). */
+ * point after every trial evaluation (trace[t*p + i], t < max_trace), and if trial_trace !=
+ * NULL the trial point of every evaluation (same layout) and its residual norm
+ * (trial_trace[max_trace*p + t]).  Returns the number of trials; *iterations, *lm_status,
+ * *accepted receive the loop counters.  fixed_width is the width of LZR_MODEL_FIXED_WIDTH
+ * (ignored by the other models). */
 int orc_lm_run(int model, const double *data, int n, double xoff, double yoff, double sigma,
                const double *theta0, const OrcOptions *opt, double *theta, double *f_out,
                double *J_out, int *iterations, int *lm_status, int *accepted, double *trace,
-               int max_trace, double fixed_width);
+               int max_trace, double fixed_width, double *trial_trace);
 
 /* Eigen-decomposition of [[sx^2, rho sx sy], [rho sx sy, sy^2]] as used at
  * source/Localization.cpp:525-547: ascending eigenvalues, sd = sqrt(lambda),

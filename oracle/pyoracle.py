@@ -40,7 +40,7 @@ class Settings(ctypes.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liblzr_oracle.so")
+        path = os.environ.get("LZR_ORACLE_LIB") or os.path.join(_HERE, "liblzr_oracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: build the oracle (make -C oracle)")
         L = ctypes.CDLL(path)
@@ -63,7 +63,7 @@ def lib():
         L.orc_covar.restype = None
         L.orc_lm_run.argtypes = [
             ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-            _dp, ctypes.POINTER(Options), _dp, _dp, _dp, _ip, _ip, _ip, _dp, ctypes.c_int, ctypes.c_double,
+            _dp, ctypes.POINTER(Options), _dp, _dp, _dp, _ip, _ip, _ip, _dp, ctypes.c_int, ctypes.c_double, _dp,
         ]
         L.orc_lm_run.restype = ctypes.c_int
         L.orc_eigen2.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
@@ -175,7 +175,8 @@ def jacobian_fixed_width(theta, width, n, xoff, yoff, sigma=1.0):
 def lm_run(model_code, data, n, xoff, yoff, theta0, sigma=1.0, options=None, max_trace=0, fixed_width=0.0):
     """Runs the reference LM loop on one window.  Returns a dict with theta, f, J,
     iterations, lm_status, accepted, trials and (if max_trace) the accepted point after
-    every trial.  fixed_width: the width of the fixed-width model."""
+    every trial ("trace"), every trial point ("trial_points") and the trial's residual norm
+    ("trial_norms").  fixed_width: the width of the fixed-width model."""
     p = abi.n_params(model_code)
     opt = options if options is not None else default_options()
     data = np.ascontiguousarray(data, np.float64).reshape(-1)
@@ -185,13 +186,18 @@ def lm_run(model_code, data, n, xoff, yoff, theta0, sigma=1.0, options=None, max
     J = np.empty((n * n, p))
     it, st, acc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     trace = np.full((max(max_trace, 1), p), np.nan)
+    ttrace = np.full(max(max_trace, 1) * (p + 1), np.nan)
     trials = lib().orc_lm_run(
         int(model_code), _ptr(data), n, float(xoff), float(yoff), float(sigma), _ptr(theta0),
         ctypes.byref(opt), _ptr(theta), _ptr(f), _ptr(J), ctypes.byref(it), ctypes.byref(st),
         ctypes.byref(acc), _ptr(trace) if max_trace else None, int(max_trace), float(fixed_width),
+        _ptr(ttrace) if max_trace else None,
     )
+    m = min(trials, max_trace)
     return dict(theta=theta, f=f, J=J, iterations=it.value, lm_status=st.value, accepted=acc.value,
-                trials=trials, trace=trace[: min(trials, max_trace)] if max_trace else None)
+                trials=trials, trace=trace[:m] if max_trace else None,
+                trial_points=ttrace[: max_trace * p].reshape(max(max_trace, 1), p)[:m] if max_trace else None,
+                trial_norms=ttrace[max_trace * p: max_trace * p + m] if max_trace else None)
 
 
 def eigen2(sx, sy, rho):

@@ -25,6 +25,12 @@ def lib():
         L.emu_ell_pass.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_void_p, ctypes.c_void_p]
         L.emu_ell_pass.restype = ctypes.c_int
+        L.emu_sep_pass_group.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.emu_sep_pass_group.restype = ctypes.c_int
+        L.emu_ell_pass_group.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p]
+        L.emu_ell_pass_group.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -72,3 +78,37 @@ def ell_pass(theta, dx, sigma, window, nloop=None):
                             out.ctypes.data)
     assert rc == 0
     return out[:21], out[21:27], out[27:33], out[-2], out[-1]
+
+
+def sep_pass_group(theta, dx, sigma, w, window, k):
+    """The separable pass as a straggler group of k lanes computes it (every lane's copy must
+    agree); returns the flat output of sep_pass."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    dx = np.ascontiguousarray(dx, np.float64)
+    p = theta.size
+    window = np.ascontiguousarray(window, np.uint16)
+    n = window.shape[0]
+    out = np.zeros((p - 1) * p // 2 + 2 * (p - 1) + 2)
+    assert lib().emu_sep_pass_group(p, theta.ctypes.data, dx.ctypes.data, float(sigma), float(w), n, int(k),
+                                    window.ctypes.data, out.ctypes.data) == 0
+    return out
+
+
+def sep_pass_flat(theta, dx, sigma, w, window):
+    return np.concatenate([np.atleast_1d(a) for a in sep_pass(theta, dx, sigma, w, window)])
+
+
+def ell_pass_group(theta, dx, sigma, window, k):
+    """The ellipsoidal moment pass as a straggler group of k lanes computes it."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    dx = np.ascontiguousarray(dx, np.float64)
+    window = np.ascontiguousarray(window, np.uint16)
+    n = window.shape[0]
+    out = np.zeros(21 + 6 + 6 + 2)
+    assert lib().emu_ell_pass_group(theta.ctypes.data, dx.ctypes.data, float(sigma), n, int(k), window.ctypes.data,
+                                    out.ctypes.data) == 0
+    return out
+
+
+def ell_pass_flat(theta, dx, sigma, window):
+    return np.concatenate([np.atleast_1d(a) for a in ell_pass(theta, dx, sigma, window)])

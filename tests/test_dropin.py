@@ -11,6 +11,7 @@ import pytest
 
 from localizer_amd import abi, synth
 from oracle import pyoracle as po
+from tests.parity import assert_parity, compare, format_stats
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "cpp", "dropin_main")
@@ -115,12 +116,11 @@ def test_dropin_matches_c_abi_and_oracle(tmp_path, model, mode):
         c = cands[cands["frame"] == fi]
         np.testing.assert_array_equal(surv[:, 0], c["x"][~erased])
     ref = po.fit_batch(frames, cands, model, w_psf)
-    agree = np.mean(ref["status"] == capi["status"])
-    both = (ref["status"] == 0) & (capi["status"] == 0)
-    dxy = np.abs(ref["params"][both][:, xi:xi + 2] - capi["params"][both][:, xi:xi + 2]).max(axis=1)
-    print(f"drop-in model={model}: status agreement {agree:.4f}, xy within 1e-3: {np.mean(dxy <= 1e-3):.4f}")
-    assert agree >= 0.99
-    assert np.mean(dxy <= 1e-3) >= 0.98
+    st = compare(ref, capi, model)
+    print(format_stats(f"drop-in model={model}", st))
+    # 600 fits: the 0.1 % bar is below one fit; at most one miss per metric (the C ABI's
+    # parity at scale is tests/test_gpu_parity.py)
+    assert_parity(st, model, slack=1)
 
 
 @pytest.mark.gpu

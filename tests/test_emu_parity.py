@@ -1,13 +1,14 @@
 """The GPU lane logic (localizer_amd/csrc/lzr_lane.h), compiled for the host, against the
-FP64 oracle.  This checks the normal-equation restatement of lmsder and its FP32/FP64
-split on the CPU; the same code runs on the GPU in tests/test_gpu_parity.py."""
+FP64 oracle.  This checks the normal-equation restatement of lmsder, the moment passes and
+the record epilogue on the CPU, to the same bar as the GPU tests (tests/parity.py); the same
+code runs on the GPU in tests/test_gpu_parity.py."""
 import numpy as np
 import pytest
 
 from localizer_amd import abi, synth
 from oracle import pyoracle as po
 from tests.emu import pyemu
-from tests.parity import compare, format_stats
+from tests.parity import assert_parity, compare, format_stats, oracle_sigma_ulp
 
 
 @pytest.fixture(scope="module")
@@ -26,10 +27,8 @@ def test_emulated_lanes_match_oracle_7x7(plain_cpu):
     got = pyemu.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.3, default_half_width=3)
     st = compare(ref, got, abi.MODEL_SYMMETRIC)
     print(format_stats("emu 7x7", st))
-    assert st["status_agreement"] >= 0.999
+    assert_parity(st, abi.MODEL_SYMMETRIC)
     assert st["xy_frac_within_1e-3"] == 1.0
-    for k in ("A", "r", "b"):
-        assert st[f"{k}_frac_within_1e-4"] >= 0.995
 
 
 def test_emulated_lanes_match_oracle_9x9(plain_gpu_sample):
@@ -38,23 +37,24 @@ def test_emulated_lanes_match_oracle_9x9(plain_gpu_sample):
     got = pyemu.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.35, default_half_width=4)
     st = compare(ref, got, abi.MODEL_SYMMETRIC)
     print(format_stats("emu 9x9", st))
-    assert st["status_agreement"] >= 0.999
-    assert st["xy_frac_within_1e-3"] >= 0.999
-    for k in ("A", "r", "b"):
-        assert st[f"{k}_frac_within_1e-4"] >= 0.995
+    assert_parity(st, abi.MODEL_SYMMETRIC)
 
 
-def test_emulated_lanes_match_oracle_ellipsoidal(plain_gpu_sample):
-    frames, _, cands = plain_gpu_sample
+@pytest.fixture(scope="module")
+def plain_20k():
+    return synth.generate(synth.PLAIN_GPU_256.scaled(400), nthreads=8)
+
+
+def test_emulated_lanes_match_oracle_ellipsoidal(plain_20k):
+    """20k fits: the 0.1 % bar is then a rate, not a matter of two or three fits."""
+    frames, _, cands = plain_20k
     ref = po.fit_batch(frames, cands, abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4)
     got = pyemu.fit_batch(frames, cands, abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4)
     st = compare(ref, got, abi.MODEL_ELLIPSOIDAL)
     print(format_stats("emu ellipsoidal 9x9", st))
-    assert st["status_agreement"] >= 0.995
-    # fits that converge in both (test_delta, same iteration count) agree tightly
-    ok = (ref["lm_status"] == 0) & (got["lm_status"] == 0) & (ref["status"] == 0) & (got["status"] == 0)
-    dxy = np.abs(ref["params"][ok][:, 3:5] - got["params"][ok][:, 3:5]).max(axis=1)
-    assert np.mean(dxy <= 1e-3) >= 0.995
+    spread = compare(ref, oracle_sigma_ulp(po, frames, cands, abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4),
+                     abi.MODEL_ELLIPSOIDAL)
+    assert_parity(st, abi.MODEL_ELLIPSOIDAL, spread=spread)
 
 
 def test_emulated_lanes_match_oracle_fixed_width(plain_gpu_sample):
@@ -63,13 +63,8 @@ def test_emulated_lanes_match_oracle_fixed_width(plain_gpu_sample):
     got = pyemu.fit_batch(frames, cands, abi.MODEL_FIXED_WIDTH, 1.35, default_half_width=4)
     st = compare(ref, got, abi.MODEL_FIXED_WIDTH)
     print(format_stats("emu fixed width 9x9", st))
-    assert st["status_agreement"] >= 0.999
-    assert st["xy_frac_within_1e-3"] >= 0.999
-    for k in ("A", "b"):
-        assert st[f"{k}_frac_within_1e-4"] >= 0.995
+    assert_parity(st, abi.MODEL_FIXED_WIDTH)
     acc = (ref["status"] == 0) & (got["status"] == 0)
-    np.testing.assert_allclose(got["integral"][acc], ref["integral"][acc], rtol=1e-3)
-    np.testing.assert_allclose(got["deviations"][acc][:, :4], ref["deviations"][acc][:, :4], rtol=1e-2)
     assert np.all(got["sd1"][acc] == 1.35)
 
 
@@ -93,3 +88,30 @@ def test_fixed_width_distance_rejection():
         assert abs(fw["params"][1][1] - 20.0) < 0.01
         sym = fit(abi.MODEL_SYMMETRIC)
         assert abi.REJECT_DISTANCE not in sym["status"]
+
+
+def test_nonfinite_trial_residual_follows_gsl_norm(plain_20k):
+    """A trial step to |rho| > 1 overflows the ellipsoidal model: the reference's residual
+    vector then holds infinite entries and GSL's scaled 2-norm is NaN for two or more of them
+    (+inf for one), which sends lmsder's trust-region update down a different branch than a
+    merely large norm.  Spots whose reference trajectory passes through a NaN-norm trial must
+    follow the oracle (same iteration and trial counts, position within 1e-3 px)."""
+    frames, _, cands = plain_20k
+    sel = np.arange(0, cands.size, 4)
+    ref = po.fit_batch(frames, cands[sel], abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4)
+    got = pyemu.fit_batch(frames, cands[sel], abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4)
+    hit = agree = 0
+    for k in np.nonzero(ref["status"] == 0)[0]:
+        c = cands[sel[k]]
+        xo, yo = int(np.floor(c["x"] - 4)), int(np.floor(c["y"] - 4))
+        D = frames[c["frame"], yo:yo + 9, xo:xo + 9].astype(np.float64)
+        r = po.lm_run(abi.MODEL_ELLIPSOIDAL, D, 9, xo, yo,
+                      np.array([c["intensity"], 1.35, 1.35, c["x"], c["y"], 0.0, c["background"]]), max_trace=250)
+        if not np.isnan(r["trial_norms"]).any():
+            continue
+        hit += 1
+        agree += (got["trials"][k] == ref["trials"][k] and got["iterations"][k] == ref["iterations"][k]
+                  and np.abs(got["params"][k][3:5] - ref["params"][k][3:5]).max() <= 1e-3)
+    print(f"{hit} spots with a NaN-norm trial, {agree} follow the oracle")
+    assert hit >= 3
+    assert agree == hit

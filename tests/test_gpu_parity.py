@@ -1,7 +1,13 @@
 """GPU parity: the HIP kernels (through the C ABI) against the FP64 oracle and against
-the host build of the same lane logic.  Tolerances: x, y within
-1e-3 px, width / amplitude / offset within 1e-4 relative, accept/reject agreement
->= 99.9%."""
+the host build of the same lane logic.  The bar (tests/parity.py): accept/reject status
+agreement >= 99.9 %; on fits both sides accept and converge, x, y within 1e-3 px and every
+other parameter and record field (deviations, chi, integral, integral deviation, sd1, sd2
+and their deviations, theta) within 1e-4 for >= 99.9 % of the fits; every both-accepted fit
+(including those that hit the iteration cap or GSL_ENOPROG, where the reference itself is
+not reproducible under a one-ulp change of its start) at the same bar for the separable
+models and 99.8 % for the ellipsoidal model.  A fit's result must not depend on its batch:
+the straggler kernel's groups compute the one-lane pass bit for bit, so results are
+bit-identical with any hand-off threshold, none included."""
 import sys
 
 import numpy as np
@@ -10,7 +16,7 @@ import pytest
 from localizer_amd import abi, native, synth
 from oracle import pyoracle as po
 from tests.emu import pyemu
-from tests.parity import compare, format_stats
+from tests.parity import assert_parity, compare, format_stats, oracle_sigma_ulp
 
 pytestmark = pytest.mark.gpu
 
@@ -30,11 +36,13 @@ def plain_gpu_sample():
     return synth.generate(synth.PLAIN_GPU_256.scaled(200))
 
 
-def _check_sym(st, xy=0.999, rel=0.995):
-    assert st["status_agreement"] >= 0.999, st
-    assert st["xy_frac_within_1e-3"] >= xy, st
-    for k in ("A", "r", "b"):
-        assert st[f"{k}_frac_within_1e-4"] >= rel, (k, st)
+def _check_sym(st):
+    assert_parity(st, abi.MODEL_SYMMETRIC)
+
+
+def _bytes_differ(a, b):
+    """Per-fit flags: records that differ in any byte."""
+    return np.any(a.view(np.uint8).reshape(a.size, -1) != b.view(np.uint8).reshape(b.size, -1), axis=1)
 
 
 def test_gpu_symmetric_7x7_matches_oracle(plain_cpu):
@@ -45,7 +53,8 @@ def test_gpu_symmetric_7x7_matches_oracle(plain_cpu):
     ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.3, default_half_width=3)
     st = compare(ref, got, abi.MODEL_SYMMETRIC)
     _say(format_stats("gpu vs oracle 7x7", st))
-    _check_sym(st, xy=1.0)
+    _check_sym(st)
+    assert st["xy_frac_within_1e-3"] == 1.0
 
 
 def test_gpu_symmetric_9x9_matches_oracle(plain_gpu_sample):
@@ -67,34 +76,23 @@ def test_gpu_matches_host_build_of_lane_logic(plain_gpu_sample):
     emu = pyemu.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.35, default_half_width=4)
     st = compare(emu, got, abi.MODEL_SYMMETRIC)
     _say(format_stats("gpu vs emulation 9x9", st))
-    assert st["status_agreement"] >= 0.999
-    assert st["iterations_equal"] >= 0.99
-
-
-def test_gpu_ellipsoidal_matches_oracle(plain_gpu_sample):
-    _say("case: ellipsoidal 9x9, 60 frames")
-    frames, _, cands = plain_gpu_sample
-    frames, cands = frames[:60], cands[cands["frame"] < 60]
-    with native.Context(abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4) as ctx:
-        got = ctx.fit(frames, cands)
-    ref = po.fit_batch(frames, cands, abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4)
-    st = compare(ref, got, abi.MODEL_ELLIPSOIDAL)
-    _say(format_stats("gpu vs oracle ellipsoidal", st))
-    assert st["status_agreement"] >= 0.995
-    ok = (ref["lm_status"] == 0) & (got["lm_status"] == 0) & (ref["status"] == 0) & (got["status"] == 0)
-    dxy = np.abs(ref["params"][ok][:, 3:5] - got["params"][ok][:, 3:5]).max(axis=1)
-    assert np.mean(dxy <= 1e-3) >= 0.995
+    assert_parity(st, abi.MODEL_SYMMETRIC)
+    assert st["iterations_equal"] >= 0.999
 
 
 def test_gpu_symmetric_psf_filter(plain_gpu_sample):
-    _say("case: ellipsoidal + symmetric-PSF filter")
+    _say("case: ellipsoidal + symmetric-PSF filter, 10000 fits")
     frames, _, cands = plain_gpu_sample
-    frames, cands = frames[:40], cands[cands["frame"] < 40]
     with native.Context(abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF, 1.35, default_half_width=4) as ctx:
         got = ctx.fit(frames, cands)
     ref = po.fit_batch(frames, cands, abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF, 1.35, default_half_width=4)
-    assert np.mean(ref["status"] == got["status"]) >= 0.995
+    st = compare(ref, got, abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF)
+    _say(format_stats("gpu vs oracle symmetric-PSF filter", st))
+    spread = compare(ref, oracle_sigma_ulp(po, frames, cands, abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF, 1.35,
+                                           default_half_width=4), abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF)
+    assert_parity(st, abi.MODEL_ELLIPSOIDAL_SYMMETRIC_PSF, spread=spread)
     assert set(np.unique(got["status"])) <= {0, 2, 3, 4, 5, 6, 7}
+    assert {abi.REJECT_ANISOTROPIC, abi.REJECT_PSF_WIDTH} & set(np.unique(got["status"]))
 
 
 def test_gpu_fixed_width_matches_oracle(plain_gpu_sample):
@@ -105,10 +103,7 @@ def test_gpu_fixed_width_matches_oracle(plain_gpu_sample):
     ref = po.fit_batch(frames, cands, abi.MODEL_FIXED_WIDTH, 1.35, default_half_width=4)
     st = compare(ref, got, abi.MODEL_FIXED_WIDTH)
     _say(format_stats("gpu vs oracle fixed width", st))
-    assert st["status_agreement"] >= 0.999, st
-    assert st["xy_frac_within_1e-3"] >= 0.999, st
-    for k in ("A", "b"):
-        assert st[f"{k}_frac_within_1e-4"] >= 0.995, (k, st)
+    assert_parity(st, abi.MODEL_FIXED_WIDTH)
     assert set(np.unique(got["status"])) <= {0, 2, 3, 9}
 
 
@@ -117,7 +112,10 @@ def plain_100k():
     w = synth.PLAIN_GPU_256.scaled(2000)
     frames, _, cands = synth.generate(w, nthreads=16)
     ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.35, default_half_width=4)
-    return frames, cands, ref
+    with native.Context(abi.MODEL_SYMMETRIC, 1.35, default_half_width=4) as ctx:
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, 0)
+        unsplit = ctx.fit(frames, cands)
+    return frames, cands, ref, unsplit
 
 
 @pytest.mark.parametrize("trials", [4, 11, 17, 24, 48])
@@ -127,8 +125,9 @@ def test_gpu_straggler_group_sizes(plain_100k, trials):
     after 4 / 11 / 17 / 24 / 48 trials about 99k / 46k / 25k / 13k / 1.7k of the 100k fits
     are handed off, which selects one lane per fit (the two-wave build; this model skips
     2-lane groups) for the first two and groups of 4, 8 and 16 lanes for the others.
-    Every group size must match the oracle, and repeat calls must be bit-identical."""
-    frames, cands, ref = plain_100k
+    Every group size must give the fits of a launch without hand-off bit for bit, and match
+    the oracle."""
+    frames, cands, ref, unsplit = plain_100k
     _say(f"case: symmetric 9x9, 100k fits, hand-off after {trials} trials")
     with native.Context(abi.MODEL_SYMMETRIC, 1.35, default_half_width=4) as ctx:
         ctx.set_option(abi.OPT_STRAGGLER_TRIALS, trials)
@@ -137,42 +136,64 @@ def test_gpu_straggler_group_sizes(plain_100k, trials):
     handed = int(np.sum(ref["trials"] > trials))
     st = compare(ref, got, abi.MODEL_SYMMETRIC)
     _say(format_stats(f"gpu vs oracle, ~{handed} fits handed off", st))
-    _check_sym(st)
+    diff = _bytes_differ(got, unsplit)
+    _say(f"records differing from the launch without hand-off: {int(diff.sum())}")
+    assert not diff.any()
     assert np.array_equal(got.view(np.uint8), again.view(np.uint8))
+    _check_sym(st)
 
 
 def test_gpu_mixed_models_and_windows():
-    _say("case: mixed models (symmetric, ellipsoidal, fixed width) and windows h=2..7")
-    w = synth.MIXED_2M.scaled(40)
+    _say("case: mixed models (symmetric, ellipsoidal, fixed width) and windows h=2..7, 20k fits")
+    w = synth.MIXED_2M.scaled(400)
     frames, _, cands = synth.generate(w)
     cands["model"][::7] = abi.MODEL_FIXED_WIDTH
     with native.Context(abi.MODEL_SYMMETRIC, 1.5) as ctx:
         ctx.set_option(abi.OPT_MAX_HALF_WIDTH, 7)
         got = ctx.fit(frames, cands)
     ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.5)
+    other = oracle_sigma_ulp(po, frames, cands, abi.MODEL_SYMMETRIC, 1.5)
     assert np.all(got["model"] == ref["model"])
     for m in (abi.MODEL_SYMMETRIC, abi.MODEL_ELLIPSOIDAL, abi.MODEL_FIXED_WIDTH):
         sel = cands["model"] == m
         st = compare(ref[sel], got[sel], m)
         _say(format_stats(f"gpu vs oracle mixed model {m}", st))
-        assert st["status_agreement"] >= 0.99
+        assert_parity(st, m, spread=compare(ref[sel], other[sel], m))
 
 
 @pytest.fixture(scope="module")
 def ellipsoidal_100k(plain_100k):
-    frames, cands, _ = plain_100k
+    frames, cands, _, _ = plain_100k
     ref = po.fit_batch(frames, cands, abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4)
-    return frames, cands, ref
+    spread = compare(ref, oracle_sigma_ulp(po, frames, cands, abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4),
+                     abi.MODEL_ELLIPSOIDAL)
+    with native.Context(abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4) as ctx:
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, 0)
+        unsplit = ctx.fit(frames, cands)
+    return frames, cands, ref, unsplit, spread
+
+
+def test_gpu_ellipsoidal_matches_oracle(ellipsoidal_100k):
+    """The ellipsoidal model on 100k 9x9 fits (default hand-off) to the parity bar, with the
+    reference's own reproducibility measured on the same fits."""
+    frames, cands, ref, unsplit, spread = ellipsoidal_100k
+    _say("case: ellipsoidal 9x9, 100k fits")
+    with native.Context(abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4) as ctx:
+        got = ctx.fit(frames, cands)
+    st = compare(ref, got, abi.MODEL_ELLIPSOIDAL)
+    _say(format_stats("gpu vs oracle ellipsoidal", st))
+    assert_parity(st, abi.MODEL_ELLIPSOIDAL, spread=spread)
+    assert not _bytes_differ(got, unsplit).any()
 
 
 @pytest.mark.parametrize("trials", [3, 8, 14, 23, 48])
 def test_gpu_ellipsoidal_straggler_group_sizes(ellipsoidal_100k, trials):
-    """The ellipsoidal straggler pass (rows of the window split over the group, moments added
-    by the butterfly) at every group size: after 3 / 8 / 14 / 23 / 48 trials about 100k /
+    """The ellipsoidal straggler pass (rows of the window split over the group, their sums
+    folded in row order) at every group size: after 3 / 8 / 14 / 23 / 48 trials about 100k /
     42k / 25k / 13k / 3k of the 100k fits are handed off (groups of 1, 2, 4, 8, 16 lanes on
-    a 256-CU MI355X).  Against the oracle as the ellipsoidal parity test, repeat calls
-    bit-identical."""
-    frames, cands, ref = ellipsoidal_100k
+    a 256-CU MI355X).  Bit-identical to the launch without hand-off, and against the oracle as
+    the ellipsoidal parity test."""
+    frames, cands, ref, unsplit, spread = ellipsoidal_100k
     _say(f"case: ellipsoidal 9x9, 100k fits, hand-off after {trials} trials")
     with native.Context(abi.MODEL_ELLIPSOIDAL, 1.35, default_half_width=4) as ctx:
         ctx.set_option(abi.OPT_STRAGGLER_TRIALS, trials)
@@ -180,11 +201,11 @@ def test_gpu_ellipsoidal_straggler_group_sizes(ellipsoidal_100k, trials):
         again = ctx.fit(frames, cands)
     st = compare(ref, got, abi.MODEL_ELLIPSOIDAL)
     _say(format_stats(f"gpu vs oracle ellipsoidal, ~{int(np.sum(ref['trials'] > trials))} fits handed off", st))
-    assert st["status_agreement"] >= 0.995
-    ok = (ref["lm_status"] == 0) & (got["lm_status"] == 0) & (ref["status"] == 0) & (got["status"] == 0)
-    dxy = np.abs(ref["params"][ok][:, 3:5] - got["params"][ok][:, 3:5]).max(axis=1)
-    assert np.mean(dxy <= 1e-3) >= 0.995
+    diff = _bytes_differ(got, unsplit)
+    _say(f"records differing from the launch without hand-off: {int(diff.sum())}")
+    assert not diff.any()
     assert np.array_equal(got.view(np.uint8), again.view(np.uint8))
+    assert_parity(st, abi.MODEL_ELLIPSOIDAL, spread=spread)
 
 
 def test_gpu_mixed_windows_bit_identical():
@@ -246,3 +267,48 @@ def test_gpu_device_entry_matches_host_entry(plain_cpu):
     dev = d_out.cpu().numpy().view(abi.FIT_DTYPE)
     assert np.array_equal(dev["status"], host["status"])
     assert np.array_equal(dev["params"], host["params"])
+
+
+def test_gpu_results_independent_of_batching():
+    """The mixed workload (windows 5x5 to 15x15, three models) fitted as one call and as four
+    blocks of frames: every record bit-identical (a fit's result depends on its own spot only,
+    source/Localization.cpp:107-212 fits each particle on its own)."""
+    _say("case: mixed windows and models, 80k fits, one call vs four blocks")
+    w = synth.MIXED_2M.scaled(1600)
+    frames, _, cands = synth.generate(w, nthreads=16)
+    cands["model"][::7] = abi.MODEL_FIXED_WIDTH
+    with native.Context(abi.MODEL_SYMMETRIC, 1.5) as ctx:
+        ctx.set_option(abi.OPT_MAX_HALF_WIDTH, 7)
+        whole = ctx.fit(frames, cands)
+        parts = []
+        edges = np.linspace(0, cands.size, 5).astype(int)
+        for a, b in zip(edges[:-1], edges[1:]):
+            parts.append(ctx.fit(frames, cands[a:b]))
+    blocks = np.concatenate(parts)
+    diff = _bytes_differ(whole, blocks)
+    _say(f"records differing between one call and four blocks: {int(diff.sum())}")
+    assert not diff.any()
+
+
+@pytest.fixture(scope="module")
+def mixed_100k():
+    w = synth.MIXED_2M.scaled(2000)
+    frames, _, cands = synth.generate(w, nthreads=16)
+    ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.5)
+    other = oracle_sigma_ulp(po, frames, cands, abi.MODEL_SYMMETRIC, 1.5)
+    return frames, cands, ref, other
+
+
+def test_gpu_mixed_100k_matches_oracle(mixed_100k):
+    """The batched uneven workload (h ~ U{2..7}, 60 % symmetric / 40 % ellipsoidal) at 100k
+    ROIs in one call: each model class to the parity bar."""
+    frames, cands, ref, other = mixed_100k
+    _say(f"case: mixed 100k, {cands.size} fits")
+    with native.Context(abi.MODEL_SYMMETRIC, 1.5) as ctx:
+        ctx.set_option(abi.OPT_MAX_HALF_WIDTH, 7)
+        got = ctx.fit(frames, cands)
+    for m in (abi.MODEL_SYMMETRIC, abi.MODEL_ELLIPSOIDAL):
+        sel = got["model"] == m
+        st = compare(ref[sel], got[sel], m)
+        _say(format_stats(f"gpu vs oracle mixed 100k model {m}", st))
+        assert_parity(st, m, spread=compare(ref[sel], other[sel], m))

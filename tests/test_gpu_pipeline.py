@@ -32,6 +32,16 @@ def _oracle_pipeline(frames, w, pfa, model):
     return rows
 
 
+def _record_row(k, r, model):
+    """The reference matrix row (source/Storage.cpp:268-290 / :614-640) of an oracle fit."""
+    p = r["params"]
+    d = r["deviations"]
+    if model == abi.MODEL_ELLIPSOIDAL:
+        return [k, r["integral"], r["sd1"], r["sd2"], p[3], p[4], r["theta"], p[6], r["integral_deviation"],
+                r["sd1_deviation"], r["sd2_deviation"], d[3], d[4], 0.0, d[6], 1]
+    return [k, r["integral"], p[1], p[2], p[3], p[4], r["integral_deviation"], d[1], d[2], d[3], d[4], 1]
+
+
 @pytest.mark.parametrize("model", [abi.MODEL_ELLIPSOIDAL, abi.MODEL_SYMMETRIC])
 def test_gpu_pipeline_matches_oracle_pipeline(model):
     _say(f"case: pipeline model {model}, 5 frames 192x160, w 1.3, PFA 20")
@@ -43,15 +53,32 @@ def test_gpu_pipeline_matches_oracle_pipeline(model):
     assert m.shape[1] == (16 if model == abi.MODEL_ELLIPSOIDAL else 12)
     assert np.all(np.diff(m[:, 0]) >= 0)  # sorted by frame
     assert abs(len(m) - len(ref)) <= max(1, 0.01 * len(ref))
+    # the finders are bit-exact, so both pipelines fit the same candidates: pair each oracle row
+    # with the GPU row of its frame nearest in position, then compare every column
     xi = (4, 5) if model == abi.MODEL_ELLIPSOIDAL else (3, 4)
-    pi = (3, 4) if model == abi.MODEL_ELLIPSOIDAL else (2, 3)
-    got = {(int(r[0]), round(r[xi[0]], 2), round(r[xi[1]], 2)) for r in m}
-    hits = 0
-    for k, r in ref:
-        key = (k, round(r["params"][pi[0]], 2), round(r["params"][pi[1]], 2))
-        hits += key in got
-    _say(f"pipeline: {hits} of {len(ref)} oracle rows found")
-    assert hits >= 0.98 * len(ref)
+    mat = np.array([_record_row(k, r, model) for k, r in ref])
+    paired, misses = 0, {}
+    for row in mat:
+        cand = m[m[:, 0] == row[0]]
+        if cand.size == 0:
+            continue
+        d = np.hypot(cand[:, xi[0]] - row[xi[0]], cand[:, xi[1]] - row[xi[1]])
+        g = cand[np.argmin(d)]
+        if d.min() > 0.5:
+            continue
+        paired += 1
+        for c in range(1, m.shape[1]):
+            if c in xi:
+                ok = abs(g[c] - row[c]) <= 1e-3
+            elif model == abi.MODEL_ELLIPSOIDAL and c in (6, 13):  # theta, thetaDeviation: absolute
+                ok = abs(g[c] - row[c]) <= 1e-4
+            else:
+                ok = abs(g[c] - row[c]) <= 1e-4 * abs(row[c]) or g[c] == row[c]
+            misses[c] = misses.get(c, 0) + (not ok)
+    _say(f"pipeline: {paired} of {len(ref)} oracle rows paired; misses per column {misses}")
+    assert paired >= len(ref) - 1
+    # about 200 rows: at most one miss per column (the 0.1 % bar at scale: tests/test_gpu_parity.py)
+    assert max(misses.values()) <= 1, misses
     if model == abi.MODEL_ELLIPSOIDAL:
         n2, cols = None, None
         import ctypes

@@ -201,3 +201,38 @@ def test_ellipsoidal_direct_fallback_and_masking():
     _check_ell(theta, np.zeros(7), 1.0, small, nloop=15, rtol=1e-10, mtol=1e-11)
     for a, b in zip(pyemu.ell_pass(theta, np.zeros(7), 1.0, small, 5), pyemu.ell_pass(theta, np.zeros(7), 1.0, small, 15)):
         assert np.array_equal(np.atleast_1d(a), np.atleast_1d(b))
+
+
+@pytest.mark.parametrize("n", [5, 9, 11, 15])
+def test_group_pass_bit_identical_to_one_lane(n):
+    """A straggler group of k lanes splits a window's rows over its lanes and folds their
+    results in row order after an exchange (lzr_fit.hip straggler_groups).  Emulated lane by
+    lane on the host, every k in 1..16 must give the one-lane pass's sums bit for bit, for the
+    separable and the ellipsoidal passes, including the direct-exponential fallbacks and a
+    residual overflow (|rho| > 1) whose count the ellipsoidal pass carries."""
+    rng = np.random.default_rng(40 + n)
+    c = (n - 1) / 2
+    for trial in range(12):
+        t5 = [rng.uniform(20, 3000), rng.uniform(0.6, 2.2), c + rng.uniform(-1.5, 1.5), c + rng.uniform(-1.5, 1.5),
+              rng.uniform(50, 200)]
+        if trial == 0:
+            t5[1] = 0.03  # collapsed width: per-column exponentials
+        win = _window(rng, n, *t5)
+        dx5 = rng.normal(0, [30, 0.05, 0.2, 0.2, 3])
+        one = pyemu.sep_pass_flat(t5, dx5, 1.0, 1.35, win)
+        t4 = [t5[0], t5[2], t5[3], t5[4]]
+        one4 = pyemu.sep_pass_flat(t4, dx5[[0, 2, 3, 4]], 1.0, 1.35, win)
+        t7 = [rng.uniform(100, 3000), rng.uniform(0.8, 2.2), rng.uniform(0.8, 2.2), c + rng.uniform(-1.5, 1.5),
+              c + rng.uniform(-1.5, 1.5), rng.uniform(-0.6, 0.6), rng.uniform(50, 200)]
+        if trial == 1:
+            t7[5] = 0.995  # unfactored
+        if trial == 2:
+            t7[5] = -1.02  # |rho| > 1: overflowing residuals
+        wel = _window_ell(rng, n, *[t7[0], t7[1], t7[2], t7[3], t7[4], min(abs(t7[5]), 0.9), t7[6]])
+        dx7 = rng.normal(0, [30, 0.05, 0.05, 0.2, 0.2, 0.01, 3])
+        one7 = pyemu.ell_pass_flat(t7, dx7, 1.0, wel)
+        for k in (1, 2, 4, 8, 16):
+            assert np.array_equal(pyemu.sep_pass_group(t5, dx5, 1.0, 1.35, win, k), one, equal_nan=True), (trial, k)
+            assert np.array_equal(pyemu.sep_pass_group(t4, dx5[[0, 2, 3, 4]], 1.0, 1.35, win, k), one4,
+                                  equal_nan=True), (trial, k)
+            assert np.array_equal(pyemu.ell_pass_group(t7, dx7, 1.0, wel, k), one7, equal_nan=True), (trial, k)
