@@ -178,14 +178,14 @@ __global__ void __launch_bounds__(kSortThreads) scatter_kernel(const LzrCandidat
         if (key[k] >= 0) order[base[key[k]] + local[k]] = first + k * kSortThreads;
 }
 
-/* Turns the raw end states into the reference records (post-checks; for accepted fits J^T J
- * rebuilt at the final point from the window and its FP64 covariance; 2x2
+/* Turns the raw end states into the reference records (post-checks; for accepted fits the
+ * covariance from a QR factor of the reference Jacobian at the final point; 2x2
  * eigen-decomposition), one thread per candidate. */
-__global__ void epilogue_kernel(const uint16_t *frames, int nx, int ny, const LzrCandidate *cands, int ncands,
-                                const int *d_count, LzrFit *out, LaneConfig cfg) {
+__global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, const int *d_count, LzrFit *out,
+                                LaneConfig cfg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= live_count(d_count, ncands)) return;
-    finish_any(out[i], cfg, cands[i], frames, nx, ny);
+    finish_any(out[i], cfg, cands[i]);
 }
 
 __global__ void reset_counters(unsigned *counters, int n) {
@@ -933,8 +933,8 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
         LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
         LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
     }
-    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, frames, nx, ny, cands, ncands,
-                       d_count, out, ctx->cfg);
+    hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
+                       ctx->cfg);
     LZR_CHECK(hipGetLastError());
     LZR_CHECK(ctx->order.end(stream));
 }

@@ -1013,6 +1013,75 @@ LZR_HD LZR_INLINE void ell_finish(const EllIn &q, int npix, const double (&s)[kE
     a.bad = s[kEBad] == 0.0 ? 0 : (s[kEBad] == 1.0 ? 1 : 2);
 }
 
+/* The reference Jacobian at t, one row (pixel (i, j) of the window, window coordinates) at
+ * a time: source/Localization.cpp:1128-1181 (symmetric, width column / sqrt(2)), :1183-1225
+ * (fixed width), :1227-1295 (ellipsoidal, the rho column's quirk included).  The reciprocals
+ * are formed once (is = 1/sigma, w = the fixed width). */
+template <int P>
+struct JacRows {
+    double t[P];
+    double is, r0, r1, r2, r3;
+    LZR_HD LZR_INLINE void init(const double (&th)[P], double is_, double w) {
+        LZR_NO_CONTRACT
+#pragma unroll
+        for (int c = 0; c < P; ++c) t[c] = th[c];
+        is = is_;
+        r0 = r1 = r2 = r3 = 0.0;
+        if constexpr (P == 4) {
+            r0 = 1.0 / (w * w); /* 1/w^2 */
+        } else if constexpr (P == 5) {
+            r0 = 1.0 / t[1]; /* 1/r */
+            r1 = r0 * r0;    /* 1/r^2 */
+        } else {
+            r0 = 1.0 / t[1];                /* 1/sx */
+            r1 = 1.0 / t[2];                /* 1/sy */
+            r2 = 1.0 / (1.0 - t[5] * t[5]); /* 1/(1 - rho^2) */
+            r3 = 0.5 * r2 * is;             /* 1/den */
+        }
+    }
+    LZR_HD LZR_INLINE void row(int i, int j, double (&o)[P]) const {
+        LZR_NO_CONTRACT
+        if constexpr (P == 4) {
+            const double A = t[0];
+            const double ux = i - t[1], uy = j - t[2];
+            const double e = dexp(-(ux * ux + uy * uy) * (0.5 * r0));
+            const double ae = A * e * r0 * is;
+            o[0] = e * is;
+            o[1] = ae * ux;
+            o[2] = ae * uy;
+            o[3] = is;
+        } else if constexpr (P == 5) {
+            const double A = t[0];
+            const double ux = i - t[2], uy = j - t[3];
+            const double d2 = ux * ux + uy * uy;
+            const double e = dexp(-d2 * (0.5 * r1));
+            const double ae = A * e * r1 * is;
+            o[0] = e * is;
+            o[1] = ae * d2 * r0 * (1.0 / kSqrt2);
+            o[2] = ae * ux;
+            o[3] = ae * uy;
+            o[4] = is;
+        } else {
+            const double A = t[0], rho = t[5];
+            const double ux = i - t[3], uy = j - t[4];
+            const double vx = ux * r0, vy = uy * r1; /* ux/sx, uy/sy */
+            const double q = vx * vx + vy * vy - 2.0 * rho * vx * vy;
+            const double e = dexp(-q * (0.5 * r2));
+            const double aed = A * e * r3;
+            o[0] = e * is;
+            o[1] = -aed * (2.0 * rho * vx * vy - 2.0 * vx * vx) * r0;
+            o[2] = -aed * (2.0 * rho * vx * vy - 2.0 * vy * vy) * r1;
+            o[3] = -aed * (2.0 * rho * vy - 2.0 * vx) * r0;
+            o[4] = -aed * (2.0 * rho * vx - 2.0 * vy) * r1;
+            /* the reference rho column divides the unscaled quadratic form by sx sy
+             * (exact only for sx = sy, source/Localization.cpp:1280) */
+            const double qr = (ux * ux + uy * uy) * r0 * r1 - 2.0 * rho * vx * vy;
+            o[5] = A * e * is * r2 * (vx * vy - rho * qr * r2);
+            o[6] = is;
+        }
+    }
+};
+
 /* Column norms of the first `rows` Jacobian rows at t (GSL 1.16 update_diag reads the
  * first p rows of J).  Pixel k = i + n j, window coordinates; w is the fixed width. */
 template <int P>
@@ -1021,59 +1090,11 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
     LZR_NO_CONTRACT
 #pragma unroll
     for (int c = 0; c < P; ++c) cn[c] = 0.0;
-    /* reciprocals once per call (is = 1/sigma); the rows then take products only */
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    if constexpr (P == 4) {
-        r0 = 1.0 / (w * w); /* 1/w^2 */
-    } else if constexpr (P == 5) {
-        r0 = 1.0 / t[1];    /* 1/r */
-        r1 = r0 * r0;       /* 1/r^2 */
-    } else {
-        r0 = 1.0 / t[1];                    /* 1/sx */
-        r1 = 1.0 / t[2];                    /* 1/sy */
-        r2 = 1.0 / (1.0 - t[5] * t[5]);     /* 1/(1 - rho^2) */
-        r3 = 0.5 * r2 * is;                 /* 1/den */
-    }
+    JacRows<P> jr;
+    jr.init(t, is, w);
     for (int k = 0, i = 0, j = 0; k < rows; ++k) {
         double row[P];
-        if constexpr (P == 4) {
-            const double A = t[0];
-            const double ux = i - t[1], uy = j - t[2];
-            const double e = dexp(-(ux * ux + uy * uy) * (0.5 * r0));
-            const double ae = A * e * r0 * is;
-            row[0] = e * is;
-            row[1] = ae * ux;
-            row[2] = ae * uy;
-            row[3] = is;
-        } else if constexpr (P == 5) {
-            const double A = t[0];
-            const double ux = i - t[2], uy = j - t[3];
-            const double d2 = ux * ux + uy * uy;
-            const double e = dexp(-d2 * (0.5 * r1));
-            const double ae = A * e * r1 * is;
-            row[0] = e * is;
-            row[1] = ae * d2 * r0 * (1.0 / kSqrt2);
-            row[2] = ae * ux;
-            row[3] = ae * uy;
-            row[4] = is;
-        } else {
-            const double A = t[0], rho = t[5];
-            const double ux = i - t[3], uy = j - t[4];
-            const double vx = ux * r0, vy = uy * r1; /* ux/sx, uy/sy */
-            const double q = vx * vx + vy * vy - 2.0 * rho * vx * vy;
-            const double e = dexp(-q * (0.5 * r2));
-            const double aed = A * e * r3;
-            row[0] = e * is;
-            row[1] = -aed * (2.0 * rho * vx * vy - 2.0 * vx * vx) * r0;
-            row[2] = -aed * (2.0 * rho * vx * vy - 2.0 * vy * vy) * r1;
-            row[3] = -aed * (2.0 * rho * vy - 2.0 * vx) * r0;
-            row[4] = -aed * (2.0 * rho * vx - 2.0 * vy) * r1;
-            /* the reference rho column divides the unscaled quadratic form by sx sy
-             * (exact only for sx = sy, source/Localization.cpp:1280) */
-            const double qr = (ux * ux + uy * uy) * r0 * r1 - 2.0 * rho * vx * vy;
-            row[5] = A * e * is * r2 * (vx * vy - rho * qr * r2);
-            row[6] = is;
-        }
+        jr.row(i, j, row);
 #pragma unroll
         for (int c = 0; c < P; ++c) cn[c] = dfma(row[c], row[c], cn[c]);
         /* pixel k + 1 = i + n j, without a division */
@@ -1508,75 +1529,86 @@ LZR_HD LZR_INLINE double dsin(double v) {
 #endif
 }
 
-/* Diagonal of (J^T J)^-1 at the final point (gsl_multifit_covar(J, 0.0, C) diagonal), by
- * Cholesky in FP64.  Returns false (and +inf) if J^T J is numerically singular. */
+/* Diagonal of the covariance gsl_multifit_covar(J, 0.0, C) (source/Localization.cpp:185, :556)
+ * of the reference Jacobian at the window-coordinate point t of an n x n window: J is reduced
+ * row by row (pixel k = i + n j) to its triangular QR factor R by Givens rotations, which is
+ * backward stable like GSL's QRPT, so the covariance keeps GSL's accuracy of about
+ * cond(J) ulps instead of the cond(J)^2 of inverting J^T J.  As GSL with epsrel = 0, a
+ * column whose pivot is exactly 0 (an exactly-zero Jacobian column, e.g. an amplitude
+ * factor that underflows everywhere) is dropped: its covariance entries are 0 and the others
+ * come from the remaining columns. */
 template <int P>
-LZR_HD LZR_INLINE bool covariance_diagonal(const double (&H)[NP<P>::S], double (&cdiag)[P]) {
+LZR_HD LZR_INLINE void covariance_qr(const double (&t)[P], int n, double is, double w, double (&cdiag)[P]) {
     LZR_NO_CONTRACT
-    double L[NP<P>::S], id[P];
-    if (!cholesky<P>(H, L, id)) {
-        /* numerically singular: the parameters are undetermined (GSL's QRPT covariance gives
-         * rounding-noise values of 1e10 and more there) */
+    constexpr int S = NP<P>::S;
+    JacRows<P> jr;
+    jr.init(t, is, w);
+    double R[S]; /* upper triangle, R_km at tri(k, m) */
 #pragma unroll
-        for (int i = 0; i < P; ++i) cdiag[i] = __builtin_inf();
-        return false;
-    }
-    /* C = L^-T L^-1; C_kk = sum_i (L^-1)_{ik}^2 */
+    for (int k = 0; k < S; ++k) R[k] = 0.0;
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double row[P];
+            jr.row(i, j, row);
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const double b = row[k];
+                if (b == 0.0) continue;
+                const double a = R[tri(k, k)];
+                /* r = hypot(a, b) scaled by the larger magnitude: entries far out in the window
+                 * are tiny (exponentials near underflow), their squares would vanish */
+                const double aa = dabs(a), ab = dabs(b);
+                const double mx = dmax(aa, ab), q = dmin(aa, ab) * drcp(mx);
+                const double r = mx * dsqrt(dfma(q, q, 1.0));
+                const double ir = drcp(r);
+                const double c = a * ir, sn = b * ir;
+                R[tri(k, k)] = r;
+#pragma unroll
+                for (int m = k + 1; m < P; ++m) {
+                    const double rm = R[tri(k, m)];
+                    R[tri(k, m)] = dfma(c, rm, sn * row[m]);
+                    row[m] = dfma(c, row[m], -(sn * rm));
+                }
+            }
+        }
+    /* C = R^-1 R^-T over the columns with a non-zero pivot: C_kk = sum_m (R^-1)_km^2 */
+    bool on[P];
+    double id[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        double e[P], y[P];
-#pragma unroll
-        for (int i = 0; i < P; ++i) e[i] = i == k ? 1.0 : 0.0;
-        forward<P>(L, id, e, y);
-        double s2 = 0.0;
-#pragma unroll
-        for (int i = 0; i < P; ++i) s2 = dfma(y[i], y[i], s2);
-        cdiag[k] = s2;
+        on[k] = R[tri(k, k)] != 0.0;
+        id[k] = on[k] ? 1.0 / R[tri(k, k)] : 0.0;
+        cdiag[k] = 0.0;
     }
-    return true;
-}
-
-/* J^T J at the window-coordinate point t (the same pass, in the same order, as the lane's
- * evaluation at that point: a fit's final J^T J, bit for bit).  pix.raw(i, j) reads pixel (i, j)
- * of the n x n window. */
-template <int P, typename Pix>
-LZR_HD LZR_INLINE void normal_matrix(const double (&t)[P], const LaneConfig &cfg, int n, const Pix &pix,
-                                     double (&H)[NP<P>::S]) {
-    LZR_NO_CONTRACT
-    double zero[P], g[P];
 #pragma unroll
-    for (int i = 0; i < P; ++i) zero[i] = 0.0;
-    struct {
-        double v[2 * LZR_MAX_HALF_WIDTH + 1];
-        LZR_HD void set(int i, double x) { v[i] = x; }
-        LZR_HD double get(int i) const { return v[i]; }
-    } xb;
-    PassAcc<P, double> a;
-    a.zero();
-    if constexpr (P == 7) {
-        EllIn q;
-        make_ell_in(t, zero, cfg.inv_sigma, q);
-        double v[kEllSums];
-        ell_pass(q, n, n, false, pix, xb, v);
-        ell_finish<double>(q, n * n, v, true, 0.0, a);
-    } else {
-        SepIn q;
-        make_sep_in(t, zero, cfg.inv_sigma, cfg.initial_psf_width, q);
-        double v[kSepSums];
-        sep_pass(q, n, n, false, pix, xb, v);
-        sep_finish<P, double>(q, n * n, v, true, 0.0, a);
+    for (int m = 0; m < P; ++m) {
+        if (!on[m]) continue;
+        double x[P]; /* column m of R^-1 (rows > m are 0) */
+#pragma unroll
+        for (int k = P - 1; k >= 0; --k) {
+            double v = k == m ? 1.0 : 0.0;
+            if (k < m && on[k]) {
+#pragma unroll
+                for (int q = k + 1; q <= m; ++q)
+                    if (on[q]) v = dfma(-R[tri(k, q)], x[q], v);
+            }
+            x[k] = k <= m && on[k] ? v * id[k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k <= m; ++k) cdiag[k] = dfma(x[k], x[k], cdiag[k]);
     }
-    assemble<P, double>(a, n * n, cfg.inv_sigma, H, g);
+    /* a numerically singular factor (a pivot near underflow) leaves inf * 0 in the inverse:
+     * the variance is unbounded, as GSL reports it there */
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        if (cdiag[k] != cdiag[k]) cdiag[k] = __builtin_inf();
 }
 
 /* Post-checks and output record of the reference drivers
  * (source/Localization.cpp:157-211 symmetric, :336-378 fixed width, :506-596 ellipsoidal,
- * :606-631 filter), from the raw end state r of candidate c; pix reads the candidate's window
- * (for the covariance of accepted fits: gsl_multifit_covar(J, 0.0, C), :185, :556, from
- * J^T J in FP64). */
-template <int P, typename Pix>
-LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, const Pix &pix,
-                                     LzrFit &out) {
+ * :606-631 filter), from the raw end state r of candidate c. */
+template <int P>
+LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, LzrFit &out) {
     LZR_NO_CONTRACT
     const int model = r.model;
     const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
@@ -1591,7 +1623,6 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
     x[x_index<P>()] += xo;
     x[x_index<P>() + 1] += yo;
     const double chi = r.chi;
-    double H[NP<P>::S];
     clear_record(out);
     out.iterations = r.iterations;
     out.lm_status = r.lm_status;
@@ -1616,8 +1647,7 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        normal_matrix<P>(t, cfg, n, pix, H);
-        covariance_diagonal<P>(H, cd);
+        covariance_qr<P>(t, n, cfg.inv_sigma, w, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 4);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1647,8 +1677,7 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        normal_matrix<P>(t, cfg, n, pix, H);
-        covariance_diagonal<P>(H, cd);
+        covariance_qr<P>(t, n, cfg.inv_sigma, w, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 5);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1687,8 +1716,7 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        normal_matrix<P>(t, cfg, n, pix, H);
-        covariance_diagonal<P>(H, cd);
+        covariance_qr<P>(t, n, cfg.inv_sigma, w, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1720,26 +1748,17 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
 }
 
 /* Epilogue entry: finishes a record written by store_final (no-op for records already
- * final, i.e. rejected before fitting).  frames: the stack (nx, ny) the fit read. */
-template <typename FrameT>
-LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c, const FrameT *frames,
-                                  int nx, int ny) {
+ * final, i.e. rejected before fitting). */
+LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c) {
     LzrRawFit r;
     __builtin_memcpy(&r, &rec, sizeof r);
     if (r.status != kPending) return;
-    const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
-    const int xoff = (int)(c.x - (double)h), yoff = (int)(c.y - (double)h);
-    struct {
-        const FrameT *src;
-        int nx;
-        LZR_HD FrameT raw(int i, int j) const { return src[(size_t)j * nx + i]; }
-    } pix{frames + ((size_t)c.frame * ny + yoff) * nx + xoff, nx};
     if (r.model == LZR_MODEL_SYMMETRIC)
-        finish_record<5>(r, cfg, c, pix, rec);
+        finish_record<5>(r, cfg, c, rec);
     else if (r.model == LZR_MODEL_FIXED_WIDTH)
-        finish_record<4>(r, cfg, c, pix, rec);
+        finish_record<4>(r, cfg, c, rec);
     else
-        finish_record<7>(r, cfg, c, pix, rec);
+        finish_record<7>(r, cfg, c, rec);
 }
 
 }  // namespace lzr

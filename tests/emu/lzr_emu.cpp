@@ -47,7 +47,7 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             sep_finish<P, double>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
         }
         if (!lane_after_pass<P, double>(s, cfg, a, model, reinterpret_cast<LzrRawFit *>(&out))) {
-            finish_any(out, cfg, c, frames, nx, ny);
+            finish_any(out, cfg, c);
             return;
         }
     }
