@@ -831,11 +831,17 @@ static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds,
     if (a.strag_trials > 0) {
         const size_t slds = lds + (size_t)xchg_slots(P) * 64 * sizeof(double); /* + the row exchange */
         constexpr int kRole = P == 7 ? kAllGroups : kMultiLane;
+        /* Groups run latency-bound chains: one wave per SIMD, the LDS request reserving a quarter
+         * of the CU's LDS so that no second wave shares a SIMD (with fewer registers than the
+         * register file holds twice over, the occupancy query alone would allow two, and two
+         * interleaved waves take ~1.4x longer per trial) */
+        const size_t glds = std::max(slds, ctx->lds_per_cu / 4);
         int per_cu = 0;
-        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1, kRole>, 64, slds));
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1, kRole>, 64, glds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
+        per_cu = std::min(per_cu, 4);
         const unsigned lanes = (unsigned)(per_cu * ctx->num_cus) * 64u; /* sets the group size in both launches */
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole>), dim3(per_cu * ctx->num_cus), dim3(64), slds, stream, a,
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole>), dim3(per_cu * ctx->num_cus), dim3(64), glds, stream, a,
                            cls, lanes);
         if constexpr (P != 7) {
             int per_cu2 = 0;

@@ -228,7 +228,8 @@ LZR_HD LZR_INLINE double dabs(double v) { return v < 0 ? -v : v; }
 LZR_HD LZR_INLINE double dmax(double a, double b) { return a > b ? a : b; }
 LZR_HD LZR_INLINE double dmin(double a, double b) { return a < b ? a : b; }
 
-/* Cholesky M = L L^T (packed); false on a non-positive pivot. */
+/* Cholesky M = L L^T (packed); false on a non-positive pivot.  M and L may be the same array
+ * (column j reads M's column j before writing L's). */
 template <int P>
 LZR_HD LZR_INLINE bool cholesky(const double (&M)[NP<P>::S], double (&L)[NP<P>::S], double (&inv_diag)[P]) {
     LZR_NO_CONTRACT
@@ -357,14 +358,14 @@ LZR_HD LZR_INLINE void lmpar(const double (&H)[NP<P>::S], const double (&g)[P], 
     else if (par < parl)
         par = parl;
     if (par == 0.0 && full_rank) par = ddiv(gnorm, dxnorm);
-    double M[S];
     for (int iter = 1;; ++iter) {
         if (par == 0.0) par = dmax(0.001 * paru, kDblMin);
+        /* H + par D^2, factored in place (the factor overwrites the matrix column by column) */
 #pragma unroll
-        for (int i = 0; i < S; ++i) M[i] = H[i];
+        for (int i = 0; i < S; ++i) L[i] = H[i];
 #pragma unroll
-        for (int i = 0; i < P; ++i) M[tri(i, i)] = dfma(par * diag[i], diag[i], M[tri(i, i)]);
-        if (!cholesky<P>(M, L, id)) {
+        for (int i = 0; i < P; ++i) L[tri(i, i)] = dfma(par * diag[i], diag[i], L[tri(i, i)]);
+        if (!cholesky<P>(L, L, id)) {
             /* (H + par D^2) lost definiteness in rounding: keep the last x */
             break;
         }
@@ -1529,6 +1530,73 @@ LZR_HD LZR_INLINE double dsin(double v) {
 #endif
 }
 
+/* J^T J at the window-coordinate point t of an n x n window from the moment passes the lane
+ * runs (no pixel data is involved: a window of zeros).  The separable models need only the
+ * 1-D moments of the column and row factors (O(n)). */
+template <int P>
+LZR_HD LZR_INLINE void moment_normal_matrix(const double (&t)[P], int n, double is, double w, double (&H)[NP<P>::S]) {
+    LZR_NO_CONTRACT
+    double zero[P], g[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) zero[i] = 0.0;
+    PassAcc<P, double> a;
+    a.zero();
+    if constexpr (P == 7) {
+        struct {
+            LZR_HD unsigned raw(int, int) const { return 0u; }
+        } pix;
+        struct {
+            double v[2 * LZR_MAX_HALF_WIDTH + 1];
+            LZR_HD void set(int i, double x) { v[i] = x; }
+            LZR_HD double get(int i) const { return v[i]; }
+        } xb;
+        EllIn q;
+        make_ell_in(t, zero, is, q);
+        double v[kEllSums];
+        ell_pass(q, n, n, false, pix, xb, v);
+        ell_finish<double>(q, n * n, v, true, 0.0, a);
+    } else {
+        struct {
+            LZR_HD void set(int, double) {}
+        } xb;
+        SepIn q;
+        make_sep_in(t, zero, is, w, q);
+        double v[kSepSums];
+        GaussRec gy;
+        sep_columns(q, n, n, xb, v, gy);
+        for (int j = 0; j < n; ++j) sep_row_factor(q, gy, j, n, v);
+        sep_finish<P, double>(q, n * n, v, true, 0.0, a);
+    }
+    assemble<P, double>(a, n * n, is, H, g);
+}
+
+/* Diagonal of (J^T J)^-1 by Cholesky in FP64, and the estimate max_k H_kk C_kk of the scaled
+ * condition number (within a factor P^2 of cond(D H D), D = diag(H)^-1/2; the relative error
+ * of C_kk is about that times the unit roundoff).  Returns false if the factorisation breaks
+ * down. */
+template <int P>
+LZR_HD LZR_INLINE bool covariance_cholesky(const double (&H)[NP<P>::S], double (&cdiag)[P], double &kappa) {
+    LZR_NO_CONTRACT
+    double L[NP<P>::S], id[P];
+    kappa = __builtin_inf();
+    if (!cholesky<P>(H, L, id)) return false;
+    kappa = 0.0;
+    /* C = L^-T L^-1: C_kk is the squared norm of column k of L^-1 */
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        double e[P], y[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) e[i] = i == k ? 1.0 : 0.0;
+        forward<P>(L, id, e, y);
+        double s2 = 0.0;
+#pragma unroll
+        for (int i = k; i < P; ++i) s2 = dfma(y[i], y[i], s2);
+        cdiag[k] = s2;
+        kappa = dmax(kappa, H[tri(k, k)] * s2);
+    }
+    return true;
+}
+
 /* Diagonal of the covariance gsl_multifit_covar(J, 0.0, C) (source/Localization.cpp:185, :556)
  * of the reference Jacobian at the window-coordinate point t of an n x n window: J is reduced
  * row by row (pixel k = i + n j) to its triangular QR factor R by Givens rotations, which is
@@ -1604,6 +1672,18 @@ LZR_HD LZR_INLINE void covariance_qr(const double (&t)[P], int n, double is, dou
         if (cdiag[k] != cdiag[k]) cdiag[k] = __builtin_inf();
 }
 
+/* The covariance diagonal of gsl_multifit_covar(J, 0.0, C) at t: by Cholesky of J^T J (from the
+ * moment pass) where the condition estimate keeps its error below ~1e-9 relative, otherwise
+ * by the QR factor of J as GSL computes it. */
+constexpr double kCovKappaMax = 1.0e6;
+template <int P>
+LZR_HD LZR_INLINE void covariance(const double (&t)[P], int n, double is, double w, double (&cdiag)[P]) {
+    double H[NP<P>::S], kappa;
+    moment_normal_matrix<P>(t, n, is, w, H);
+    if (covariance_cholesky<P>(H, cdiag, kappa) && kappa <= kCovKappaMax) return;
+    covariance_qr<P>(t, n, is, w, cdiag);
+}
+
 /* Post-checks and output record of the reference drivers
  * (source/Localization.cpp:157-211 symmetric, :336-378 fixed width, :506-596 ellipsoidal,
  * :606-631 filter), from the raw end state r of candidate c. */
@@ -1647,7 +1727,7 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance_qr<P>(t, n, cfg.inv_sigma, w, cd);
+        covariance<P>(t, n, cfg.inv_sigma, w, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 4);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1677,7 +1757,7 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance_qr<P>(t, n, cfg.inv_sigma, w, cd);
+        covariance<P>(t, n, cfg.inv_sigma, w, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 5);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;
@@ -1716,7 +1796,7 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance_qr<P>(t, n, cfg.inv_sigma, w, cd);
+        covariance<P>(t, n, cfg.inv_sigma, w, cd);
         const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
         const double q = chi / dsqrt(dof);
         const double cs = 1.0 > q ? 1.0 : q;

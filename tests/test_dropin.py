@@ -11,7 +11,7 @@ import pytest
 
 from localizer_amd import abi, synth
 from oracle import pyoracle as po
-from tests.parity import assert_parity, compare, format_stats
+from tests.parity import assert_parity, compare, format_stats, oracle_sigma_ulp
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "cpp", "dropin_main")
@@ -118,9 +118,10 @@ def test_dropin_matches_c_abi_and_oracle(tmp_path, model, mode):
     ref = po.fit_batch(frames, cands, model, w_psf)
     st = compare(ref, capi, model)
     print(format_stats(f"drop-in model={model}", st))
-    # 600 fits: the 0.1 % bar is below one fit; at most one miss per metric (the C ABI's
-    # parity at scale is tests/test_gpu_parity.py)
-    assert_parity(st, model, slack=1)
+    # 600 fits: the 0.1 % bar is below one fit; at most one miss per metric beyond the
+    # reference's own spread (the C ABI's parity at scale is tests/test_gpu_parity.py)
+    spread = compare(ref, oracle_sigma_ulp(po, frames, cands, model, w_psf), model)
+    assert_parity(st, model, spread=spread, slack=1)
 
 
 @pytest.mark.gpu
