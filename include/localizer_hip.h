@@ -168,8 +168,9 @@ LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value);
  * once on fixed device buffers (frames, candidates, results) and replayed per frame
  * block: the caller refills the buffers and *d_count (device int, the block's candidate
  * count, at most max_candidates), then launches the graph.  Options set on the context
- * after capture do not affect the graph.  A context's graphs and calls share its work
- * buffers: enqueue them on one stream (or order them) -- overlapping replays race. */
+ * after capture do not affect the graph.  A graph owns its work buffers (sized at creation):
+ * eager calls on the context, of any size, never touch them.  Replays of one graph share
+ * them: enqueue a graph's replays on one stream (or order them) -- overlapping replays race. */
 typedef struct LzrGraph LzrGraph;
 LZR_EXPORT int lzr_graph_create(LzrContext *ctx, LzrGraph **graph, const uint16_t *d_frames, int nx, int ny,
                                 int nframes, const LzrCandidate *d_candidates, const int *d_count,

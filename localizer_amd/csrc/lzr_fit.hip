@@ -57,12 +57,14 @@ struct KernelArgs {
     LzrFit *out;
     LaneConfig cfg;
     /* long fits: a fit still running after strag_trials trials is handed off to the
-     * straggler kernel (0: never) through strag[] (class c's region starts at entry
-     * first_c of kParkedMax bytes, first_c = the queue entries of the classes before c)
-     * and strag_ctr[2c], [2c+1] = (count, head) */
+     * straggler kernel (0: never) through strag[] (entries of kParkedMax bytes; class c's
+     * region is its share of strag_entries, strag_region) and strag_ctr[2c], [2c+1] =
+     * (count, head).  A fit that finds its class's region full stays where it is: a fit's
+     * result does not depend on which kernel runs it. */
     int strag_trials;
     void *strag;
     unsigned *strag_ctr;
+    unsigned strag_entries; /* capacity of strag[] in entries; a class's region is its share */
 };
 
 /* Queue class of a model: 0 symmetric (5 parameters), 1 ellipsoidal and the
@@ -207,12 +209,23 @@ struct Parked {
 constexpr size_t kParkedMax = sizeof(Parked<7>) > sizeof(Parked<5>) ? sizeof(Parked<7>) : sizeof(Parked<5>);
 static_assert(sizeof(Parked<4>) <= kParkedMax, "hand-off entry size");
 
-/* A class's hand-off region. */
+/* A class's hand-off region and its capacity: entries [first_c, first_c + count_c) when the
+ * buffer holds an entry per queued candidate, else the same share of the smaller buffer. */
 template <int P>
-__device__ __forceinline__ Parked<P> *strag_region(const KernelArgs &args, int cls) {
-    unsigned first = 0;
-    for (int c = 0; c < cls; ++c) first += args.count[c];
-    return reinterpret_cast<Parked<P> *>(static_cast<char *>(args.strag) + (size_t)first * kParkedMax);
+__device__ __forceinline__ Parked<P> *strag_region(const KernelArgs &args, int cls, unsigned &cap) {
+    unsigned long long first = 0, total = 0;
+    for (int c = 0; c < kClasses; ++c) {
+        if (c < cls) first += args.count[c];
+        total += args.count[c];
+    }
+    const unsigned long long cnt = args.count[cls], E = args.strag_entries;
+    unsigned long long lo = first, hi = first + cnt;
+    if (E < total) {
+        lo = first * E / total;
+        hi = (first + cnt) * E / total;
+    }
+    cap = (unsigned)(hi - lo);
+    return reinterpret_cast<Parked<P> *>(static_cast<char *>(args.strag) + (size_t)lo * kParkedMax);
 }
 
 /* One round of the fitting kernel.  Round 0 takes fresh candidates from the sorted
@@ -256,6 +269,9 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     int model = 0;
     bool fresh = false; /* this lane's window must be gathered */
     bool open = true;   /* wave-uniform: the queue may still hold work */
+    bool stay = false;  /* the hand-off region was full: this fit finishes here */
+    unsigned strag_cap = 0;
+    Parked<P> *strag = args.strag_trials > 0 ? strag_region<P>(args, cls, strag_cap) : nullptr;
 
     for (;;) {
         /* ---- refill idle lanes from the queue ---- */
@@ -288,8 +304,10 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
                         LzrStart st;
                         if (args.starts) st = args.starts[idx];
                         if (lane_start<P>(s, args.cfg, c, idx, args.nx, args.ny, args.nframes, model, args.out[idx],
-                                          args.starts ? &st : nullptr))
+                                          args.starts ? &st : nullptr)) {
                             fresh = true;
+                            stay = false;
+                        }
                         else
                             s.phase = kIdle;
                     }
@@ -426,7 +444,7 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, reinterpret_cast<LzrRawFit *>(args.out + s.roi)))
                 s.phase = open ? kIdle : kFinished;
             else
-                hand_off = args.strag_trials > 0 && s.trials >= args.strag_trials;
+                hand_off = args.strag_trials > 0 && !stay && s.trials >= args.strag_trials;
         }
         /* ---- hand long fits (a proposed trial pending) to the straggler kernel ---- */
         const unsigned long long hmask = __ballot(hand_off);
@@ -436,10 +454,15 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             if (lane == leader) base = atomicAdd(args.strag_ctr + 2 * cls, (unsigned)__popcll(hmask));
             base = __shfl(base, leader, 64);
             if (hand_off) {
-                Parked<P> &pk = strag_region<P>(args, cls)[base + (unsigned)__popcll(hmask & below)];
-                pk.s = s;
-                pk.model = model;
-                s.phase = open ? kIdle : kFinished;
+                const unsigned q = base + (unsigned)__popcll(hmask & below);
+                if (q < strag_cap) {
+                    Parked<P> &pk = strag[q];
+                    pk.s = s;
+                    pk.model = model;
+                    s.phase = open ? kIdle : kFinished;
+                } else {
+                    stay = true;
+                }
             }
         }
         __syncthreads();
@@ -696,9 +719,10 @@ template <int P, typename Acc, int W, int Role>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     extern __shared__ double lds_words[];
-    const unsigned count = args.strag_ctr[2 * cls];
+    unsigned cap = 0;
+    const Parked<P> *park = strag_region<P>(args, cls, cap);
+    const unsigned count = min(args.strag_ctr[2 * cls], cap); /* hand-offs beyond cap stayed */
     unsigned *head = args.strag_ctr + 2 * cls + 1;
-    const Parked<P> *park = strag_region<P>(args, cls);
     int lk = kMaxGroup == 16 ? 4 : 3; /* log2 of the group size */
     while (lk > 0 && (unsigned long long)count << lk > 2ull * lanes_total) --lk; /* <= 2 fits per group */
     /* The one-lane build runs two waves per SIMD and so holds four times the fits of 2-lane
@@ -735,6 +759,24 @@ using namespace lzr;
 /* C ABI                                                                            */
 /* ------------------------------------------------------------------------------ */
 
+/* Device memory one fit launch works in: the sorted work queue and its counters, the
+ * hand-off region of long fits and (rounds > 1) the parking buffers.  The context owns one for
+ * its eager calls, grown on demand; each frame-block graph owns one sized at creation, so no
+ * later call can move what a graph's nodes point to.  owned false: memory the workspace does
+ * not manage (never grown or freed). */
+struct Workspace {
+    int *order = nullptr;
+    size_t order_cap = 0;         /* queue entries */
+    unsigned *counters = nullptr; /* sort counters, then (count, head) per class for the
+                                     hand-offs, then (count, head) x 2 buffers per class for parking */
+    uint8_t *strag = nullptr;
+    size_t strag_entries = 0;     /* hand-off entries of kParkedMax bytes */
+    void *park[kClasses][2] = {};
+    size_t park_cap[kClasses][2] = {};
+    bool owned = true;
+};
+constexpr int kWorkspaceCounters = kCounterWords + 2 * kClasses + 4 * kClasses;
+
 struct LzrContext {
     int device = 0;
     LaneConfig cfg{};
@@ -743,29 +785,21 @@ struct LzrContext {
     size_t lds_per_cu = 160 * 1024;
     int rounds = 1;          /* launches per class; fits are parked between them */
     int park_threshold = 16; /* park a wave's fits when this few lanes are still busy */
-    void *d_park[kClasses][2] = {}; /* [class][buffer] */
-    size_t park_cap[kClasses][2] = {};
-    unsigned *d_park_ctr = nullptr; /* [class][buffer] x (count, head) */
-    int strag_trials = 48;             /* hand a fit to the straggler kernel after this many trials */
-    uint8_t *d_strag = nullptr;        /* handed-off fits of the running class */
-    size_t strag_cap = 0;              /* bytes */
-    unsigned *d_strag_ctr = nullptr;   /* per class (count, head) */
+    int strag_trials = 48;   /* hand a fit to the straggler kernel after this many trials */
+    Workspace ws;            /* the eager calls' work memory */
     hipStream_t side[kClasses - 1] = {}; /* classes 1, 2 run beside class 0 */
     hipEvent_t fork = nullptr, join[kClasses - 1] = {};
-    LzrStreamOrder order;              /* calls on different streams share the buffers above */
+    LzrStreamOrder order;    /* eager calls on different streams share ws */
     int num_cus = 0;
     std::string err;
     std::mutex mu;
-    /* device buffers */
+    /* staging of the host-buffer entry point */
     uint16_t *d_frames = nullptr;
     size_t frames_cap = 0;
     LzrCandidate *d_cands = nullptr;
     size_t cands_cap = 0;
     LzrFit *d_out = nullptr;
     size_t out_cap = 0;
-    int *d_order = nullptr;
-    size_t order_cap = 0;
-    unsigned *d_counters = nullptr;
     hipStream_t stream = nullptr;
 };
 
@@ -782,8 +816,9 @@ static void ensure(T *&ptr, size_t &cap, size_t need) {
     cap = n;
 }
 
+struct Workspace;
 template <int P, typename Acc, int W>
-static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream);
+static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream);
 
 template <int P, typename Acc, int W>
 static int grid_for(LzrContext *ctx, size_t lds) {
@@ -796,34 +831,34 @@ static int grid_for(LzrContext *ctx, size_t lds) {
 
 /* Parking buffers of a class (used when rounds > 1). */
 template <int P, typename Acc, int W>
-static void ensure_park(LzrContext *ctx, int cls, size_t lds) {
+static void reserve_park(LzrContext *ctx, Workspace &ws, int cls, size_t lds) {
     if (ctx->rounds <= 1) return;
     const size_t cap = (size_t)grid_for<P, Acc, W>(ctx, lds) * 64;
     for (int b = 0; b < 2; ++b) {
-        if (ctx->park_cap[cls][b] < cap) {
-            if (ctx->d_park[cls][b]) LZR_CHECK(hipFree(ctx->d_park[cls][b]));
-            ctx->d_park[cls][b] = nullptr;
-            ctx->park_cap[cls][b] = 0;
-            LZR_CHECK(hipMalloc(&ctx->d_park[cls][b], cap * sizeof(Parked<P>)));
-            ctx->park_cap[cls][b] = cap;
-        }
+        if (ws.park_cap[cls][b] >= cap) continue;
+        if (!ws.owned) throw std::string("parking rounds (LZR_OPT_ROUNDS > 1) need allocated buffers");
+        if (ws.park[cls][b]) LZR_CHECK(hipFree(ws.park[cls][b]));
+        ws.park[cls][b] = nullptr;
+        ws.park_cap[cls][b] = 0;
+        LZR_CHECK(hipMalloc(&ws.park[cls][b], cap * sizeof(Parked<P>)));
+        ws.park_cap[cls][b] = cap;
     }
 }
 
 template <int P, typename Acc, int W>
-static void run_class(LzrContext *ctx, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
+static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
     const int grid = grid_for<P, Acc, W>(ctx, lds);
-    ensure_park<P, Acc, W>(ctx, cls, lds);
-    unsigned *ctr = ctx->d_park_ctr + 4 * cls; /* buffer b: count ctr[2b], head ctr[2b+1] */
+    reserve_park<P, Acc, W>(ctx, ws, cls, lds);
+    unsigned *ctr = ws.counters + kCounterWords + 2 * kClasses + 4 * cls; /* buffer b: count ctr[2b], head ctr[2b+1] */
     for (int round = 0; round < ctx->rounds; ++round) {
         const int ob = round & 1, ib = ob ^ 1;
         hipLaunchKernelGGL(reset_park, dim3(1), dim3(64), 0, stream, ctr + 2 * ob);
         RoundArgs r;
         r.resume = round > 0;
-        r.in = ctx->d_park[cls][ib];
+        r.in = ws.park[cls][ib];
         r.in_count = ctr + 2 * ib;
         r.in_head = ctr + 2 * ib + 1;
-        r.out = ctx->d_park[cls][ob];
+        r.out = ws.park[cls][ob];
         r.out_count = ctr + 2 * ob;
         r.threshold = round == ctx->rounds - 1 ? 0 : ctx->park_threshold;
         hipLaunchKernelGGL((lm_persistent<P, Acc, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
@@ -860,15 +895,6 @@ static size_t launch_lds(const LzrContext *ctx) {
     return (size_t)nmax * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * sizeof(uint16_t);
 }
 
-/* Allocates everything a launch of up to ncands candidates needs, so that the launch
- * itself allocates nothing (stream capture). */
-static void ensure_strag(LzrContext *ctx, int ncands) {
-    if (ctx->strag_trials <= 0) return;
-    size_t cap = ctx->strag_cap;
-    ensure(ctx->d_strag, cap, (size_t)ncands * kParkedMax);
-    ctx->strag_cap = cap;
-}
-
 /* Waves per SIMD the 5-parameter kernel is built for.  Automatic: 2 when the LDS tiles
  * of at least 6 waves fit a CU (windows up to 11x11), so most SIMDs hold two: the
  * two-wave build spills a few refill-path values but hides the pass's latencies (500k
@@ -879,34 +905,59 @@ static int occupancy5(const LzrContext *ctx) {
     return launch_lds(ctx) * 6 <= ctx->lds_per_cu ? 2 : 1;
 }
 
-static void prepare(LzrContext *ctx, int ncands) {
-    ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
-    ensure_strag(ctx, ncands);
+/* Makes ws hold a launch of up to ncands candidates (allocating, for an owned workspace), so
+ * that the launch itself allocates nothing.  A fixed workspace must already hold the queue;
+ * its hand-off region may be smaller than ncands entries (fits that find it full stay in the
+ * first kernel). */
+static void reserve(LzrContext *ctx, Workspace &ws, int ncands) {
+    const size_t n = (size_t)std::max(ncands, 1);
+    if (ws.owned) {
+        ensure(ws.order, ws.order_cap, n);
+        if (!ws.counters) LZR_CHECK(hipMalloc((void **)&ws.counters, kWorkspaceCounters * sizeof(unsigned)));
+        if (ctx->strag_trials > 0) {
+            size_t bytes = ws.strag_entries * kParkedMax;
+            ensure(ws.strag, bytes, n * kParkedMax);
+            ws.strag_entries = bytes / kParkedMax;
+        }
+    } else if (ws.order_cap < n || !ws.counters) {
+        throw std::string("batch exceeds the fixed workspace");
+    }
     const size_t lds = launch_lds(ctx);
     if (occupancy5(ctx) == 1)
-        ensure_park<5, AccT, 1>(ctx, 0, lds);
+        reserve_park<5, AccT, 1>(ctx, ws, 0, lds);
     else
-        ensure_park<5, AccT, 2>(ctx, 0, lds);
-    ensure_park<7, AccT, 1>(ctx, 1, lds);
-    ensure_park<4, AccT, 2>(ctx, 2, lds);
+        reserve_park<5, AccT, 2>(ctx, ws, 0, lds);
+    reserve_park<7, AccT, 1>(ctx, ws, 1, lds);
+    reserve_park<4, AccT, 2>(ctx, ws, 2, lds);
+}
+
+static void release(Workspace &ws) {
+    if (!ws.owned) return;
+    (void)hipFree(ws.order);
+    (void)hipFree(ws.counters);
+    (void)hipFree(ws.strag);
+    for (int c = 0; c < kClasses; ++c)
+        for (int b = 0; b < 2; ++b) (void)hipFree(ws.park[c][b]);
+    ws = Workspace();
 }
 
 /* Enqueues one fit of ncands candidates (d_count, when not null, holds the live count
- * on the device, at most ncands).  Allocates nothing once prepare() covered ncands. */
-static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int nframes,
+ * on the device, at most ncands) on workspace ws.  Allocates nothing once reserve() covered
+ * ncands; order (eager calls) waits for the previous call on ws when the stream differs. */
+static void launch(LzrContext *ctx, Workspace &ws, const uint16_t *frames, int nx, int ny, int nframes,
                    const LzrCandidate *cands, const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream,
-                   const int *d_count = nullptr) {
-    ensure(ctx->d_order, ctx->order_cap, (size_t)ncands);
-    ensure_strag(ctx, ncands);
-    /* the context's queue, counters and hand-off buffers are reused by every call */
-    LZR_CHECK(ctx->order.begin(stream));
-    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, ctx->d_counters, kCounterWords);
+                   const int *d_count = nullptr, LzrStreamOrder *order = nullptr) {
+    reserve(ctx, ws, ncands);
+    /* the workspace's queue, counters and hand-off buffers are reused by every call on it */
+    if (order) LZR_CHECK(order->begin(stream));
+    unsigned *counters = ws.counters, *strag_ctr = ws.counters + kCounterWords;
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(256), 0, stream, counters, kWorkspaceCounters);
     const int nb = (ncands + kSortBlock - 1) / kSortBlock;
     hipLaunchKernelGGL(histogram_kernel, dim3(nb), dim3(kSortThreads), 0, stream, cands, starts, ncands, d_count,
-                       ctx->cfg.model, ctx->d_counters);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kKeys), 0, stream, ctx->d_counters);
+                       ctx->cfg.model, counters);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kKeys), 0, stream, counters);
     hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(kSortThreads), 0, stream, cands, starts, ncands, d_count,
-                       ctx->cfg.model, ctx->d_counters, ctx->d_order);
+                       ctx->cfg.model, counters, ws.order);
     KernelArgs a;
     a.frames = frames;
     a.nx = nx;
@@ -914,27 +965,27 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     a.nframes = nframes;
     a.cands = cands;
     a.starts = starts;
-    a.order = ctx->d_order;
-    a.count = ctx->d_counters;
-    a.head = ctx->d_counters + kClasses;
+    a.order = ws.order;
+    a.count = counters;
+    a.head = counters + kClasses;
     a.out = out;
     a.cfg = ctx->cfg;
-    a.strag_trials = ctx->strag_trials;
-    a.strag = ctx->d_strag;
-    a.strag_ctr = ctx->d_strag_ctr;
+    a.strag_trials = ws.strag_entries > 0 ? ctx->strag_trials : 0;
+    a.strag = ws.strag;
+    a.strag_ctr = strag_ctr;
+    a.strag_entries = (unsigned)std::min(ws.strag_entries, (size_t)0xffffffffu);
     const size_t lds = launch_lds(ctx);
-    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, stream, ctx->d_strag_ctr, 2 * kClasses);
     /* The classes share nothing but the sorted queue: classes 1 and 2 run on the
      * context's side streams beside class 0, so one class's straggler tail overlaps
      * another class's main kernel (fork/join by events; also inside a graph capture). */
     LZR_CHECK(hipEventRecord(ctx->fork, stream));
     for (int c = 0; c < kClasses - 1; ++c) LZR_CHECK(hipStreamWaitEvent(ctx->side[c], ctx->fork, 0));
-    run_class<7, AccT, 1>(ctx, a, 1, lds, ctx->side[0]);
-    run_class<4, AccT, 2>(ctx, a, 2, lds, ctx->side[1]);
+    run_class<7, AccT, 1>(ctx, ws, a, 1, lds, ctx->side[0]);
+    run_class<4, AccT, 2>(ctx, ws, a, 2, lds, ctx->side[1]);
     if (occupancy5(ctx) == 1)
-        run_class<5, AccT, 1>(ctx, a, 0, lds, stream);
+        run_class<5, AccT, 1>(ctx, ws, a, 0, lds, stream);
     else
-        run_class<5, AccT, 2>(ctx, a, 0, lds, stream);
+        run_class<5, AccT, 2>(ctx, ws, a, 0, lds, stream);
     for (int c = 0; c < kClasses - 1; ++c) {
         LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
         LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
@@ -942,7 +993,13 @@ static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int 
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
                        ctx->cfg);
     LZR_CHECK(hipGetLastError());
-    LZR_CHECK(ctx->order.end(stream));
+    if (order) LZR_CHECK(order->end(stream));
+}
+
+/* An eager call on the context's own workspace. */
+static void launch(LzrContext *ctx, const uint16_t *frames, int nx, int ny, int nframes, const LzrCandidate *cands,
+                   const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream) {
+    launch(ctx, ctx->ws, frames, nx, ny, nframes, cands, starts, ncands, out, stream, nullptr, &ctx->order);
 }
 
 static int fail(LzrContext *ctx, const std::string &msg) {
@@ -982,9 +1039,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         LZR_CHECK(hipGetDeviceProperties(&prop, ctx->device));
         ctx->num_cus = prop.multiProcessorCount;
         if (prop.maxSharedMemoryPerMultiProcessor > 0) ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
-        LZR_CHECK(hipMalloc((void **)&ctx->d_counters, kCounterWords * sizeof(unsigned)));
-        LZR_CHECK(hipMalloc((void **)&ctx->d_park_ctr, 4 * kClasses * sizeof(unsigned)));
-        LZR_CHECK(hipMalloc((void **)&ctx->d_strag_ctr, 2 * kClasses * sizeof(unsigned)));
+        LZR_CHECK(hipMalloc((void **)&ctx->ws.counters, kWorkspaceCounters * sizeof(unsigned)));
         for (int c = 0; c < kClasses - 1; ++c) {
             LZR_CHECK(hipStreamCreateWithFlags(&ctx->side[c], hipStreamNonBlocking));
             LZR_CHECK(hipEventCreateWithFlags(&ctx->join[c], hipEventDisableTiming));
@@ -1014,19 +1069,13 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_frames);
     (void)hipFree(ctx->d_cands);
     (void)hipFree(ctx->d_out);
-    (void)hipFree(ctx->d_order);
-    (void)hipFree(ctx->d_counters);
-    (void)hipFree(ctx->d_park_ctr);
-    (void)hipFree(ctx->d_strag_ctr);
+    release(ctx->ws);
     for (int c = 0; c < kClasses - 1; ++c) {
         if (ctx->side[c]) (void)hipStreamDestroy(ctx->side[c]);
         if (ctx->join[c]) (void)hipEventDestroy(ctx->join[c]);
     }
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     ctx->order.destroy();
-    (void)hipFree(ctx->d_strag);
-    for (int c = 0; c < kClasses; ++c)
-        for (int b = 0; b < 2; ++b) (void)hipFree(ctx->d_park[c][b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -1245,6 +1294,7 @@ LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value) {
 
 struct LzrGraph {
     LzrContext *ctx = nullptr;
+    Workspace ws; /* the graph's own queue, counters and hand-off buffers (sized at creation) */
     hipStream_t capture = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1255,6 +1305,7 @@ LZR_EXPORT int lzr_graph_destroy(LzrGraph *g) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->capture) (void)hipStreamDestroy(g->capture);
+    release(g->ws);
     delete g;
     return 0;
 }
@@ -1273,12 +1324,13 @@ LZR_EXPORT int lzr_graph_create(LzrContext *ctx, LzrGraph **out, const uint16_t 
     g->ctx = ctx;
     try {
         LZR_CHECK(hipSetDevice(ctx->device));
-        prepare(ctx, max_candidates);
+        reserve(ctx, g->ws, max_candidates);
         LZR_CHECK(hipDeviceSynchronize());
         LZR_CHECK(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
         LZR_CHECK(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
         try {
-            launch(ctx, d_frames, nx, ny, nframes, d_cands, nullptr, max_candidates, d_results, g->capture, d_count);
+            launch(ctx, g->ws, d_frames, nx, ny, nframes, d_cands, nullptr, max_candidates, d_results, g->capture,
+                   d_count);
         } catch (...) {
             hipGraph_t partial = nullptr;
             (void)hipStreamEndCapture(g->capture, &partial);

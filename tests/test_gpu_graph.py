@@ -59,6 +59,41 @@ def test_gpu_graph_replays_match_eager_calls():
         g.close()
 
 
+def test_gpu_graph_survives_larger_eager_calls():
+    """A graph owns its work buffers: an eager call on the same context with twice the graph's
+    capacity (which grows the context's own queue and hand-off buffers) must not move what the
+    graph's nodes point to.  Graph of N candidates, eager call of 2N, replay, compare."""
+    import torch
+
+    _say("case: graph of N, eager 2N in between, replay")
+    w = synth.PLAIN_GPU_256.scaled(80)
+    frames, _, cands = synth.generate(w)
+    n = cands.size // 2
+    part = cands[:n]
+    st_frames = device.empty(np.uint16, frames.size)
+    st_cands = device.empty(abi.CANDIDATE_DTYPE, n)
+    st_count = torch.zeros(1, dtype=torch.int32, device="cuda")
+    st_out = device.empty(abi.FIT_DTYPE, n)
+    s = device.stream_handle()
+    with native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, default_half_width=4) as ctx:
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, 11)  # many hand-offs: the hand-off region is in use
+        g = ctx.graph(st_frames.ptr, w.nx, w.ny, frames.shape[0], st_cands.ptr, st_count.data_ptr(), n, st_out.ptr)
+        big = ctx.fit(frames, cands)  # 2N: the context's buffers are reallocated
+        small = ctx.fit(frames, part)
+        st_frames.t.copy_(torch.from_numpy(frames.reshape(-1).view(np.uint8)))
+        st_cands.t.copy_(torch.from_numpy(part.view(np.uint8)))
+        st_count.fill_(n)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            g.launch(s)
+            torch.cuda.synchronize()
+            got = st_out.t.cpu().numpy().view(abi.FIT_DTYPE)[:n]
+            assert np.array_equal(got.view(np.uint8), small.view(np.uint8))
+            ctx.fit(frames, cands)  # and again after another large eager call
+        assert np.array_equal(big[:n].view(np.uint8), small.view(np.uint8))  # batch independence
+        g.close()
+
+
 def test_gpu_autotuner_keeps_results():
     _say("case: autotuner on a mixed batch")
     w = synth.MIXED_2M.scaled(60)
