@@ -155,6 +155,20 @@ LZR_EXPORT int lzr_fit_batch_device(LzrContext *ctx, const uint16_t *d_frames, i
                                     int nframes, const LzrCandidate *d_candidates, int ncandidates,
                                     LzrFit *d_results, void *stream);
 
+/* As lzr_fit_batch_device, but allocating nothing: no device allocation, no allocating
+ * library call.  No counterpart in the reference; it is for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * work memory is a fixed 64 MiB device array per GPU, held until the process ends: at most
+ * LZR_NOALLOC_MAX_CANDIDATES candidates per call (larger batches return -1); fits run in
+ * one round (LZR_OPT_ROUNDS is ignored); long fits that find the array's hand-off entries
+ * taken finish in the first kernel, with the same results.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (any context, any stream) race -- order them.
+ * Results are bit-identical to lzr_fit_batch_device's. */
+#define LZR_NOALLOC_MAX_CANDIDATES (2 << 20)
+LZR_EXPORT int lzr_fit_batch_device_noalloc(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny,
+                                            int nframes, const LzrCandidate *d_candidates, int ncandidates,
+                                            LzrFit *d_results, void *stream);
+
 /* ABI version (LZR_ABI_VERSION). */
 LZR_EXPORT int lzr_abi_version(void);
 

@@ -762,8 +762,8 @@ using namespace lzr;
 /* Device memory one fit launch works in: the sorted work queue and its counters, the
  * hand-off region of long fits and (rounds > 1) the parking buffers.  The context owns one for
  * its eager calls, grown on demand; each frame-block graph owns one sized at creation, so no
- * later call can move what a graph's nodes point to.  owned false: memory the workspace does
- * not manage (never grown or freed). */
+ * later call can move what a graph's nodes point to; the allocation-free entry point uses one
+ * carved from a fixed device array (owned false: never grown or freed). */
 struct Workspace {
     int *order = nullptr;
     size_t order_cap = 0;         /* queue entries */
@@ -791,6 +791,7 @@ struct LzrContext {
     hipEvent_t fork = nullptr, join[kClasses - 1] = {};
     LzrStreamOrder order;    /* eager calls on different streams share ws */
     int num_cus = 0;
+    void *scratch = nullptr; /* this device's fixed scratch array (allocation-free entry) */
     std::string err;
     std::mutex mu;
     /* staging of the host-buffer entry point */
@@ -804,6 +805,35 @@ struct LzrContext {
 };
 
 static thread_local std::string g_create_error;
+
+/* ---- fixed scratch of the allocation-free entry point --------------------------------
+ * One array per GPU, allocated with the code object and held until the process ends:
+ * sort counters, the work queue of up to kScratchMaxCandidates candidates, and the rest as
+ * hand-off entries (long fits that find them all taken finish in the first kernel, with the
+ * same result).  Host code never passes `scratch` itself to a kernel: its address on each
+ * device comes from hipGetSymbolAddress at context creation (g_scratch). */
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr size_t kScratchMaxCandidates = (size_t)LZR_NOALLOC_MAX_CANDIDATES;
+constexpr size_t kScratchCountersBytes = (kWorkspaceCounters * sizeof(unsigned) + 255) & ~(size_t)255;
+constexpr size_t kScratchOrderBytes = kScratchMaxCandidates * sizeof(int);
+static_assert(kScratchCountersBytes + kScratchOrderBytes + 4096 * kParkedMax <= kScratchBytes, "scratch layout");
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch[64]; /* per device ordinal */
+
+/* The scratch array's address on the current device (looked up once per device). */
+static void *scratch_address(int device) {
+    if (device < 0 || device >= 64) throw std::string("device ordinal out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        LZR_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::string("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[device] = p;
+    }
+    return g_scratch[device];
+}
+
 
 template <typename T>
 static void ensure(T *&ptr, size_t &cap, size_t need) {
@@ -1038,6 +1068,7 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
         hipDeviceProp_t prop;
         LZR_CHECK(hipGetDeviceProperties(&prop, ctx->device));
         ctx->num_cus = prop.multiProcessorCount;
+        ctx->scratch = scratch_address(ctx->device);
         if (prop.maxSharedMemoryPerMultiProcessor > 0) ctx->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
         LZR_CHECK(hipMalloc((void **)&ctx->ws.counters, kWorkspaceCounters * sizeof(unsigned)));
         for (int c = 0; c < kClasses - 1; ++c) {
@@ -1161,6 +1192,38 @@ LZR_EXPORT int lzr_fit_batch_device(LzrContext *ctx, const uint16_t *d_frames, i
     } catch (const std::string &e) {
         return fail(ctx, e);
     }
+    return 0;
+}
+
+LZR_EXPORT int lzr_fit_batch_device_noalloc(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                            const LzrCandidate *d_cands, int ncands, LzrFit *d_out, void *stream) {
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+    if (ncands == 0) return 0;
+    if ((size_t)ncands > kScratchMaxCandidates)
+        return fail(ctx, "lzr_fit_batch_device_noalloc: " + std::to_string(ncands) +
+                             " candidates exceed the allocation-free maximum of " + std::to_string(kScratchMaxCandidates));
+    if (!d_frames || !d_cands || !d_out) return fail(ctx, "null device pointer");
+    if (!ctx->scratch) return fail(ctx, "lzr_fit_batch_device_noalloc: no scratch address for this device");
+    Workspace ws;
+    ws.owned = false;
+    uint8_t *base = static_cast<uint8_t *>(ctx->scratch);
+    ws.counters = reinterpret_cast<unsigned *>(base);
+    ws.order = reinterpret_cast<int *>(base + kScratchCountersBytes);
+    ws.order_cap = kScratchMaxCandidates;
+    ws.strag = base + kScratchCountersBytes + kScratchOrderBytes;
+    ws.strag_entries = (kScratchBytes - kScratchCountersBytes - kScratchOrderBytes) / kParkedMax;
+    const int rounds = ctx->rounds;
+    ctx->rounds = 1; /* no parking buffers here */
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        launch(ctx, ws, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_out, (hipStream_t)stream);
+    } catch (const std::string &e) {
+        ctx->rounds = rounds;
+        return fail(ctx, e);
+    }
+    ctx->rounds = rounds;
     return 0;
 }
 

@@ -24,6 +24,7 @@ EXPORTED = (
     "lzr_abi_version",
     "lzr_fit_batch_device_start",
     "lzr_fit_batch_device_netinit",
+    "lzr_fit_batch_device_noalloc",
     "lzr_net_create",
     "lzr_net_destroy",
     "lzr_net_last_error",
@@ -99,6 +100,7 @@ def lib():
         L.lzr_net_predict_device.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
         L.lzr_net_train_step_device.argtypes = [vp, vp, i, i, i, vp, vp, i, f, i, ctypes.POINTER(f), vp, vp]
         L.lzr_get_option.argtypes = [vp, i, ctypes.POINTER(d)]
+        L.lzr_fit_batch_device_noalloc.argtypes = [vp, vp, i, i, i, vp, i, vp, vp]
         L.lzr_graph_create.argtypes = [vp, ctypes.POINTER(vp), vp, i, i, i, vp, vp, i, vp]
         L.lzr_graph_launch.argtypes = [vp, vp]
         L.lzr_graph_destroy.argtypes = [vp]
