@@ -79,6 +79,8 @@ def build_dropin_driver():
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", os.path.join(cpp, "dropin_main"),
           os.path.join(cpp, "dropin_main.cpp"), f"-L{LIB}", "-llocalizer_hip",
           "-Wl,-rpath,$ORIGIN/../../localizer_amd/lib", "-lpthread"])
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", os.path.join(cpp, "storage_main"),
+          os.path.join(cpp, "storage_main.cpp")])
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", os.path.join(cpp, "render_main"),
           os.path.join(cpp, "render_main.cpp"), f"-L{LIB}", "-llocalizer_hip",
           "-Wl,-rpath,$ORIGIN/../../localizer_amd/lib", "-lpthread"])
