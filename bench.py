@@ -55,14 +55,35 @@ def algorithmic_flops(fits, cands, default_h):
 
 
 def algorithmic_bytes(fits, cands, default_h):
-    """uint16 window pixels + candidate record + result record, per ROI."""
+    """The frozen per-ROI byte count: uint16 window pixels 2N + a 16-byte candidate (x, y, A0,
+    b0 in FP32) + a 4(2p + 3)-byte result (p parameters, p deviations, 3 more fields in FP32):
+    230 B for a 9x9 symmetric ROI."""
+    h = np.where(cands["half_width"] > 0, cands["half_width"], default_h).astype(np.float64)
+    N = (2 * h + 1) ** 2
+    p = np.where(fits["model"] == abi.MODEL_SYMMETRIC, 5, np.where(fits["model"] == abi.MODEL_FIXED_WIDTH, 4, 7))
+    return float(np.sum(2 * N + 16 + 4 * (2 * p + 3)))
+
+
+def record_bytes(fits, cands, default_h):
+    """What this engine's records move per ROI: window pixels + its 40-byte candidate + its
+    192-byte result record."""
     h = np.where(cands["half_width"] > 0, cands["half_width"], default_h).astype(np.float64)
     N = (2 * h + 1) ** 2
     return float(np.sum(2 * N + abi.CANDIDATE_DTYPE.itemsize + abi.FIT_DTYPE.itemsize))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, frames, cands, target_seconds, nthreads):
-    """Oracle throughput on a bounded prefix of the workload (whole frames)."""
+    """Oracle throughput on a bounded prefix of the workload (whole frames), best of 3."""
     from oracle import pyoracle as po
 
     def run(nf):
@@ -75,17 +96,62 @@ def cpu_baseline(w, frames, cands, target_seconds, nthreads):
 
     nf = min(50, w.nframes)
     acc, n, dt = run(nf)
-    if dt < target_seconds and nf < w.nframes:
-        nf = int(min(w.nframes, max(nf, nf * target_seconds / max(dt, 1e-3))))
-        acc, n, dt = run(nf)
+    if dt < target_seconds / 3 and nf < w.nframes:
+        nf = int(min(w.nframes, max(nf, nf * target_seconds / 3 / max(dt, 1e-3))))
+    runs = [run(nf) for _ in range(3)]
+    acc, n, dt = min(runs, key=lambda r: r[2])
     return {
         "value": acc / dt,
         "unit": "localizations/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": f"{nf} of {w.nframes} frames ({n} ROIs, {acc} accepted) in {dt:.2f} s, FP64 oracle, "
-        f"{nthreads} threads, one frame per task",
+        "sample": f"{nf} of {w.nframes} frames ({n} ROIs, {acc} accepted), best of 3 runs "
+        f"({', '.join(f'{r[2]:.2f}' for r in runs)} s): the FP64 oracle (oracle/lzr_oracle.c, gcc -O2, "
+        f"no fast-math; the reference's own fitter needs libgsl, absent) on {nthreads} threads of a "
+        f"{cpu_model()}, one frame per task as the reference's PALMAnalysisController",
     }
+
+
+def parity_vs_oracle(w, frames, cands, fits, nthreads):
+    """Per-parameter agreement of the timed GPU fits with the FP64 oracle over the whole
+    workload (tests/parity.py metrics; the bar: x, y within 1e-3 px, the rest within 1e-4)."""
+    from oracle import pyoracle as po
+    from tests.parity import compare
+
+    t0 = time.perf_counter()
+    ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, w.initial_psf_width, default_half_width=w.half_width[0],
+                       nthreads=nthreads)
+    st = compare(ref, fits, abi.MODEL_SYMMETRIC)
+    keep = ("status_agreement", "both", "xy_max_px", "xy_frac_within_1e-3", "conv_n", "conv_xy_max_px",
+            "conv_xy_frac_within_1e-3")
+    out = {k: st[k] for k in keep if k in st}
+    for nm in ("A", "r", "b"):
+        for pre in ("", "conv_"):
+            for suf in ("frac_within_1e-4", "max_rel", "max_abs"):
+                k = f"{pre}{nm}_{suf}"
+                if k in st:
+                    out[k] = st[k]
+    for nm in ("A", "r", "x0", "y0", "b"):
+        out[f"dev_{nm}_frac_within_1e-4"] = st[f"dev_{nm}_frac_within_1e-4"]
+    for f in ("chi", "integral", "integral_deviation"):
+        out[f"{f}_frac_within_1e-4"] = st[f"{f}_frac_within_1e-4"]
+    out["oracle_seconds"] = time.perf_counter() - t0
+    return out
+
+
+def pcie_inclusive(ctx, frames, cands, reps=3):
+    """lzr_fit_batch on host buffers: frames and candidates copied in, results copied out
+    (pageable memory), per call.  Not the headline value (inputs there are resident in HBM)."""
+    ctx.fit(frames[:1], cands[cands["frame"] < 1])  # staging allocated
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = ctx.fit(frames, cands)
+        best = min(best, time.perf_counter() - t0)
+    acc = int(np.sum(res["status"] == 0))
+    return {"value": acc / best, "unit": "localizations/s", "ms_per_call": best * 1e3,
+            "bytes_in": int(frames.nbytes + cands.nbytes), "bytes_out": int(res.nbytes),
+            "entry": "lzr_fit_batch (host buffers, pageable)", "best_of": reps}
 
 
 NET_FLOPS_PER_ROI = 2.0 * (81 * 32 * 9 + 49 * 32 * 288 + 1568 * 64 + 64 * 5)
@@ -258,8 +324,8 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
 
     ms_graph = timed(blocks)
     gfits = d_out.numpy()
-    # a handed-off fit's last bits depend on its batch (the straggler group size follows the
-    # batch's hand-off count), so blocks may differ from the one call in borderline decisions
+    # a fit's result depends on its spot only (straggler groups compute the one-lane pass bit
+    # for bit): the blocks must reproduce the one call exactly
     status_diff = int(np.sum(gfits["status"] != fits["status"]))
     both = np.flatnonzero((gfits["status"] == 0) & (fits["status"] == 0))
     xi = np.where(cands["model"][both] == abi.MODEL_ELLIPSOIDAL, 3, 2)
@@ -288,9 +354,11 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
                          "positions_differing_over_1e-3_px": far},
         "trials_mean": float(fits["trials"].mean()),
     }
+    identical = bool(np.array_equal(gfits.view(np.uint8), fits.view(np.uint8)))
+    out["graph_blocks"]["records_identical_to_one_call"] = identical
     log(f"mixed: one call {ms_default:.2f} ms, autotuned {ms_tuned:.2f} ms ({chosen}), graph blocks {ms_graph:.2f} ms "
-        f"(status differs from the one call for {status_diff} fits, positions by <= {dpos:.1e} px, "
-        f"{far} by more than 1e-3 px)")
+        f"(records identical to the one call: {identical}; status differs for {status_diff} fits, positions by "
+        f"<= {dpos:.1e} px)")
     return out
 
 
@@ -471,7 +539,20 @@ def main():
     ap.add_argument("--net-train-rois", type=int, default=4 << 20)
     ap.add_argument("--net-eval-rois", type=int, default=4000000)
     ap.add_argument("--net-epochs", type=float, default=2.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device once (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -538,11 +619,21 @@ def main():
     value = accepted * args.steps / elapsed
     flops = algorithmic_flops(fits, cands, w.half_width[0])
     nbytes = algorithmic_bytes(fits, cands, w.half_width[0])
+    rbytes = record_bytes(fits, cands, w.half_width[0])
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     print(f"accepted {accepted}/{cands.size}, {ms_per_step:.3f} ms/step, event {kernel_ms:.3f} ms, "
           f"{achieved:.2f} TFLOP/s algorithmic, mean trials {fits['trials'].mean():.2f}",
           file=sys.stderr, flush=True)
 
+    pcie = pcie_inclusive(ctx, frames, cands)
+    print(f"pcie-inclusive (host buffers): {pcie['ms_per_call']:.2f} ms per call, {pcie['value']:.4g} loc/s",
+          file=sys.stderr, flush=True)
+    parity = None
+    if not args.no_cpu:
+        parity = parity_vs_oracle(w, frames, cands, fits, nthreads)
+        print(f"parity vs FP64 oracle over {cands.size} fits: status {parity['status_agreement']:.6f}, x/y within "
+              f"1e-3 px {parity['xy_frac_within_1e-3']:.6f} (max {parity['xy_max_px']:.2e} px)",
+              file=sys.stderr, flush=True)
     render = None
     if not args.no_render:
         render = render_leg(d_out, cands.size, w, lambda m: print(m, file=sys.stderr, flush=True))
@@ -602,9 +693,14 @@ def main():
             "kernel_ms": kernel_ms,
             "algorithmic_flops_per_launch": flops,
             "algorithmic_bytes_per_launch": nbytes,
+            "algorithmic_bytes_per_roi": nbytes / cands.size,
+            "record_bytes_per_launch": rbytes,
             "gather_gbs": nbytes / (kernel_ms * 1e-3) / 1e9,
+            "launch": "one lzr_fit_batch_device call (queue sort, fitting kernels, straggler kernels, epilogue)",
         },
         "cpu_baseline": cpu,
+        "parity_vs_fp64_oracle": parity,
+        "pcie_inclusive": pcie,
         "net_init": net,
         "mixed": mixed,
         "render": render,
