@@ -158,7 +158,33 @@ NET_FLOPS_PER_ROI = 2.0 * (81 * 32 * 9 + 49 * 32 * 288 + 1568 * 64 + 64 * 5)
 PEAK_BF16_TFLOPS = 2500.0
 
 
-def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log):
+def netinit_vs_oracle(W, frames, cands, plain, net, m):
+    """Plain and net-init records against the FP64 oracle (reference start) on the first m
+    ROIs: accepted/rejected agreement, x/y within 1e-3 px and A, r, b within 1e-4 relative
+    where both accept."""
+    from oracle import pyoracle as po
+
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    ref = po.fit_batch(frames[:m], cands[:m], abi.MODEL_SYMMETRIC, W.initial_psf_width,
+                       default_half_width=abi.NET_HALF_WIDTH, nthreads=nthreads)
+    out = {"rois": int(m), "oracle_seconds": time.perf_counter() - t0}
+    racc = ref["status"] == 0
+    for name, fits in (("plain", plain[:m]), ("netinit", net[:m])):
+        acc = fits["status"] == 0
+        both = acc & racc
+        d = np.abs(fits["params"][both] - ref["params"][both])
+        rel = d / np.maximum(np.abs(ref["params"][both]), 1e-300)
+        out[name] = {"accepted_agreement": float(np.mean(acc == racc)),
+                     "status_agreement": float(np.mean(fits["status"] == ref["status"])),
+                     "both_accepted": int(both.sum()),
+                     "xy_within_1e-3": float(np.mean(d[:, 2:4].max(1) <= 1e-3)),
+                     "xy_max_px": float(d[:, 2:4].max()) if both.any() else 0.0,
+                     "A_r_b_within_1e-4": [float(np.mean(rel[:, k] <= 1e-4)) for k in (0, 1, 4)]}
+    return out
+
+
+def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log, oracle_rois=100000):
     """Plain vs net-initialized LM on the learned-initializer workload (one GPU)."""
     import torch
 
@@ -174,6 +200,7 @@ def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log):
     d_net = device.empty(abi.FIT_DTYPE, n)
     d_work = device.empty(abi.START_DTYPE, n)
     d_starts = device.empty(abi.START_DTYPE, n)
+    d_nrefit = device.empty(np.uint32, 1)
     ctx = native.Context(abi.MODEL_SYMMETRIC, W.initial_psf_width, 1.0, abi.NET_HALF_WIDTH)
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
@@ -182,7 +209,15 @@ def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log):
         ctx.fit_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_plain.ptr, s)
 
     def netinit_fit():
-        ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_net.ptr, d_work.ptr, 3.0, stream=s)
+        ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_net.ptr, d_work.ptr, 3.0,
+                               d_nrefit=d_nrefit.ptr, stream=s)
+
+    d_strict = device.empty(abi.FIT_DTYPE, n)
+    d_work2 = device.empty(abi.START_DTYPE, n)
+
+    def netinit_strict():
+        ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_strict.ptr, d_work2.ptr, 3.0,
+                               refit_rejected=True, stream=s)
 
     def predict():
         net.predict_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_starts.ptr, s)
@@ -202,7 +237,9 @@ def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log):
     ms_plain = timed(plain)
     ms_net = timed(netinit_fit)
     ms_pred = timed(predict)
+    ms_strict = timed(netinit_strict)
     P, F, wk = d_plain.numpy(), d_net.numpy(), d_work.numpy()
+    S = d_strict.numpy()
     acc = P["status"] == 0
     both = acc & (F["status"] == 0)
     dxy = np.abs(F["params"][:, 2:4] - P["params"][:, 2:4]).max(axis=1)
@@ -225,8 +262,18 @@ def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log):
         "xy_within_1e-3": float(np.mean(dxy[both] <= 1e-3)),
         "status_agreement": float(np.mean(F["status"] == P["status"])),
         "fallback_fraction": float(refit.mean()),
+        "fallback_counted_on_device": int(d_nrefit.numpy()[0]),
+        "fallback_rule": "refit from the reference start when the net-start fit ends with |f| above 3x |f| at "
+                         "the reference start",
+        "accepted_agreement_with_plain": float(np.mean((F["status"] == 0) == acc)),
         "trials_plain": float(P["trials"].mean()),
         "trials_netinit_final": float(F["trials"].mean()),
+        "refit_rejected_policy": {
+            "ms": ms_strict, "fallback_fraction": float(np.mean(d_work2.numpy()["width"] == 0.0)),
+            "accepted_agreement_with_plain": float(np.mean((S["status"] == 0) == acc)),
+            "status_agreement": float(np.mean(S["status"] == P["status"])),
+            "rule": "also refit net-start fits that were rejected or did not converge"},
+        "vs_fp64_oracle": netinit_vs_oracle(W, frames, cands, P, F, oracle_rois),
         "train": {"rois": train_rois, "steps": tr["steps"], "batch": tr["batch"], "seconds": tr["train_s"],
                   "last_loss": tr["losses"][-1][1]},
         "network": {"bound": "mfma", "achieved": NET_FLOPS_PER_ROI * n / (ms_pred * 1e-3) / 1e12,

@@ -322,17 +322,22 @@ LZR_EXPORT int lzr_net_train_step_device(LzrNet *net, const uint16_t *d_frames, 
                                          float *grads, void *stream);
 
 /* Net-initialized LM with fallback: starts from lzr_net_predict_device, then refits
- * from the reference start every candidate whose net-start fit is a high-residual
- * spot: normalized residual chi/sqrt(dof) above residual_factor * sqrt(max(b, 1))
- * (b = fitted offset; sqrt(b) is the residual Poisson noise alone leaves).  With
- * refit_rejected != 0, net-start fits that were rejected or did not converge
- * (LZR_LM_SUCCESS) are refitted too.  d_work: ncandidates LzrStart of scratch;
- * afterwards its width is 0 for refitted candidates and -1 for the others.
+ * from the reference start (source/Localization.cpp:138-142, :485-491) every candidate
+ * whose net-start fit
+ *   - was rejected or did not converge (LZR_LM_SUCCESS), when refit_rejected != 0, or
+ *   - was accepted with a residual |f| above residual_factor times the residual at the
+ *     reference start point (the normalized residuals chi/sqrt(dof) share the dof).
+ * A refitted candidate's record is the plain fit's record bit for bit.  d_work:
+ * ncandidates LzrStart of scratch; afterwards its width is 0 for refitted candidates and
+ * -1 for the others.  d_nrefit (may be null): one device unsigned, set to the number of
+ * refitted candidates.  No reference counterpart (the reference has no network start);
+ * the records it returns are those of lzr_fit_batch_device for the refitted candidates.
  * Returns 0 or -1. */
 LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const uint16_t *d_frames, int nx,
                                             int ny, int nframes, const LzrCandidate *d_candidates,
                                             int ncandidates, LzrFit *d_results, LzrStart *d_work,
-                                            double residual_factor, int refit_rejected, void *stream);
+                                            double residual_factor, int refit_rejected, unsigned *d_nrefit,
+                                            void *stream);
 
 /* ---- Localization bitmap ------------------------------------------------------
  * PALMBitmapImageCalculator::CalculateImage(positions, xSize, ySize, outputScaleFactor)

@@ -792,6 +792,7 @@ struct LzrContext {
     LzrStreamOrder order;    /* eager calls on different streams share ws */
     int num_cus = 0;
     void *scratch = nullptr; /* this device's fixed scratch array (allocation-free entry) */
+    unsigned *netinit_words = nullptr; /* net-init: refit count, live count of the refit launch */
     std::string err;
     std::mutex mu;
     /* staging of the host-buffer entry point */
@@ -1100,6 +1101,7 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_frames);
     (void)hipFree(ctx->d_cands);
     (void)hipFree(ctx->d_out);
+    (void)hipFree(ctx->netinit_words);
     release(ctx->ws);
     for (int c = 0; c < kClasses - 1; ++c) {
         if (ctx->side[c]) (void)hipStreamDestroy(ctx->side[c]);
@@ -1275,30 +1277,32 @@ LZR_EXPORT int lzr_fit_batch_device_start(LzrContext *ctx, const uint16_t *d_fra
 
 namespace lzr {
 
-/* Marks the candidates whose net-start fit is refitted from the reference start:
- * width 0 (refit) when the net start was used (width > 0) and the fit left a
- * normalized residual chi/sqrt(dof) above factor*sqrt(max(b,1)) (or, with
- * refit_rejected, was rejected or did not converge); width -1 (keep) otherwise. */
-__global__ void mark_fallback_kernel(const LzrCandidate *cands, const LzrFit *fits, LzrStart *starts, int n,
-                                     LaneConfig cfg, double factor, int refit_rejected) {
+/* Marks the candidates whose net-start fit is refitted from the reference start: width 0
+ * (refit) when the net start was used (width > 0) and the fit was rejected or did not
+ * converge (with refit_rejected), or ended with a residual |f| above factor times the residual
+ * at the reference's own start point (the normalized residuals chi/sqrt(dof) share the dof);
+ * width -1 (keep) otherwise.  *nrefit (zeroed before) counts the refits. */
+__global__ void mark_fallback_kernel(const uint16_t *frames, int nx, int ny, int nframes, const LzrCandidate *cands,
+                                     const LzrFit *fits, LzrStart *starts, int n, LaneConfig cfg, double factor,
+                                     int refit_rejected, unsigned *nrefit) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const LzrFit &f = fits[i];
-    const LzrCandidate c = cands[i];
     bool refit = false;
-    if (starts[i].width > 0.0) {
+    if (i < n && starts[i].width > 0.0) {
+        const LzrFit &f = fits[i];
         const bool failed = f.status != LZR_ACCEPTED || f.lm_status != LZR_LM_SUCCESS;
         refit = refit_rejected && failed;
-        if (!refit && f.status == LZR_ACCEPTED) {
-            const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
-            const int p = f.model == LZR_MODEL_SYMMETRIC ? 5 : (f.model == LZR_MODEL_FIXED_WIDTH ? 4 : 7);
-            const double dof = (double)((2 * h - 1) * (2 * h - 1) - p);
-            const double b = f.params[p - 1];
-            const double norm = dof > 0.0 ? f.chi / sqrt(dof) : 0.0;
-            refit = !(norm <= factor * sqrt(b > 1.0 ? b : 1.0));
-        }
+        if (!refit && f.status == LZR_ACCEPTED)
+            refit = f.chi > factor * start_residual(cfg, cands[i], frames, nx, ny, nframes);
     }
-    starts[i].width = refit ? 0.0 : -1.0;
+    if (i < n) starts[i].width = refit ? 0.0 : -1.0;
+    const unsigned long long m = __ballot(refit);
+    if (m && (threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(nrefit, (unsigned)__popcll(m));
+}
+
+/* The refit launch's live candidate count: all of them when any is marked, else none (every
+ * kernel of that launch then returns at once). */
+__global__ void refit_live_kernel(const unsigned *nrefit, int *live, int n) {
+    if (threadIdx.x == 0) *live = *nrefit ? n : 0;
 }
 
 }  // namespace lzr
@@ -1306,7 +1310,7 @@ __global__ void mark_fallback_kernel(const LzrCandidate *cands, const LzrFit *fi
 LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const uint16_t *d_frames, int nx, int ny,
                                             int nframes, const LzrCandidate *d_cands, int ncands, LzrFit *d_out,
                                             LzrStart *d_work, double residual_factor, int refit_rejected,
-                                            void *stream) {
+                                            unsigned *d_nrefit, void *stream) {
     if (!ctx) return -1;
     if (!net) return fail(ctx, "null network");
     if (!d_work && ncands > 0) return fail(ctx, "null work buffer");
@@ -1322,10 +1326,15 @@ LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const 
     try {
         LZR_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = (hipStream_t)stream;
+        if (!ctx->netinit_words) LZR_CHECK(hipMalloc(&ctx->netinit_words, 2 * sizeof(unsigned)));
+        unsigned *nrefit = d_nrefit ? d_nrefit : ctx->netinit_words;
+        int *live = reinterpret_cast<int *>(ctx->netinit_words + 1);
         launch(ctx, d_frames, nx, ny, nframes, d_cands, d_work, ncands, d_out, s);
-        hipLaunchKernelGGL(mark_fallback_kernel, dim3((ncands + 255) / 256), dim3(256), 0, s, d_cands, d_out, d_work,
-                           ncands, ctx->cfg, residual_factor, refit_rejected);
-        launch(ctx, d_frames, nx, ny, nframes, d_cands, d_work, ncands, d_out, s);
+        hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, s, nrefit, 1);
+        hipLaunchKernelGGL(mark_fallback_kernel, dim3((ncands + 255) / 256), dim3(256), 0, s, d_frames, nx, ny, nframes,
+                           d_cands, d_out, d_work, ncands, ctx->cfg, residual_factor, refit_rejected, nrefit);
+        hipLaunchKernelGGL(refit_live_kernel, dim3(1), dim3(64), 0, s, nrefit, live, ncands);
+        launch(ctx, ctx->ws, d_frames, nx, ny, nframes, d_cands, d_work, ncands, d_out, s, live, &ctx->order);
         LZR_CHECK(hipGetLastError());
     } catch (const std::string &e) {
         return fail(ctx, e);

@@ -1827,6 +1827,75 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
     }
 }
 
+/* Separable sums of start_residual over an n x n window (n <= NMAX): the loops are unrolled
+ * to NMAX with the columns beyond n masked, so the column factors stay in registers. */
+template <int NMAX, typename FrameT>
+LZR_HD LZR_INLINE double start_sse(const FrameT *src, int nx, int n, double x0, double y0, double k, double A,
+                                   double b) {
+    LZR_NO_CONTRACT
+    double ex[NMAX];
+    double sx = 0.0, sxx = 0.0;
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) {
+        const double u = (double)i - x0;
+        ex[i] = i < n ? dexp(-(u * u) * k) : 0.0;
+        sx += ex[i];
+        sxx = dfma(ex[i], ex[i], sxx);
+    }
+    double sy = 0.0, syy = 0.0, sed = 0.0, sd = 0.0, sdd = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double v = (double)j - y0;
+        const double ey = dexp(-(v * v) * k);
+        sy += ey;
+        syy = dfma(ey, ey, syy);
+        const FrameT *row = src + (size_t)j * nx;
+        double red = 0.0;
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) {
+            if (i < n) {
+                const double d = (double)row[i];
+                red = dfma(ex[i], d, red);
+                sd += d;
+                sdd = dfma(d, d, sdd);
+            }
+        }
+        sed = dfma(ey, red, sed);
+    }
+    const double N = (double)(n * n);
+    return ((A * A) * (sxx * syy) + 2.0 * (A * b) * (sx * sy) - 2.0 * A * sed) + ((N * b) * b - 2.0 * b * sd + sdd);
+}
+
+/* |f| at candidate c's reference start point (source/Localization.cpp:138-142, :485-491: the
+ * particle's intensity, background and position, the initial width; rho 0) over the window the
+ * fit uses (frames in global memory); NaN when the window leaves the frame or the request is
+ * invalid.  At that point every model is the same round Gaussian e(i, j) = Ex(i) Ey(j), so
+ * sum_ij (A e + b - D)^2 = A^2 Sxx Syy + 2 A b Sx Sy - 2 A sum_j Ey(j) sum_i Ex(i) D(i, j)
+ *                          + N b^2 - 2 b sum D + sum D^2
+ * costs 2n exponentials and one multiply-add per pixel.  The net-start fallback compares a
+ * net-start fit's final |f| with it (the normalized residuals chi/sqrt(dof) share the dof). */
+template <typename FrameT>
+LZR_HD LZR_INLINE double start_residual(const LaneConfig &cfg, const LzrCandidate &c, const FrameT *frames, int nx,
+                                        int ny, int nframes) {
+    LZR_NO_CONTRACT
+    LaneState<5> s;
+    LzrFit unused;
+    if (!lane_start<5>(s, cfg, c, 0, nx, ny, nframes, LZR_MODEL_SYMMETRIC, unused)) return __builtin_nan("");
+    const int n = s.n;
+    const double k = 0.5 / (cfg.initial_psf_width * cfg.initial_psf_width);
+    const double x0 = c.x - s.xoff, y0 = c.y - s.yoff; /* window coordinates */
+    const FrameT *src = frames + ((size_t)s.frame * ny + s.yoff) * nx + s.xoff;
+    const double A = c.intensity, b = c.background;
+    static_assert(2 * LZR_MAX_HALF_WIDTH + 1 <= 31, "start_residual window classes");
+    double sse;
+    if (n <= 11)
+        sse = start_sse<11>(src, nx, n, x0, y0, k, A, b);
+    else if (n <= 19)
+        sse = start_sse<19>(src, nx, n, x0, y0, k, A, b);
+    else
+        sse = start_sse<31>(src, nx, n, x0, y0, k, A, b);
+    return dsqrt(dmax(sse, 0.0)) * cfg.inv_sigma;
+}
+
 /* Epilogue entry: finishes a record written by store_final (no-op for records already
  * final, i.e. rejected before fitting). */
 LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c) {

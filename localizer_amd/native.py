@@ -90,7 +90,7 @@ def lib():
         L.lzr_abi_version.restype = ctypes.c_int
         i, d, f = ctypes.c_int, ctypes.c_double, ctypes.c_float
         L.lzr_fit_batch_device_start.argtypes = [vp, vp, i, i, i, vp, vp, i, vp, vp]
-        L.lzr_fit_batch_device_netinit.argtypes = [vp, vp, vp, i, i, i, vp, i, vp, vp, d, i, vp]
+        L.lzr_fit_batch_device_netinit.argtypes = [vp, vp, vp, i, i, i, vp, i, vp, vp, d, i, vp, vp]
         L.lzr_net_create.argtypes = [ctypes.POINTER(vp), d, ctypes.c_ulonglong]
         L.lzr_net_destroy.argtypes = [vp]
         L.lzr_net_last_error.argtypes = [vp, ctypes.c_char_p, i]
@@ -189,13 +189,16 @@ class Context:
         return rc
 
     def fit_device_netinit(self, net, d_frames, nx, ny, nframes, d_cands, ncands, d_out, d_work,
-                           residual_factor=3.0, refit_rejected=False, stream=0):
-        """lzr_fit_batch_device_netinit: network start, fallback to the reference start."""
+                           residual_factor=3.0, refit_rejected=False, d_nrefit=0, stream=0):
+        """lzr_fit_batch_device_netinit: network start, fallback to the reference start for
+        fits whose residual exceeds residual_factor times the residual at the reference start
+        and, with refit_rejected, for rejected or non-converged fits; d_nrefit: optional device
+        uint32 that receives the number of refits."""
         rc = lib().lzr_fit_batch_device_netinit(self._h, net._h, ctypes.c_void_p(d_frames), int(nx), int(ny),
                                                 int(nframes), ctypes.c_void_p(d_cands), int(ncands),
                                                 ctypes.c_void_p(d_out), ctypes.c_void_p(d_work),
                                                 float(residual_factor), int(bool(refit_rejected)),
-                                                ctypes.c_void_p(stream))
+                                                ctypes.c_void_p(d_nrefit), ctypes.c_void_p(stream))
         if rc < 0:
             raise LocalizerError(_error(self._h))
         return rc

@@ -53,9 +53,8 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
     }
 }
 
-extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, int max_h, int diag_rows,
-                             int max_iter, double epsabs, double epsrel, const uint16_t *frames, int nx, int ny,
-                             int nframes, const LzrCandidate *cands, int ncands, LzrFit *out) {
+static LaneConfig make_cfg(int model, double w, double sigma, int default_h, int max_h, int diag_rows, int max_iter,
+                           double epsabs, double epsrel) {
     LaneConfig cfg;
     cfg.initial_psf_width = w;
     cfg.sigma = sigma;
@@ -67,6 +66,13 @@ extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, i
     cfg.max_half_width = max_h;
     cfg.diag_update_rows = diag_rows;
     cfg.model = model;
+    return cfg;
+}
+
+extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, int max_h, int diag_rows,
+                             int max_iter, double epsabs, double epsrel, const uint16_t *frames, int nx, int ny,
+                             int nframes, const LzrCandidate *cands, int ncands, LzrFit *out) {
+    const LaneConfig cfg = make_cfg(model, w, sigma, default_h, max_h, diag_rows, max_iter, epsabs, epsrel);
     int accepted = 0;
     for (int i = 0; i < ncands; ++i) {
         const int m = cands[i].model >= 0 ? cands[i].model : model;
@@ -79,6 +85,14 @@ extern "C" int emu_fit_batch(int model, double w, double sigma, int default_h, i
         accepted += out[i].status == LZR_ACCEPTED;
     }
     return accepted;
+}
+
+// |f| at each candidate's reference start point (start_residual: the net-start fallback's
+// yardstick); NaN for a window outside the frame.
+extern "C" void emu_start_residual(int model, double w, double sigma, int default_h, int max_h, const uint16_t *frames,
+                                   int nx, int ny, int nframes, const LzrCandidate *cands, int ncands, double *out) {
+    const LaneConfig cfg = make_cfg(model, w, sigma, default_h, max_h, -1, 200, 1e-4, 1e-4);
+    for (int i = 0; i < ncands; ++i) out[i] = start_residual(cfg, cands[i], frames, nx, ny, nframes);
 }
 
 // One separable pass (symmetric P = 5, fixed width P = 4) at theta + dx over an n x n

@@ -31,6 +31,10 @@ def lib():
         L.emu_ell_pass_group.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p]
         L.emu_ell_pass_group.restype = ctypes.c_int
+        L.emu_start_residual.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_int, ctypes.c_void_p]
+        L.emu_start_residual.restype = None
         _LIB = L
     return _LIB
 
@@ -47,6 +51,20 @@ def fit_batch(frames, cands, model, w, sigma=1.0, default_half_width=0, max_half
     lib().emu_fit_batch(int(model), float(w), float(sigma), int(h), int(max(max_half_width, h)), int(diag_rows),
                         int(max_iter), float(epsabs), float(epsrel), frames.ctypes.data,
                         nx, ny, nf, cands.ctypes.data, cands.size, out.ctypes.data)
+    return out
+
+
+def start_residual(frames, cands, model, w, sigma=1.0, default_half_width=0):
+    """|f| at each candidate's reference start point (lzr_lane.h start_residual)."""
+    frames = np.ascontiguousarray(frames, np.uint16)
+    if frames.ndim == 2:
+        frames = frames[None]
+    nf, ny, nx = frames.shape
+    cands = np.ascontiguousarray(cands, abi.CANDIDATE_DTYPE)
+    h = default_half_width if default_half_width > 0 else abi.default_half_width(w)
+    out = np.zeros(cands.size)
+    lib().emu_start_residual(int(model), float(w), float(sigma), int(h), 15, frames.ctypes.data, nx, ny, nf,
+                             cands.ctypes.data, cands.size, out.ctypes.data)
     return out
 
 

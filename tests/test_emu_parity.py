@@ -115,3 +115,40 @@ def test_nonfinite_trial_residual_follows_gsl_norm(plain_20k):
     print(f"{hit} spots with a NaN-norm trial, {agree} follow the oracle")
     assert hit >= 3
     assert agree == hit
+
+
+@pytest.mark.parametrize("model", [abi.MODEL_SYMMETRIC, abi.MODEL_FIXED_WIDTH, abi.MODEL_ELLIPSOIDAL])
+def test_start_residual_matches_pixel_sum(plain_gpu_sample, model):
+    """The lane's residual at the reference start point (the net-start fallback's yardstick,
+    from the moment pass) equals the oracle's |f| summed pixel by pixel at that point
+    (source/Localization.cpp:138-142: intensity, initial width, position, background; rho 0),
+    to 1e-10 relative; windows outside the frame give NaN."""
+    frames, _, cands = plain_gpu_sample
+    cands = cands[:3000].copy()
+    # the default window, and every window class of the unrolled loops (n = 5 ... 31)
+    cands["half_width"] = np.random.default_rng(5).choice([0, 2, 5, 6, 9, 10, 15], len(cands))
+    cands["x"][:3] = [-4.0, 2.0, frames.shape[2] + 1.0]  # outside on the left / near edge / right
+    cands["half_width"][:3] = 0
+    w = synth.PLAIN_GPU_256.initial_psf_width
+    got = pyemu.start_residual(frames, cands, model, w)
+    ny, nx = frames.shape[1:]
+    want = np.full(len(cands), np.nan)
+    for i, c in enumerate(cands):
+        h = int(c["half_width"]) or abi.default_half_width(w)
+        n = 2 * h + 1
+        xoff, yoff = np.floor(c["x"] - h), np.floor(c["y"] - h)
+        if xoff < 0 or yoff < 0 or xoff + n > nx or yoff + n > ny:
+            continue
+        data = frames[c["frame"], int(yoff):int(yoff) + n, int(xoff):int(xoff) + n]
+        if model == abi.MODEL_SYMMETRIC:
+            _, f = po.model(model, [c["intensity"], w, c["x"], c["y"], c["background"]], data, n, xoff, yoff)
+        elif model == abi.MODEL_FIXED_WIDTH:
+            _, f = po.model_fixed_width([c["intensity"], c["x"], c["y"], c["background"]], w, data, n, xoff, yoff)
+        else:
+            _, f = po.model(model, [c["intensity"], w, w, c["x"], c["y"], 0.0, c["background"]], data, n, xoff, yoff)
+        want[i] = np.sqrt(np.sum(f * f))
+    assert np.array_equal(np.isnan(got), np.isnan(want)) and np.isnan(want[[0, 2]]).all()
+    ok = ~np.isnan(want)
+    rel = np.abs(got[ok] - want[ok]) / want[ok]
+    print(f"start residual model {model}: {ok.sum()} windows, max rel err {rel.max():.3g}")
+    assert rel.max() <= 1e-10

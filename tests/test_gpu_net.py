@@ -8,6 +8,7 @@ import pytest
 
 from localizer_amd import abi, device, native, netinit, synth
 from oracle import pynet
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -111,46 +112,141 @@ def test_gpu_start_points_drive_the_lm(rois):
     assert np.array_equal(got["params"][keep], plain["params"][keep])
 
 
-def test_gpu_trained_network_cuts_lm_work():
-    """Train on the GPU, then compare net-start LM (residual fallback) against the plain
-    LM on fresh ROIs: fewer trial evaluations, the same fits to 1e-3 px where both
-    accept.  (The LM's convergence is linear -- the reference's width column is the
-    true derivative / sqrt(2) -- so a better start saves a logarithmic number of steps.)"""
+@pytest.fixture(scope="module")
+def trained():
+    """A network trained on the GPU (1048576 ROIs, 512 steps of 2048) and 20000 fresh ROIs."""
     _say("case: train 1048576 ROIs (512 steps x 2048), evaluate 20000 fresh ROIs")
     with native.Net(W.initial_psf_width, seed=3) as net:
         stats = netinit.train(net, W, 1 << 20, epochs=1.0, batch=2048, lr=2e-3, log=None)
-        _say(f"train: {stats['steps']} steps in {stats['train_s']:.2f} s, losses {stats['losses']}")
-        assert stats["losses"][-1][1] < 0.3 * stats["losses"][0][1]
-        frames, _, cands = netinit.simulate(W, 20000, W.extra["eval_stream"])
-        n = len(cands)
-        d_frames, d_cands = device.upload(frames), device.upload(cands)
-        d_plain = device.empty(abi.FIT_DTYPE, n)
-        d_net = device.empty(abi.FIT_DTYPE, n)
-        d_first = device.empty(abi.FIT_DTYPE, n)
-        d_starts = device.empty(abi.START_DTYPE, n)
-        d_work = device.empty(abi.START_DTYPE, n)
-        s = device.stream_handle()
-        with native.Context(abi.MODEL_SYMMETRIC, W.initial_psf_width, default_half_width=5) as ctx:
-            ctx.fit_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_plain.ptr, s)
-            net.predict_device(d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_starts.ptr, s)
-            ctx.fit_device_start(d_frames.ptr, 11, 11, n, d_cands.ptr, d_starts.ptr, n, d_first.ptr, s)
-            ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, n, d_cands.ptr, n, d_net.ptr, d_work.ptr, 3.0,
-                                   stream=s)
-        plain, first, final, work = d_plain.numpy(), d_first.numpy(), d_net.numpy(), d_work.numpy()
-    refit = work["width"] == 0.0
-    assert np.all((work["width"] == 0.0) | (work["width"] == -1.0))
-    # the net-init result is the net-start fit, refitted from the reference start where flagged
-    keep = ~refit
-    assert np.array_equal(final["params"][keep], first["params"][keep])
+        params = net.get_params()
+    _say(f"train: {stats['steps']} steps in {stats['train_s']:.2f} s, losses {stats['losses']}")
+    frames, _, cands = netinit.simulate(W, 20000, W.extra["eval_stream"])
+    return stats, params, frames, cands
+
+
+def _start_chi(frames, cands, h=abi.NET_HALF_WIDTH):
+    """|f| at the reference start point (intensity, initial width, position, background of
+    the candidate; source/Localization.cpp:138-142) by the oracle's model; NaN outside."""
+    n = 2 * h + 1
+    ny, nx = frames.shape[1:]
+    out = np.full(len(cands), np.nan)
+    for i, c in enumerate(cands):
+        xoff, yoff = np.floor(c["x"] - h), np.floor(c["y"] - h)
+        if xoff < 0 or yoff < 0 or xoff + n > nx or yoff + n > ny:
+            continue
+        xo, yo = int(xoff), int(yoff)
+        data = frames[c["frame"], yo:yo + n, xo:xo + n]
+        theta = [c["intensity"], W.initial_psf_width, c["x"], c["y"], c["background"]]
+        _, f = po.model(abi.MODEL_SYMMETRIC, theta, data, n, xoff, yoff)
+        out[i] = np.sqrt(np.sum(f * f))
+    return out
+
+
+def _net_runs(params, frames, cands, refit_rejected=False):
+    """plain, net-start, net-init records, the net's starts, the refit marks and the device
+    refit count for one set of network weights."""
+    n = len(cands)
+    d_frames, d_cands = device.upload(frames), device.upload(cands)
+    d_plain, d_first, d_final = (device.empty(abi.FIT_DTYPE, n) for _ in range(3))
+    d_starts, d_work = device.empty(abi.START_DTYPE, n), device.empty(abi.START_DTYPE, n)
+    d_nrefit = device.upload(np.array([12345], np.uint32))
+    s = device.stream_handle()
+    with native.Net(W.initial_psf_width) as net, \
+            native.Context(abi.MODEL_SYMMETRIC, W.initial_psf_width, default_half_width=5) as ctx:
+        net.set_params(params)
+        ctx.fit_device(d_frames.ptr, 11, 11, len(frames), d_cands.ptr, n, d_plain.ptr, s)
+        net.predict_device(d_frames.ptr, 11, 11, len(frames), d_cands.ptr, n, d_starts.ptr, s)
+        ctx.fit_device_start(d_frames.ptr, 11, 11, len(frames), d_cands.ptr, d_starts.ptr, n, d_first.ptr, s)
+        ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, len(frames), d_cands.ptr, n, d_final.ptr, d_work.ptr,
+                               3.0, refit_rejected=refit_rejected, d_nrefit=d_nrefit.ptr, stream=s)
+    out = dict(plain=d_plain.numpy(), first=d_first.numpy(), final=d_final.numpy(), starts=d_starts.numpy(),
+               work=d_work.numpy(), nrefit=int(d_nrefit.numpy()[0]))
+    assert np.all((out["work"]["width"] == 0.0) | (out["work"]["width"] == -1.0))
+    out["refit"] = out["work"]["width"] == 0.0
+    return out
+
+
+def _check_fallback(r, chi0, refit_rejected):
+    """The refit set is exactly what the criterion names, refitted records are the plain
+    records bit for bit, and the others are the net-start records."""
+    refit, used = r["refit"], r["starts"]["width"] > 0.0
+    first, final, plain = r["first"], r["final"], r["plain"]
+    assert r["nrefit"] == int(refit.sum())
+    assert not np.any(refit & ~used)  # a reference-start fit is never refitted
+    assert final[refit].tobytes() == plain[refit].tobytes()
+    assert final[~refit].tobytes() == first[~refit].tobytes()
+    failed = (first["status"] != abi.ACCEPTED) | (first["lm_status"] != abi.LM_SUCCESS)
+    high = (first["status"] == abi.ACCEPTED) & (first["chi"] > 3.0 * chi0)
+    want = used & ((refit_rejected & failed) | high)
+    # the device evaluates the start residual by moments, the oracle pixel by pixel
+    tie = np.abs(first["chi"] - 3.0 * chi0) <= 1e-9 * np.abs(first["chi"])
+    assert np.all((refit == want) | tie), np.flatnonzero((refit != want) & ~tie)[:10]
+    return failed, high
+
+
+def _xy_vs_oracle(final, ref):
+    """accepted/rejected agreement, x/y within 1e-3 px where both accept, both-accepted count"""
+    acc, racc = final["status"] == 0, ref["status"] == 0
+    both = acc & racc
+    dxy = np.abs(final["params"][both][:, 2:4] - ref["params"][both][:, 2:4]).max(axis=1)
+    return float(np.mean(acc == racc)), float(np.mean(dxy <= 1e-3)), int(both.sum())
+
+
+def test_gpu_trained_network_cuts_lm_work(trained):
+    """Net-start LM (with the residual fallback) against the plain LM on fresh ROIs: fewer
+    trial evaluations, the same fits to 1e-3 px where both accept, and the FP64 oracle's
+    records (from the reference start) to the same bar.  (The LM's convergence is linear
+    -- the reference's width column is the true derivative / sqrt(2) -- so a better start
+    saves a logarithmic number of steps.)  Accepted/rejected agreement is ~0.96, not 0.999:
+    ~3.6 % of the ROIs are dim spots (100-500 photons) that the reference start drives into
+    a width / amplitude / position rejection (often at the iteration cap) while the net start
+    converges to an accepted fit."""
+    stats, params, frames, cands = trained
+    assert stats["losses"][-1][1] < 0.3 * stats["losses"][0][1]
+    n = len(cands)
+    r = _net_runs(params, frames, cands)
+    chi0 = _start_chi(frames, cands)
+    _check_fallback(r, chi0, False)
+    plain, first, final, refit = r["plain"], r["first"], r["final"], r["refit"]
     t_plain = plain["trials"].mean()
     t_net = (first["trials"].sum() + final["trials"][refit].sum()) / n
     acc = plain["status"] == 0
     both = acc & (final["status"] == 0)
     dxy = np.abs(final["params"][both][:, 2:4] - plain["params"][both][:, 2:4]).max(axis=1)
+    ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, W.initial_psf_width, default_half_width=5)
+    st_agree, xy_ok, nboth = _xy_vs_oracle(final, ref)
     _say(f"trials per ROI: plain {t_plain:.2f} (accepted {plain['trials'][acc].mean():.2f}), net-init {t_net:.2f} "
-         f"(accepted {first['trials'][acc].mean():.2f}), fallback {refit.mean():.4f}; accepted plain {acc.mean():.4f} "
-         f"net-init {np.mean(final['status'] == 0):.4f}; xy within 1e-3 where both accept: {np.mean(dxy <= 1e-3):.5f}")
+         f"(accepted {first['trials'][acc].mean():.2f}), fallback {refit.mean():.4f} ({r['nrefit']} on the device); "
+         f"accepted plain {acc.mean():.4f} net-init {np.mean(final['status'] == 0):.4f}; xy within 1e-3 where both "
+         f"accept: {np.mean(dxy <= 1e-3):.5f}; vs FP64 oracle: status {st_agree:.5f}, xy {xy_ok:.5f} of {nboth}")
     assert first["trials"][acc].mean() < 0.8 * plain["trials"][acc].mean()
     assert t_net < 0.85 * t_plain
     assert np.sum(acc & (final["status"] != 0)) <= 0.01 * acc.sum()
     assert np.mean(dxy <= 1e-3) >= 0.98
+    assert st_agree >= 0.95 and xy_ok >= 0.98  # measured 0.960 and 0.988
+
+
+@pytest.mark.parametrize("refit_rejected", [True, False])
+def test_gpu_corrupted_network_falls_back_to_plain_records(trained, refit_rejected):
+    """A trained network with a corrupted last layer (x start 2.5 px off, width e times too
+    wide): the fallback refits what the criterion names -- rejected or non-converged fits
+    (refit_rejected) and fits whose residual exceeds 3x the residual at the reference start --
+    those records are the plain records bit for bit, and the device counts the refits."""
+    _, params, frames, cands = trained
+    bad = params.copy()
+    o = abi.NET_OFF["fc2_b"][0]
+    bad[o + 1] += 1.0
+    bad[o + 2] += 2.5
+    r = _net_runs(bad, frames, cands, refit_rejected)
+    chi0 = _start_chi(frames, cands)
+    failed, high = _check_fallback(r, chi0, refit_rejected)
+    used = r["starts"]["width"] > 0.0
+    _say(f"corrupted network (refit_rejected={refit_rejected}): net start used {used.mean():.4f}, "
+         f"net-start fits failed {np.mean(failed & used):.4f}, high residual {np.mean(high & used):.4f}, "
+         f"refitted {r['refit'].mean():.4f} ({r['nrefit']} counted on the device)")
+    if refit_rejected:
+        assert r["refit"].mean() >= 0.05
+        ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, W.initial_psf_width, default_half_width=5)
+        st_agree, xy_ok, nboth = _xy_vs_oracle(r["final"], ref)
+        _say(f"vs FP64 oracle: status {st_agree:.5f}, xy within 1e-3 {xy_ok:.5f} of {nboth}")
+        assert st_agree >= 0.95 and xy_ok >= 0.98
