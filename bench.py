@@ -158,6 +158,72 @@ NET_FLOPS_PER_ROI = 2.0 * (81 * 32 * 9 + 49 * 32 * 288 + 1568 * 64 + 64 * 5)
 PEAK_BF16_TFLOPS = 2500.0
 
 
+def controller_leg(ctx, w, frames, cands, fits, log, thread_counts=(16, 64, 256), inflight=4, direct_frames=400):
+    """The reference controller's call pattern: nthreads workers sharing one fitter, one call
+    per frame (source/PALMAnalysis.cpp:204-252, call at :234), host buffers in and out,
+    through the host batcher (lzr_batcher_fit_frame); beside it the same pattern with one
+    lzr_fit_batch call per frame."""
+    import threading
+
+    order = np.argsort(cands["frame"], kind="stable")
+    sc = cands[order]
+    bounds = np.searchsorted(sc["frame"], np.arange(w.nframes + 1))
+    lists = []
+    for f in range(w.nframes):
+        c = sc[bounds[f]:bounds[f + 1]].copy()
+        c["frame"] = 0
+        lists.append(c)
+
+    def run(fn, nf):
+        out, errs = [None] * nf, []
+
+        def work(t):
+            try:
+                for f in range(t, nf, nthreads):
+                    out[f] = fn(f)
+            except Exception as e:  # noqa: BLE001 -- reported
+                errs.append(repr(e))
+
+        th = [threading.Thread(target=work, args=(t,)) for t in range(nthreads)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if errs:
+            raise RuntimeError(errs[0])
+        return time.perf_counter() - t0, out
+
+    want = fits[order].tobytes()
+    sweep = []
+    for nt in thread_counts:
+        nthreads = nt
+        with native.Batcher(ctx, w.nx, w.ny, max_frames=256, max_rois=1 << 15, max_wait_us=500,
+                            inflight=inflight) as b:
+            run(lambda f: b.fit_frame(frames[f], lists[f]), min(256, w.nframes))  # warm-up
+            st0 = b.stats()
+            sec, out = run(lambda f: b.fit_frame(frames[f], lists[f]), w.nframes)
+            st1 = b.stats()
+        got = np.concatenate(out)
+        acc = int(np.sum(got["status"] == 0))
+        nb = st1["batches"] - st0["batches"]
+        sweep.append({"threads": nt, "seconds": sec, "frames_per_s": w.nframes / sec, "localizations_per_s": acc / sec,
+                      "batches": nb, "mean_frames_per_batch": w.nframes / max(nb, 1),
+                      "records_identical_to_device_call": bool(got.tobytes() == want)})
+        log(f"controller pattern ({nt} threads, one call per frame, {inflight} batches in flight): "
+            f"{sweep[-1]['frames_per_s']:.0f} frames/s ({sweep[-1]['localizations_per_s']:.4g} loc/s, "
+            f"{sweep[-1]['mean_frames_per_batch']:.1f} frames per batch, identical "
+            f"{sweep[-1]['records_identical_to_device_call']})")
+    nthreads = thread_counts[0]
+    nd = min(direct_frames, w.nframes)
+    sec_d, _ = run(lambda f: ctx.fit(frames[f], lists[f]), nd)
+    log(f"one lzr_fit_batch call per frame ({nthreads} threads): {nd / sec_d:.0f} frames/s")
+    return {"frames": w.nframes, "inflight": inflight, "batcher": sweep,
+            "one_call_per_frame": {"threads": nthreads, "frames": nd, "frames_per_s": nd / sec_d,
+                                   "entry": "lzr_fit_batch per frame (nframes 1), calls serialised on the context"},
+            "entry": "lzr_batcher_fit_frame from Python threads (host uint16 frame in, records out)"}
+
+
 def netinit_vs_oracle(W, frames, cands, plain, net, m):
     """Plain and net-init records against the FP64 oracle (reference start) on the first m
     ROIs: accepted/rejected agreement, x/y within 1e-3 px and A, r, b within 1e-4 relative
@@ -580,6 +646,7 @@ def main():
     ap.add_argument("--block-frames", type=int, default=10000, help="frames per graph-replayed block (mixed leg)")
     ap.add_argument("--no-dense", action="store_true", help="skip the dense end-to-end stream leg")
     ap.add_argument("--no-render", action="store_true", help="skip the localization-bitmap leg")
+    ap.add_argument("--no-controller", action="store_true", help="skip the per-frame (controller pattern) leg")
     ap.add_argument("--dense-frames", type=int, default=1000)
     ap.add_argument("--dense-block", type=int, default=50, help="frames per block of the dense stream")
     ap.add_argument("--dense-pipelines", type=int, default=DENSE_PIPELINES, help="frame blocks in flight")
@@ -675,6 +742,9 @@ def main():
     pcie = pcie_inclusive(ctx, frames, cands)
     print(f"pcie-inclusive (host buffers): {pcie['ms_per_call']:.2f} ms per call, {pcie['value']:.4g} loc/s",
           file=sys.stderr, flush=True)
+    controller = None
+    if not args.no_controller:
+        controller = controller_leg(ctx, w, frames, cands, fits, lambda m: print(m, file=sys.stderr, flush=True))
     parity = None
     if not args.no_cpu:
         parity = parity_vs_oracle(w, frames, cands, fits, nthreads)
@@ -748,6 +818,7 @@ def main():
         "cpu_baseline": cpu,
         "parity_vs_fp64_oracle": parity,
         "pcie_inclusive": pcie,
+        "controller_pattern": controller,
         "net_init": net,
         "mixed": mixed,
         "render": render,

@@ -115,6 +115,7 @@ typedef struct LzrContext LzrContext;
 #define LZR_OPT_ROUNDS 8           /* fitting-kernel launches per model class; unfinished fits are parked between them (default 1) */
 #define LZR_OPT_PARK_THRESHOLD 9   /* a wave parks its fits once the queue is drained and at most this many lanes are busy (default 16) */
 #define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues in the straggler kernel, on 1..16 lanes by the count of such fits (default 48, 0 = off) */
+#define LZR_OPT_DEFAULT_HALF_WIDTH 11 /* window half-width of candidates with half_width 0 (read-only: set at lzr_create) */
 
 /* Creates a fitting context on the current HIP device.
  * Replaces the constructors FitPositions_SymmetricGaussian(initialPSFWidth, sigma)
@@ -125,6 +126,11 @@ typedef struct LzrContext LzrContext;
 LZR_EXPORT int lzr_create(LzrContext **ctx, int model, double initial_psf_width, double sigma,
                           int default_half_width);
 /* (also FitPositions_FixedWidthGaussian(initialPSFWidth, sigma), source/Localization.h:83-95) */
+
+/* A second context with this one's model, widths, sigma and options (its own work memory
+ * and streams): fits on the two are identical, and the two may fit at the same time.
+ * (No reference counterpart: a FitPositions instance is stateless between calls.) */
+LZR_EXPORT int lzr_clone(LzrContext *ctx, LzrContext **out);
 
 /* Releases a context and its device memory.  Returns 0 or -1.
  * Replaces the fitter's destructor and LocalizerFree (source/LocalizerDLL.cpp:131-133). */
@@ -396,5 +402,44 @@ LZR_EXPORT int lzr_render(LzrRenderer *renderer, const LzrPosition *positions, i
  * *d_count (device) receives the number written. */
 LZR_EXPORT int lzr_positions_from_fits_device(LzrRenderer *renderer, const LzrFit *d_fits, int n_fits,
                                               LzrPosition *d_positions, int *d_count, void *stream);
+
+/* ---- host batcher: the reference's per-frame call pattern ------------------------ */
+
+/* The reference's analysis controller shares one fitter between hardware_concurrency()
+ * worker threads, each calling fit_positions(image, particles) once per frame
+ * (source/PALMAnalysis.cpp:96-130, :204-252, call at :234; source/Localization.h:61).
+ * A batcher gathers such concurrent per-frame calls into one device launch: each caller
+ * deposits its candidates' fitting windows (only those pixels travel, through pinned
+ * memory) and blocks until its batch is fitted.  Every record equals the one
+ * lzr_fit_batch returns for that frame alone (nframes = 1), bit for bit.
+ * nx, ny: the frame size of every call; max_frames / max_rois: one batch's capacity
+ * (frames, candidates; window pixels for max_rois candidates at the default window);
+ * max_wait_us: how long an open batch waits for more callers after its first one (it
+ * is launched earlier when full or when every caller inside the batcher has joined it);
+ * inflight (1..64): batches fitted at the same time, each on its own clone of ctx
+ * (lzr_clone at creation: options set on ctx later do not reach the batcher). */
+typedef struct LzrBatcher LzrBatcher;
+LZR_EXPORT int lzr_batcher_create(LzrBatcher **batcher, LzrContext *ctx, int nx, int ny, int max_frames,
+                                  int max_rois, int max_wait_us, int inflight);
+
+/* One frame (nx*ny, x fast) and its candidates (frame field 0; any other value is an
+ * invalid request, as in a one-frame lzr_fit_batch call).  Safe to call from any number
+ * of threads.  Returns the number of accepted fits, or -1 (frame size differs from the
+ * batcher's, the frame exceeds one batch's capacity, or a device error). */
+LZR_EXPORT int lzr_batcher_fit_frame(LzrBatcher *batcher, const uint16_t *frame, int nx, int ny,
+                                     const LzrCandidate *candidates, int ncandidates, LzrFit *results);
+
+/* The same for an FP64 frame (the reference's Image, column-major with rows = x, i.e. x
+ * fast): the pixels inside fitting windows must be integer counts in [0, 65535] (-1
+ * otherwise, for this call only). */
+LZR_EXPORT int lzr_batcher_fit_frame_f64(LzrBatcher *batcher, const double *frame, int nx, int ny,
+                                         const LzrCandidate *candidates, int ncandidates, LzrFit *results);
+
+/* Batches launched and frames fitted so far. */
+LZR_EXPORT int lzr_batcher_stats(LzrBatcher *batcher, unsigned long long *batches, unsigned long long *frames);
+/* Error text of the last failing call (batcher NULL: of lzr_batcher_create). */
+LZR_EXPORT int lzr_batcher_last_error(const LzrBatcher *batcher, char *buf, int len);
+/* Stops the worker and frees the buffers; no call may be in progress. */
+LZR_EXPORT int lzr_batcher_destroy(LzrBatcher *batcher);
 
 #endif /* LOCALIZER_HIP_H */

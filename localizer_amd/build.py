@@ -69,7 +69,7 @@ def build_hip():
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "lzr_fit.hip"), os.path.join(CSRC, "lzr_net.hip"),
           os.path.join(CSRC, "lzr_find.hip"), os.path.join(CSRC, "lzr_pipeline.hip"),
-          os.path.join(CSRC, "lzr_render.hip")])
+          os.path.join(CSRC, "lzr_render.hip"), os.path.join(CSRC, "lzr_batch.hip")])
     os.replace(tmp, out)
 
 

@@ -1096,6 +1096,44 @@ LZR_EXPORT int lzr_create(LzrContext **out, int model, double initial_psf_width,
     return 0;
 }
 
+LZR_EXPORT int lzr_clone(LzrContext *src, LzrContext **out) {
+    if (!out) {
+        g_create_error = "lzr_clone: null output pointer";
+        return -1;
+    }
+    *out = nullptr;
+    if (!src) {
+        g_create_error = "lzr_clone: null context";
+        return -1;
+    }
+    LaneConfig cfg;
+    int device;
+    {
+        std::lock_guard<std::mutex> lock(src->mu);
+        cfg = src->cfg;
+        device = src->device;
+    }
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    if (hipSetDevice(device) != hipSuccess) {
+        g_create_error = "lzr_clone: hipSetDevice failed";
+        return -1;
+    }
+    LzrContext *c = nullptr;
+    const int rc = lzr_create(&c, cfg.model, cfg.initial_psf_width, cfg.sigma, cfg.default_half_width);
+    (void)hipSetDevice(prev);
+    if (rc != 0) return -1;
+    std::lock_guard<std::mutex> lock(src->mu);
+    c->cfg = src->cfg;
+    c->waves_per_cu = src->waves_per_cu;
+    c->occupancy5 = src->occupancy5;
+    c->rounds = src->rounds;
+    c->park_threshold = src->park_threshold;
+    c->strag_trials = src->strag_trials;
+    *out = c;
+    return 0;
+}
+
 LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     if (!ctx) return -1;
     (void)hipFree(ctx->d_frames);
@@ -1168,6 +1206,8 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
         if (!(value >= 1 && value <= LZR_MAX_HALF_WIDTH)) return fail(ctx, "max half-width out of range");
         ctx->cfg.max_half_width = std::max((int)value, ctx->cfg.default_half_width);
         return 0;
+    case LZR_OPT_DEFAULT_HALF_WIDTH:
+        return fail(ctx, "the default half-width is fixed at lzr_create");
     default:
         return fail(ctx, "unknown option");
     }
@@ -1360,6 +1400,7 @@ LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value) {
     case LZR_OPT_ROUNDS: *value = ctx->rounds; return 0;
     case LZR_OPT_PARK_THRESHOLD: *value = ctx->park_threshold; return 0;
     case LZR_OPT_STRAGGLER_TRIALS: *value = ctx->strag_trials; return 0;
+    case LZR_OPT_DEFAULT_HALF_WIDTH: *value = ctx->cfg.default_half_width; return 0;
     default: return fail(ctx, "unknown option");
     }
 }

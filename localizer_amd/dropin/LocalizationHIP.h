@@ -14,7 +14,9 @@
  * fit's rejections are, source/Localization.cpp:606-631), the same error convention
  * (std::runtime_error, source/Errors.h).  Instances are safe to call from several
  * threads at once (PALMAnalysisController shares one fitter between its workers,
- * source/PALMAnalysis.cpp:234): calls on one instance are serialised on the device.
+ * source/PALMAnalysis.cpp:234): concurrent fit_positions calls on one instance are
+ * gathered into device batches by the library's host batcher (lzr_batcher_*), which
+ * returns each frame the records a one-frame call would.
  *
  * Compile inside the reference code base with LZR_WITH_LOCALIZER_HEADERS defined (the
  * reference's Localization.h supplies FitPositions, Particle, Image and the
@@ -29,6 +31,7 @@
 #include <cstdint>
 #include <list>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -71,6 +74,7 @@ public:
             throw std::runtime_error("LocalizationHIP: " + last_error(nullptr));
     }
     ~FitterCore() {
+        if (batcher_) lzr_batcher_destroy(batcher_);
         if (ctx_) lzr_destroy(ctx_);
     }
     FitterCore(const FitterCore &) = delete;
@@ -98,17 +102,7 @@ public:
             }
             const int fi = (int)(stack.size() / ((size_t)nx * ny));
             append_frame(*images[f], stack);
-            for (const Particle &p : *lists[f]) {
-                LzrCandidate c;
-                c.x = p.x;
-                c.y = p.y;
-                c.intensity = p.intensity;
-                c.background = p.background;
-                c.frame = fi;
-                c.half_width = 0;
-                c.model = -1;
-                cands.push_back(c);
-            }
+            for (const Particle &p : *lists[f]) cands.push_back(candidate(p, fi));
         }
         if (cands.empty()) return out;
         std::vector<LzrFit> res(cands.size());
@@ -134,15 +128,74 @@ public:
         return out;
     }
 
+    /* One frame (the reference's fit_positions): through the batcher, created for the
+     * first frame's size; frames of another size, or more particles than one batch holds,
+     * take the one-call path. */
     template <class Container>
     std::shared_ptr<LocalizedPositionsContainer> fit_one(const ImagePtr image,
                                                          std::shared_ptr<std::list<Particle> > particles) {
-        std::vector<ImagePtr> im(1, image);
-        std::vector<std::shared_ptr<std::list<Particle> > > li(1, particles);
-        return fit_many<Container>(im, li)[0];
+        LzrBatcher *b = batcher_for(image->rows(), image->cols());
+        if (!b || particles->size() > (size_t)kBatchRois) {
+            std::vector<ImagePtr> im(1, image);
+            std::vector<std::shared_ptr<std::list<Particle> > > li(1, particles);
+            return fit_many<Container>(im, li)[0];
+        }
+        std::shared_ptr<LocalizedPositionsContainer> out(new Container());
+        if (particles->empty()) return out;
+        std::vector<LzrCandidate> cands;
+        cands.reserve(particles->size());
+        for (const Particle &p : *particles) cands.push_back(candidate(p, 0));
+        std::vector<LzrFit> res(cands.size());
+        if (lzr_batcher_fit_frame_f64(b, image->data(), (int)image->rows(), (int)image->cols(), cands.data(),
+                                      (int)cands.size(), res.data()) < 0) {
+            char buf[512];
+            lzr_batcher_last_error(b, buf, (int)sizeof buf);
+            throw std::runtime_error(std::string("LocalizationHIP: ") + buf);
+        }
+        size_t k = 0;
+        for (auto it = particles->begin(); it != particles->end();) {
+            const LzrFit &r = res[k++];
+            if (r.status == LZR_ACCEPTED) {
+                add_record(*out, r);
+                ++it;
+            } else if (erases(r.status)) {
+                it = particles->erase(it);
+            } else {
+                ++it;
+            }
+        }
+        return out;
     }
 
 private:
+    static constexpr int kBatchRois = 1 << 16;
+
+    static LzrCandidate candidate(const Particle &p, int frame) {
+        LzrCandidate c;
+        c.x = p.x;
+        c.y = p.y;
+        c.intensity = p.intensity;
+        c.background = p.background;
+        c.frame = frame;
+        c.half_width = 0;
+        c.model = -1;
+        return c;
+    }
+
+    LzrBatcher *batcher_for(long nx, long ny) {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (!batcher_ && !batcher_failed_) {
+            if (lzr_batcher_create(&batcher_, ctx_, (int)nx, (int)ny, 64, kBatchRois, 500, 4) != 0) {
+                batcher_ = nullptr;
+                batcher_failed_ = true;
+            } else {
+                bnx_ = nx;
+                bny_ = ny;
+            }
+        }
+        return batcher_ && nx == bnx_ && ny == bny_ ? batcher_ : nullptr;
+    }
+
     /* The symmetric-PSF filter's rejections stay in the caller's list (the reference
      * erases only the ellipsoidal fit's rejections). */
     bool erases(int status) const {
@@ -198,6 +251,10 @@ private:
 
     LzrContext *ctx_ = nullptr;
     int model_;
+    std::mutex mu_;
+    LzrBatcher *batcher_ = nullptr;
+    bool batcher_failed_ = false;
+    long bnx_ = 0, bny_ = 0;
 };
 
 }  // namespace lzr_dropin

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libl
 # Every function declared in include/localizer_hip.h.
 EXPORTED = (
     "lzr_create",
+    "lzr_clone",
     "lzr_destroy",
     "lzr_last_error",
     "lzr_set_option",
@@ -53,6 +54,12 @@ EXPORTED = (
     "lzr_render_device",
     "lzr_render",
     "lzr_positions_from_fits_device",
+    "lzr_batcher_create",
+    "lzr_batcher_fit_frame",
+    "lzr_batcher_fit_frame_f64",
+    "lzr_batcher_stats",
+    "lzr_batcher_last_error",
+    "lzr_batcher_destroy",
 )
 
 
@@ -118,9 +125,17 @@ def lib():
         L.lzr_free.restype = None
         L.lzr_pipeline_last_error.argtypes = [ctypes.c_char_p, i]
         L.lzr_pipeline_last_error.restype = ctypes.c_int
+        L.lzr_batcher_create.argtypes = [ctypes.POINTER(vp), vp, i, i, i, i, i, i]
+        L.lzr_clone.argtypes = [vp, ctypes.POINTER(vp)]
+        L.lzr_clone.restype = ctypes.c_int
+        L.lzr_batcher_fit_frame.argtypes = [vp, vp, i, i, vp, i, vp]
+        L.lzr_batcher_fit_frame_f64.argtypes = [vp, vp, i, i, vp, i, vp]
+        L.lzr_batcher_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
+        L.lzr_batcher_last_error.argtypes = [vp, ctypes.c_char_p, i]
+        L.lzr_batcher_destroy.argtypes = [vp]
         for name in EXPORTED:
             if name.startswith(("lzr_net", "lzr_fit_batch_device_", "lzr_graph", "lzr_get", "lzr_autotune", "lzr_find",
-                                "lzr_render", "lzr_positions")):
+                                "lzr_render", "lzr_positions", "lzr_batcher")):
                 getattr(L, name).restype = ctypes.c_int
         if L.lzr_abi_version() != 1:
             raise LocalizerError("ABI version mismatch")
@@ -442,3 +457,56 @@ class Renderer:
         if lib().lzr_positions_from_fits_device(self._h, ctypes.c_void_p(d_fits), int(n), ctypes.c_void_p(d_positions),
                                                 ctypes.c_void_p(d_count), ctypes.c_void_p(stream)) != 0:
             raise LocalizerError(_render_error(self._h))
+
+
+def _batcher_error(h):
+    buf = ctypes.create_string_buffer(1024)
+    lib().lzr_batcher_last_error(h, buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+class Batcher:
+    """lzr_batcher_*: concurrent per-frame fit calls (the reference controller's worker
+    threads) gathered into one device launch.  fit_frame may be called from any number
+    of threads (ctypes releases the GIL during the call)."""
+
+    def __init__(self, ctx, nx, ny, max_frames=64, max_rois=1 << 16, max_wait_us=500, inflight=4):
+        h = ctypes.c_void_p()
+        if lib().lzr_batcher_create(ctypes.byref(h), ctx._h, int(nx), int(ny), int(max_frames), int(max_rois),
+                                    int(max_wait_us), int(inflight)) != 0:
+            raise LocalizerError(_batcher_error(None))
+        self._h = h
+        self._destroy = lib().lzr_batcher_destroy
+        self.nx, self.ny = int(nx), int(ny)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def fit_frame(self, frame, candidates):
+        """One frame (ny x nx, uint16 or float64) and its candidates (frame field 0) ->
+        FIT_DTYPE records, equal to a one-frame lzr_fit_batch call."""
+        frame = np.asarray(frame)
+        f64 = frame.dtype == np.float64
+        frame = np.ascontiguousarray(frame, np.float64 if f64 else np.uint16)
+        ny, nx = frame.shape
+        cands = np.ascontiguousarray(candidates, abi.CANDIDATE_DTYPE)
+        out = np.zeros(cands.size, abi.FIT_DTYPE)
+        fn = lib().lzr_batcher_fit_frame_f64 if f64 else lib().lzr_batcher_fit_frame
+        if fn(self._h, frame.ctypes.data, nx, ny, cands.ctypes.data, cands.size, out.ctypes.data) < 0:
+            raise LocalizerError(_batcher_error(self._h))
+        return out
+
+    def stats(self):
+        b, f = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+        lib().lzr_batcher_stats(self._h, ctypes.byref(b), ctypes.byref(f))
+        return {"batches": b.value, "frames": f.value}
