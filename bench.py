@@ -36,7 +36,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per fitting call on the default workload (lm_persistent<5> two-wave build +
 # lm_straggler<5>), from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the reading of
 # a plain 512 MiB copy kernel (fetch x2.0, write x1.0).
-MEASURED_TRAFFIC_BYTES = {"plain-lm-256px-10000frames-9x9": 905.2e6}
+MEASURED_TRAFFIC_BYTES = {"plain-lm-256px-10000frames-9x9": 2320.2e6}
 # Algorithmic work per ROI (symmetric model): W = N[(1+T) F_f + (1+A)(F_J + F_NE)] + T F_solve
 F_F, F_J, F_NE, F_SOLVE = 12.0, 6.0, 40.0, 300.0
 F_F7, F_J7, F_NE7, F_SOLVE7 = 16.0, 30.0, 70.0, 700.0
@@ -640,6 +640,7 @@ def main():
     ap.add_argument("--park-threshold", type=int, default=-1, help="busy lanes at which a wave parks its fits")
     ap.add_argument("--straggler-trials", type=int, default=-1,
                     help="trials after which a fit moves to the 8-lane straggler kernel (0: off)")
+    ap.add_argument("--straggler-group", type=int, default=-1, help="lanes per handed-off fit (0: by count)")
     ap.add_argument("--no-net", action="store_true", help="skip the net-initialized LM leg")
     ap.add_argument("--only-net", action="store_true", help="run only the net-initialized LM leg (profiling)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-workload leg")
@@ -706,6 +707,8 @@ def main():
         ctx.set_option(abi.OPT_PARK_THRESHOLD, args.park_threshold)
     if args.straggler_trials >= 0:
         ctx.set_option(abi.OPT_STRAGGLER_TRIALS, args.straggler_trials)
+    if args.straggler_group >= 0:
+        ctx.set_option(abi.OPT_STRAGGLER_GROUP, args.straggler_group)
     stream = torch.cuda.current_stream()
 
     def step():

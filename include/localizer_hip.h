@@ -116,6 +116,7 @@ typedef struct LzrContext LzrContext;
 #define LZR_OPT_PARK_THRESHOLD 9   /* a wave parks its fits once the queue is drained and at most this many lanes are busy (default 16) */
 #define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues in the straggler kernel, on 1..16 lanes by the count of such fits (default 48, 0 = off) */
 #define LZR_OPT_DEFAULT_HALF_WIDTH 11 /* window half-width of candidates with half_width 0 (read-only: set at lzr_create) */
+#define LZR_OPT_STRAGGLER_GROUP 12  /* lanes per fit in the straggler kernel (64/lanes ROIs per wavefront): 1, 2, 4, 8, 16, or 0 (default) = by the number of handed-off fits */
 
 /* Creates a fitting context on the current HIP device.
  * Replaces the constructors FitPositions_SymmetricGaussian(initialPSFWidth, sigma)
@@ -198,11 +199,14 @@ LZR_EXPORT int lzr_graph_create(LzrContext *ctx, LzrGraph **graph, const uint16_
 LZR_EXPORT int lzr_graph_launch(LzrGraph *graph, void *stream);
 LZR_EXPORT int lzr_graph_destroy(LzrGraph *graph);
 
-/* Autotuner: times the launch configurations (register budget LZR_OPT_OCCUPANCY,
- * persistent grid LZR_OPT_WAVES_PER_CU, continuation rounds LZR_OPT_ROUNDS) on this
- * batch -- each fits the whole batch once into d_results -- and keeps the fastest in the
- * context.  *best_ms (may be NULL) receives its time.  Synchronous.  Returns 0 or -1.
- * (No reference counterpart.) */
+/* Autotuner: times launch configurations on this batch -- each fits the whole batch once
+ * into d_results -- and keeps the fastest in the context: first the register budget
+ * (LZR_OPT_OCCUPANCY), persistent grid (LZR_OPT_WAVES_PER_CU) and continuation rounds
+ * (LZR_OPT_ROUNDS), then with the best of those the hand-off threshold
+ * (LZR_OPT_STRAGGLER_TRIALS 32/48/96) and the straggler kernel's lanes per fit, i.e. its
+ * ROIs per wavefront (LZR_OPT_STRAGGLER_GROUP auto/1/2/4/8/16).  Every configuration returns
+ * the same records.  *best_ms (may be NULL) receives the best time.  Synchronous.  Returns 0
+ * or -1.  (No reference counterpart.) */
 LZR_EXPORT int lzr_autotune_device(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
                                    const LzrCandidate *d_candidates, int ncandidates, LzrFit *d_results,
                                    void *stream, double *best_ms);

@@ -45,6 +45,7 @@ OPT_ROUNDS = 8
 OPT_PARK_THRESHOLD = 9
 OPT_STRAGGLER_TRIALS = 10
 OPT_DEFAULT_HALF_WIDTH = 11
+OPT_STRAGGLER_GROUP = 12
 
 CANDIDATE_DTYPE = np.dtype(
     [

@@ -65,6 +65,7 @@ struct KernelArgs {
     void *strag;
     unsigned *strag_ctr;
     unsigned strag_entries; /* capacity of strag[] in entries; a class's region is its share */
+    int strag_lk;           /* log2 of the lanes per handed-off fit, -1: by the hand-off count */
 };
 
 /* Queue class of a model: 0 symmetric (5 parameters), 1 ellipsoidal and the
@@ -725,14 +726,23 @@ lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     unsigned *head = args.strag_ctr + 2 * cls + 1;
     int lk = kMaxGroup == 16 ? 4 : 3; /* log2 of the group size */
     while (lk > 0 && (unsigned long long)count << lk > 2ull * lanes_total) --lk; /* <= 2 fits per group */
+    if (args.strag_lk >= 0) lk = min(args.strag_lk, kMaxGroup == 16 ? 4 : 3); /* fixed by LZR_OPT_STRAGGLER_GROUP */
     /* The one-lane build runs two waves per SIMD and so holds four times the fits of 2-lane
      * groups, whose split pass saves little of a trial (every lane repeats the LM logic):
      * the classes that have that build use it instead of 2-lane groups. */
     if (Role != kAllGroups && lk == 1) lk = 0;
     lk = __builtin_amdgcn_readfirstlane(lk);
+    /* LZR_OPT_STRAGGLER_GROUP 1 runs one lane per fit in the one-wave-per-SIMD build */
+    const bool one_wave_single = args.strag_lk == 0;
     if constexpr (Role == kOneLane) {
-        if (lk == 0) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
+        if (lk == 0 && !one_wave_single) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
         return;
+    }
+    if constexpr (Role == kMultiLane) {
+        if (lk == 0 && one_wave_single) {
+            straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
+            return;
+        }
     }
     switch (lk) {
     case 4: straggler_groups<P, Acc, 4>(args, count, head, park, lds_words); break;
@@ -786,6 +796,7 @@ struct LzrContext {
     int rounds = 1;          /* launches per class; fits are parked between them */
     int park_threshold = 16; /* park a wave's fits when this few lanes are still busy */
     int strag_trials = 48;   /* hand a fit to the straggler kernel after this many trials */
+    int strag_group = 0;     /* lanes per handed-off fit (1, 2, 4, 8, 16), 0: by the hand-off count */
     Workspace ws;            /* the eager calls' work memory */
     hipStream_t side[kClasses - 1] = {}; /* classes 1, 2 run beside class 0 */
     hipEvent_t fork = nullptr, join[kClasses - 1] = {};
@@ -1005,6 +1016,7 @@ static void launch(LzrContext *ctx, Workspace &ws, const uint16_t *frames, int n
     a.strag = ws.strag;
     a.strag_ctr = strag_ctr;
     a.strag_entries = (unsigned)std::min(ws.strag_entries, (size_t)0xffffffffu);
+    a.strag_lk = ctx->strag_group > 0 ? __builtin_ctz((unsigned)ctx->strag_group) : -1;
     const size_t lds = launch_lds(ctx);
     /* The classes share nothing but the sorted queue: classes 1 and 2 run on the
      * context's side streams beside class 0, so one class's straggler tail overlaps
@@ -1130,6 +1142,7 @@ LZR_EXPORT int lzr_clone(LzrContext *src, LzrContext **out) {
     c->rounds = src->rounds;
     c->park_threshold = src->park_threshold;
     c->strag_trials = src->strag_trials;
+    c->strag_group = src->strag_group;
     *out = c;
     return 0;
 }
@@ -1208,6 +1221,13 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
         return 0;
     case LZR_OPT_DEFAULT_HALF_WIDTH:
         return fail(ctx, "the default half-width is fixed at lzr_create");
+    case LZR_OPT_STRAGGLER_GROUP: {
+        const int g = (int)value;
+        if (!((double)g == value && (g == 0 || g == 1 || g == 2 || g == 4 || g == 8 || g == 16)))
+            return fail(ctx, "straggler group must be 0, 1, 2, 4, 8 or 16 lanes");
+        ctx->strag_group = g;
+        return 0;
+    }
     default:
         return fail(ctx, "unknown option");
     }
@@ -1401,6 +1421,7 @@ LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value) {
     case LZR_OPT_PARK_THRESHOLD: *value = ctx->park_threshold; return 0;
     case LZR_OPT_STRAGGLER_TRIALS: *value = ctx->strag_trials; return 0;
     case LZR_OPT_DEFAULT_HALF_WIDTH: *value = ctx->cfg.default_half_width; return 0;
+    case LZR_OPT_STRAGGLER_GROUP: *value = ctx->strag_group; return 0;
     default: return fail(ctx, "unknown option");
     }
 }
@@ -1475,9 +1496,26 @@ LZR_EXPORT int lzr_graph_launch(LzrGraph *g, void *stream) {
 
 /* One candidate launch configuration of the autotuner. */
 struct TuneConfig {
-    int occupancy, waves_per_cu, rounds, park_threshold;
+    int occupancy, waves_per_cu, rounds, park_threshold, strag_trials, strag_group;
 };
 
+static TuneConfig tune_get(const LzrContext *ctx) {
+    return {ctx->occupancy5, ctx->waves_per_cu, ctx->rounds, ctx->park_threshold, ctx->strag_trials, ctx->strag_group};
+}
+
+static void tune_set(LzrContext *ctx, const TuneConfig &c) {
+    ctx->occupancy5 = c.occupancy;
+    ctx->waves_per_cu = c.waves_per_cu;
+    ctx->rounds = c.rounds;
+    ctx->park_threshold = c.park_threshold;
+    ctx->strag_trials = c.strag_trials;
+    ctx->strag_group = c.strag_group;
+}
+
+/* Two stages, each timing one launch per configuration (the first launch warms up):
+ * (1) the fitting kernel's register budget, grid and rounds; (2) with the best of (1), the
+ * hand-off threshold and the straggler kernel's lanes per fit, i.e. its ROIs per wavefront
+ * (64 / lanes).  Every configuration returns the same records. */
 LZR_EXPORT int lzr_autotune_device(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
                                    const LzrCandidate *d_cands, int ncands, LzrFit *d_results, void *stream,
                                    double *best_ms) {
@@ -1486,49 +1524,58 @@ LZR_EXPORT int lzr_autotune_device(LzrContext *ctx, const uint16_t *d_frames, in
     if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
     if (ncands == 0) return 0;
     if (!d_frames || !d_cands || !d_results) return fail(ctx, "null device pointer");
-    const TuneConfig saved{ctx->occupancy5, ctx->waves_per_cu, ctx->rounds, ctx->park_threshold};
-    std::vector<TuneConfig> cand;
-    for (int occ : {1, 2})
-        for (int wpc : {0, 2})
-            for (int rounds : {1, 2}) cand.push_back({occ, wpc, rounds, rounds > 1 ? 16 : saved.park_threshold});
+    const TuneConfig saved = tune_get(ctx);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     try {
         LZR_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = (hipStream_t)stream;
         LZR_CHECK(hipEventCreate(&e0));
         LZR_CHECK(hipEventCreate(&e1));
-        float best = 1e30f;
-        TuneConfig pick = saved;
-        for (size_t i = 0; i <= cand.size(); ++i) {
-            /* index 0 is a warm-up of the current configuration */
-            const TuneConfig c = i == 0 ? saved : cand[i - 1];
-            ctx->occupancy5 = c.occupancy;
-            ctx->waves_per_cu = c.waves_per_cu;
-            ctx->rounds = c.rounds;
-            ctx->park_threshold = c.park_threshold;
+        auto time_one = [&](const TuneConfig &c) {
+            tune_set(ctx, c);
             LZR_CHECK(hipEventRecord(e0, s));
             launch(ctx, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_results, s);
             LZR_CHECK(hipEventRecord(e1, s));
             LZR_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
             LZR_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            if (i > 0 && ms < best) {
-                best = ms;
-                pick = c;
+            return ms;
+        };
+        (void)time_one(saved); /* warm-up */
+        float best = 1e30f;
+        TuneConfig pick = saved;
+        for (int occ : {1, 2})
+            for (int wpc : {0, 2})
+                for (int rounds : {1, 2}) {
+                    TuneConfig c = saved;
+                    c.occupancy = occ;
+                    c.waves_per_cu = wpc;
+                    c.rounds = rounds;
+                    c.park_threshold = rounds > 1 ? 16 : saved.park_threshold;
+                    const float ms = time_one(c);
+                    if (ms < best) {
+                        best = ms;
+                        pick = c;
+                    }
+                }
+        const TuneConfig base = pick;
+        for (int trials : {32, 48, 96})
+            for (int group : {0, 1, 2, 4, 8, 16}) {
+                TuneConfig c = base;
+                c.strag_trials = trials;
+                c.strag_group = group;
+                const float ms = time_one(c);
+                if (ms < best) {
+                    best = ms;
+                    pick = c;
+                }
             }
-        }
-        ctx->occupancy5 = pick.occupancy;
-        ctx->waves_per_cu = pick.waves_per_cu;
-        ctx->rounds = pick.rounds;
-        ctx->park_threshold = pick.park_threshold;
+        tune_set(ctx, pick);
         if (best_ms) *best_ms = best;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     } catch (const std::string &e) {
-        ctx->occupancy5 = saved.occupancy;
-        ctx->waves_per_cu = saved.waves_per_cu;
-        ctx->rounds = saved.rounds;
-        ctx->park_threshold = saved.park_threshold;
+        tune_set(ctx, saved);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         return fail(ctx, "lzr_autotune_device: " + e);
