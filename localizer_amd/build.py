@@ -8,6 +8,8 @@ products:
     oracle/liblzr_oracle.so                FP64 oracle (tests / CPU baseline only)
     oracle/_ref/ref_models                 reference model functions (when the
                                            reference sources are present)
+    tests/_san/                            sanitizer builds of the host code
+                                           (`python -m localizer_amd.build sanitize`)
 """
 import os
 import shutil
@@ -86,6 +88,34 @@ def build_dropin_driver():
           "-Wl,-rpath,$ORIGIN/../../localizer_amd/lib", "-lpthread"])
 
 
+SAN = os.path.join(ROOT, "tests", "_san")
+ASAN = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
+TSAN = ["-O1", "-g", "-fsanitize=thread"]
+
+
+def build_sanitized():
+    """Host code under the sanitizers (tests only), into tests/_san/: the oracle and the lane
+    emulation as shared libraries and the drop-in positions containers' driver with
+    AddressSanitizer + UndefinedBehaviorSanitizer, and a ThreadSanitizer driver of the
+    oracle's frame-parallel pool (tests/cpp/tsan_main.cpp)."""
+    os.makedirs(SAN, exist_ok=True)
+    orc = [os.path.join(ROOT, "oracle", f) for f in ("lzr_oracle.c", "lzr_find_oracle.c", "lzr_render_oracle.c")]
+    _run(["gcc", *ASAN, "-fPIC", "-std=c11", "-shared", "-o", os.path.join(SAN, "liblzr_oracle.so"), *orc,
+          "-lm", "-lpthread"])
+    emu = os.path.join(ROOT, "tests", "emu")
+    _run(["g++", *ASAN, "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas", "-o",
+          os.path.join(SAN, "liblzr_emu.so"), os.path.join(emu, "lzr_emu.cpp")])
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    _run(["g++", *ASAN, "-std=c++17", "-o", os.path.join(SAN, "storage_main"), os.path.join(cpp, "storage_main.cpp")])
+    objs = []
+    for src in orc:
+        obj = os.path.join(SAN, os.path.basename(src)[:-2] + "_tsan.o")
+        _run(["gcc", *TSAN, "-std=c11", "-c", "-o", obj, src])
+        objs.append(obj)
+    _run(["g++", *TSAN, "-std=c++17", "-o", os.path.join(SAN, "tsan_main"), os.path.join(cpp, "tsan_main.cpp"), *objs,
+          "-lm", "-lpthread"])
+
+
 def build_host_libs():
     build_synth()
     build_oracle()
@@ -103,4 +133,5 @@ if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "host": build_host_libs, "ref": build_ref,
-         "emu": build_emu, "oracle": build_oracle, "synth": build_synth, "dropin": build_dropin_driver}[w]()
+         "emu": build_emu, "oracle": build_oracle, "synth": build_synth, "dropin": build_dropin_driver,
+         "sanitize": build_sanitized}[w]()

@@ -13,7 +13,8 @@ _LIB = None
 def lib():
     global _LIB
     if _LIB is None:
-        L = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblzr_emu.so"))
+        path = os.environ.get("LZR_EMU_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblzr_emu.so")
+        L = ctypes.CDLL(path)
         L.emu_fit_batch.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DRIVER = os.path.join(ROOT, "tests", "cpp", "storage_main")
+DRIVER = os.environ.get("LZR_STORAGE_MAIN") or os.path.join(ROOT, "tests", "cpp", "storage_main")
 GOLDEN = np.load(os.path.join(ROOT, "tests", "golden", "ref_storage.npz"), allow_pickle=False)
 
 import sys  # noqa: E402
