@@ -355,7 +355,7 @@ def net_init_leg(train_rois, eval_rois, epochs, steps, warmup, log, oracle_rois=
     return out
 
 
-def mixed_leg(steps, warmup, log, frames_per_block=1000):
+def mixed_leg(steps, warmup, log, frames_per_block=1000, parity=True):
     """Batched uneven workload: 2M ROIs, windows 5x5..15x15, 60 % symmetric / 40 %
     ellipsoidal; one call over all ROIs (default and autotuned launch configuration) and
     frame blocks replayed through one captured HIP graph."""
@@ -368,7 +368,8 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
     log(f"mixed: generated {w.nframes} frames, {cands.size} ROIs in {time.perf_counter() - t0:.1f} s")
     n = cands.size
     d_frames, d_cands = device.upload(frames), device.upload(cands)
-    del frames
+    if not parity:
+        del frames
     d_out = device.empty(abi.FIT_DTYPE, n)
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
@@ -393,11 +394,13 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
     ms_default = timed(one_call)
     fits = d_out.numpy()
     accepted = int(np.sum(fits["status"] == 0))
-    # autotune on the first 10 % of the frames, then time the whole batch again
-    n_tune = int(np.searchsorted(cands["frame"], w.nframes // 10))
+    # autotune on the batch itself (one launch per configuration; the hand-off count, and so the
+    # straggler group size, is a property of the batch), then time it again
+    n_tune = n
     best = ctx.autotune_device(d_frames.ptr, w.nx, w.ny, w.nframes, d_cands.ptr, n_tune, d_out.ptr, s)
     chosen = {"occupancy": ctx.get_option(abi.OPT_OCCUPANCY), "waves_per_cu": ctx.get_option(abi.OPT_WAVES_PER_CU),
-              "rounds": ctx.get_option(abi.OPT_ROUNDS)}
+              "rounds": ctx.get_option(abi.OPT_ROUNDS), "straggler_trials": ctx.get_option(abi.OPT_STRAGGLER_TRIALS),
+              "straggler_group": ctx.get_option(abi.OPT_STRAGGLER_GROUP)}
     ms_tuned = timed(one_call)
     # frame blocks through captured graphs: candidates of block b are staged (frames stay
     # resident), a graph replays, results are copied out.  Blocks rotate over `pipes`
@@ -467,6 +470,29 @@ def mixed_leg(steps, warmup, log, frames_per_block=1000):
                          "positions_differing_over_1e-3_px": far},
         "trials_mean": float(fits["trials"].mean()),
     }
+    if parity:
+        # every one of the 2M one-call fits against the FP64 oracle, per model class
+        from oracle import pyoracle as po
+        from tests.parity import compare
+
+        t0 = time.perf_counter()
+        ref = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, w.initial_psf_width, nthreads=nthreads)
+        out["parity_vs_fp64_oracle"] = {"oracle_seconds": time.perf_counter() - t0}
+        for m, name in ((abi.MODEL_SYMMETRIC, "symmetric"), (abi.MODEL_ELLIPSOIDAL, "ellipsoidal")):
+            sel = fits["model"] == m
+            st = compare(ref[sel], fits[sel], m)
+            keep = {k: st[k] for k in ("n", "status_agreement", "both", "xy_frac_within_1e-3", "xy_max_px",
+                                       "conv_n", "conv_xy_frac_within_1e-3") if k in st}
+            for nm in ("A", "r", "sx", "sy", "rho", "b"):
+                for pre in ("", "conv_"):
+                    k = f"{pre}{nm}_frac_within_1e-4"
+                    if k in st:
+                        keep[k] = st[k]
+            out["parity_vs_fp64_oracle"][name] = keep
+        del frames
+        log(f"mixed parity vs FP64 oracle: " + ", ".join(
+            f"{k} status {v['status_agreement']:.6f} x/y within 1e-3 {v['xy_frac_within_1e-3']:.6f}"
+            for k, v in out["parity_vs_fp64_oracle"].items() if isinstance(v, dict)))
     identical = bool(np.array_equal(gfits.view(np.uint8), fits.view(np.uint8)))
     out["graph_blocks"]["records_identical_to_one_call"] = identical
     log(f"mixed: one call {ms_default:.2f} ms, autotuned {ms_tuned:.2f} ms ({chosen}), graph blocks {ms_graph:.2f} ms "
@@ -762,7 +788,7 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = mixed_leg(max(1, min(args.steps, 5)), 1, lambda m: print(m, file=sys.stderr, flush=True),
-                          frames_per_block=args.block_frames)
+                          frames_per_block=args.block_frames, parity=not args.no_cpu)
         torch.cuda.empty_cache()
     dense = None
     if not args.no_dense:

@@ -51,8 +51,11 @@ def test_gpu_net_forward_matches_oracle(rois):
     scale = np.abs(ref).max()
     _say(f"forward: max |gpu - oracle| = {err.max():.3g}, output scale {scale:.3g}, "
          f"99.9th pct {np.quantile(err, 0.999):.3g}")
-    assert err.max() <= 2e-2 * scale
-    assert np.quantile(err, 0.99) <= 2e-3 * scale
+    # measured: max 1.71e-4 (1.1e-3 of the 0.158 output scale), 99.9th percentile 4.8e-5; the
+    # residue is BF16 activations rounding the other way where FP32 and FP64 sums straddle a
+    # rounding boundary.  The bars are about 2x the measured values.
+    assert err.max() <= 2e-3 * scale
+    assert np.quantile(err, 0.999) <= 1e-4
 
 
 def test_gpu_net_gradients_match_oracle(rois):
@@ -166,6 +169,18 @@ def _net_runs(params, frames, cands, refit_rejected=False):
     return out
 
 
+def _check_start_residuals(work, examined, chi0):
+    """The device's start-point residuals (left in the work buffer's amplitude) equal the
+    oracle's pixel-by-pixel |f| for every examined fit; NaN for the others."""
+    got = work["amplitude"]
+    assert np.all(np.isnan(got[~examined]))
+    ok = examined & ~np.isnan(chi0)
+    assert np.all(np.isnan(got[examined & np.isnan(chi0)]))
+    rel = np.abs(got[ok] - chi0[ok]) / chi0[ok]
+    assert ok.sum() == 0 or rel.max() <= 1e-10, rel.max()
+    return int(ok.sum()), float(rel.max()) if ok.any() else 0.0
+
+
 def _check_fallback(r, chi0, refit_rejected):
     """The refit set is exactly what the criterion names, refitted records are the plain
     records bit for bit, and the others are the net-start records."""
@@ -181,6 +196,8 @@ def _check_fallback(r, chi0, refit_rejected):
     # the device evaluates the start residual by moments, the oracle pixel by pixel
     tie = np.abs(first["chi"] - 3.0 * chi0) <= 1e-9 * np.abs(first["chi"])
     assert np.all((refit == want) | tie), np.flatnonzero((refit != want) & ~tie)[:10]
+    examined = used & (first["status"] == abi.ACCEPTED) & ~(refit_rejected & failed)
+    _check_start_residuals(r["work"], examined, chi0)
     return failed, high
 
 
@@ -250,3 +267,40 @@ def test_gpu_corrupted_network_falls_back_to_plain_records(trained, refit_reject
         st_agree, xy_ok, nboth = _xy_vs_oracle(r["final"], ref)
         _say(f"vs FP64 oracle: status {st_agree:.5f}, xy within 1e-3 {xy_ok:.5f} of {nboth}")
         assert st_agree >= 0.95 and xy_ok >= 0.98
+
+
+@pytest.mark.parametrize("h", [4, 8, 12])
+def test_gpu_fallback_start_residual_all_window_classes(h):
+    """The fallback test's start-point residual on the device (window rows staged through
+    LDS, three unrolled window classes) against the oracle's |f| pixel by pixel, for fit
+    windows 9x9, 17x17 and 25x25 on 256x256 frames (an untrained network supplies the starts)."""
+    _say(f"case: fallback residual, fit window {2 * h + 1}x{2 * h + 1}")
+    wl = synth.PLAIN_GPU_256.scaled(24)
+    frames, _, cands = synth.generate(wl)
+    cands["half_width"] = 0
+    n = len(cands)
+    d_frames, d_cands = device.upload(frames), device.upload(cands)
+    d_out, d_work = device.empty(abi.FIT_DTYPE, n), device.empty(abi.START_DTYPE, n)
+    d_first, d_starts = device.empty(abi.FIT_DTYPE, n), device.empty(abi.START_DTYPE, n)
+    s = device.stream_handle()
+    with native.Net(wl.initial_psf_width, seed=5) as net, \
+            native.Context(abi.MODEL_SYMMETRIC, wl.initial_psf_width, default_half_width=h) as ctx:
+        net.predict_device(d_frames.ptr, wl.nx, wl.ny, len(frames), d_cands.ptr, n, d_starts.ptr, s)
+        ctx.fit_device_start(d_frames.ptr, wl.nx, wl.ny, len(frames), d_cands.ptr, d_starts.ptr, n, d_first.ptr, s)
+        ctx.fit_device_netinit(net, d_frames.ptr, wl.nx, wl.ny, len(frames), d_cands.ptr, n, d_out.ptr, d_work.ptr,
+                               3.0, stream=s)
+    first, starts, work = d_first.numpy(), d_starts.numpy(), d_work.numpy()
+    nwin = 2 * h + 1
+    chi0 = np.full(n, np.nan)
+    for i, c in enumerate(cands):
+        xoff, yoff = np.floor(c["x"] - h), np.floor(c["y"] - h)
+        if xoff < 0 or yoff < 0 or xoff + nwin > wl.nx or yoff + nwin > wl.ny:
+            continue
+        data = frames[c["frame"], int(yoff):int(yoff) + nwin, int(xoff):int(xoff) + nwin]
+        _, f = po.model(abi.MODEL_SYMMETRIC, [c["intensity"], wl.initial_psf_width, c["x"], c["y"], c["background"]],
+                        data, nwin, xoff, yoff)
+        chi0[i] = np.sqrt(np.sum(f * f))
+    examined = (starts["width"] > 0.0) & (first["status"] == abi.ACCEPTED)
+    m, rel = _check_start_residuals(work, examined, chi0)
+    _say(f"window {nwin}x{nwin}: {m} start residuals compared, max rel err {rel:.3g}")
+    assert m >= 50
