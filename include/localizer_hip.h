@@ -340,7 +340,8 @@ LZR_EXPORT int lzr_net_train_step_device(LzrNet *net, const uint16_t *d_frames, 
  * A refitted candidate's record is the plain fit's record bit for bit.  d_work:
  * ncandidates LzrStart of scratch; afterwards its width is 0 for refitted candidates and
  * -1 for the others, and its amplitude holds |f| at the reference start for the accepted
- * net-start fits the residual test examined (NaN for the others).  d_nrefit (may be null):
+ * net-start fits whose test read their pixels (NaN for the others: a fit whose model lies
+ * within |f| (1 - 1/residual_factor) of the start point's model passes without them).  d_nrefit (may be null):
  * one device unsigned, set to the number of refitted candidates.  No reference counterpart (the reference has no network start);
  * the records it returns are those of lzr_fit_batch_device for the refitted candidates.
  * Returns 0 or -1. */

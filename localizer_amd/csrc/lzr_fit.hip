@@ -1426,12 +1426,61 @@ __device__ double mark_sse(uint16_t *buf, MarkMeta &meta, const uint16_t *src, i
     return ((A * A) * (sxx * syy) + 2.0 * (A * b) * (sx * sy) - 2.0 * A * sed) + ((N * b) * b - 2.0 * b * sd + sdd);
 }
 
+/* An upper bound on |m_start - m_fit| over a fit's window (m: the model surface, start: the
+ * reference start point, fit: the record's parameters), from separable sums -- no pixel
+ * reads.  By the triangle inequality |f_start| >= |f_fit| - |m_start - m_fit|, so a fit with
+ * |f_fit| <= factor (|f_fit| - bound) cannot exceed factor |f_start|.  Symmetric and
+ * fixed-width records (round Gaussians); +inf for the ellipsoidal ones.  The bound adds
+ * 1e-12 of the sums' magnitude for their cancellation. */
+__device__ double start_distance_bound(const LzrFit &f, const LzrCandidate &c, int xoff, int yoff, int n, double w0) {
+    LZR_NO_CONTRACT
+    double A1, r1, x1, y1, b1;
+    if (f.model == LZR_MODEL_SYMMETRIC) {
+        A1 = f.params[0];
+        r1 = f.params[1];
+        x1 = f.params[2];
+        y1 = f.params[3];
+        b1 = f.params[4];
+    } else if (f.model == LZR_MODEL_FIXED_WIDTH) {
+        A1 = f.params[0];
+        r1 = w0;
+        x1 = f.params[1];
+        y1 = f.params[2];
+        b1 = f.params[3];
+    } else {
+        return __builtin_inf();
+    }
+    const double k0 = 0.5 / (w0 * w0), k1 = 0.5 / (r1 * r1);
+    double s00[2] = {0.0, 0.0}, s11[2] = {0.0, 0.0}, s01[2] = {0.0, 0.0}, t0[2] = {0.0, 0.0}, t1[2] = {0.0, 0.0};
+    for (int a = 0; a < 2; ++a) {
+        const double o = a == 0 ? (double)xoff : (double)yoff;
+        const double p0 = a == 0 ? c.x : c.y, p1 = a == 0 ? x1 : y1;
+        for (int i = 0; i < n; ++i) {
+            const double u0 = o + (double)i - p0, u1 = o + (double)i - p1;
+            const double e0 = dexp(-(u0 * u0) * k0), e1 = dexp(-(u1 * u1) * k1);
+            s00[a] = dfma(e0, e0, s00[a]);
+            s11[a] = dfma(e1, e1, s11[a]);
+            s01[a] = dfma(e0, e1, s01[a]);
+            t0[a] += e0;
+            t1[a] += e1;
+        }
+    }
+    const double A0 = c.intensity, db = c.background - b1, N = (double)(n * n);
+    const double S00 = s00[0] * s00[1], S11 = s11[0] * s11[1], S01 = s01[0] * s01[1];
+    const double T0 = t0[0] * t0[1], T1 = t1[0] * t1[1];
+    const double d2 = ((A0 * A0) * S00 + (A1 * A1) * S11 - 2.0 * (A0 * A1) * S01) + ((N * db) * db + 2.0 * db * (A0 * T0 - A1 * T1));
+    const double mag = (A0 * A0) * S00 + (A1 * A1) * S11 + 2.0 * dabs(A0 * A1) * S01 + (N * db) * db +
+                       2.0 * dabs(db) * (dabs(A0) * T0 + dabs(A1) * T1);
+    return dsqrt(dmax(d2, 0.0) + 1e-12 * mag);
+}
+
 /* Marks the candidates whose net-start fit is refitted from the reference start: width 0
  * (refit) when the net start was used (width > 0) and the fit was rejected or did not
  * converge (with refit_rejected), or ended with a residual |f| above factor times the residual
  * at the reference's own start point (the normalized residuals chi/sqrt(dof) share the dof);
- * width -1 (keep) otherwise.  amplitude receives that start-point residual for the fits the
- * residual test examined, NaN for the others.  *nrefit (zeroed before) counts the refits.
+ * width -1 (keep) otherwise.  A fit that start_distance_bound already clears needs no pixel
+ * reads; the others' start-point residual is summed over the window (mark_sse) and left in
+ * amplitude (NaN for every other candidate).  *nrefit (zeroed before) counts the refits.
  * One wave per workgroup (the window staging of mark_sse). */
 __global__ void __launch_bounds__(64) mark_fallback_kernel(const uint16_t *frames, int nx, int ny, int nframes,
                                                            const LzrCandidate *cands, const LzrFit *fits,
@@ -1441,7 +1490,7 @@ __global__ void __launch_bounds__(64) mark_fallback_kernel(const uint16_t *frame
     __shared__ uint16_t buf[kMarkLdsHalf];
     __shared__ MarkMeta meta;
     const int i = blockIdx.x * 64 + threadIdx.x;
-    bool refit = false, examine = false;
+    bool refit = false, examine = false, cleared = false;
     double chi = 0.0;
     if (i < n && starts[i].width > 0.0) {
         const LzrFit &f = fits[i];
@@ -1458,12 +1507,17 @@ __global__ void __launch_bounds__(64) mark_fallback_kernel(const uint16_t *frame
         LaneState<5> s;
         LzrFit unused;
         if (lane_start<5>(s, cfg, c, 0, nx, ny, nframes, LZR_MODEL_SYMMETRIC, unused)) {
-            nwin = s.n;
-            src = frames + ((size_t)s.frame * ny + s.yoff) * nx + s.xoff;
-            x0 = c.x - s.xoff;
-            y0 = c.y - s.yoff;
-            A = c.intensity;
-            b = c.background;
+            const double dm = start_distance_bound(fits[i], c, s.xoff, s.yoff, s.n, cfg.initial_psf_width);
+            /* |f_start| >= chi - dm > chi (1 + 1e-6) / factor: no refit, whatever the pixels */
+            cleared = factor * (chi - dm * cfg.inv_sigma) > chi * (1.0 + 1e-6);
+            if (!cleared) {
+                nwin = s.n;
+                src = frames + ((size_t)s.frame * ny + s.yoff) * nx + s.xoff;
+                x0 = c.x - s.xoff;
+                y0 = c.y - s.yoff;
+                A = c.intensity;
+                b = c.background;
+            }
         }
     }
     const int nmax = __builtin_amdgcn_readfirstlane(wave_max(nwin));
@@ -1480,10 +1534,10 @@ __global__ void __launch_bounds__(64) mark_fallback_kernel(const uint16_t *frame
             sse = mark_sse<31>(buf, meta, src, nwin, nmax, nx, x0, y0, k, A, b);
         if (nwin > 0) chi0 = dsqrt(dmax(sse, 0.0)) * cfg.inv_sigma;
     }
-    if (examine) refit = chi > factor * chi0; /* NaN (window outside the frame): kept */
+    if (examine && !cleared) refit = chi > factor * chi0; /* NaN (window outside the frame): kept */
     if (i < n) {
         starts[i].width = refit ? 0.0 : -1.0;
-        starts[i].amplitude = examine ? chi0 : __builtin_nan("");
+        starts[i].amplitude = examine && !cleared ? chi0 : __builtin_nan("");
     }
     const unsigned long long m = __ballot(refit);
     if (m && threadIdx.x == __ffsll((long long)m) - 1) atomicAdd(nrefit, (unsigned)__popcll(m));

@@ -162,28 +162,26 @@ struct TrainStore {
 
 /* ---- prep: patch gather, median / scale, normalized BF16 input ------------------- */
 
-/* One wave per candidate.  The 121 pixels (+ 7 keys above any pixel) are sorted by a
- * bitonic network across the wave, two keys per lane (index l and l + 64): the
- * median is sorted key 60, the maximum sorted key 120. */
-__device__ inline void bitonic128(uint32_t &a, uint32_t &b, int lane) {
+/* One wave per candidate, two pixels per lane (index l and l + 64; the second only for
+ * l + 64 < 121).  The median (sorted key 60) is selected bit by bit from the top with wave
+ * ballots: the largest t < 2^16 with #{keys < t} <= 60 is the key of rank 60, exactly --
+ * about a dozen uniform steps of two compares and two ballots instead of the 28
+ * exchange stages of a 128-key sorting network. */
+__device__ inline uint32_t wave_max16(uint32_t v) {
 #pragma unroll
-    for (int k = 2; k <= 128; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j == 64) { /* partner of index l is l + 64: same lane; k == 128, ascending */
-                const uint32_t lo = min(a, b), hi = max(a, b);
-                a = lo;
-                b = hi;
-            } else {
-                const uint32_t pa = (uint32_t)__shfl_xor((int)a, j, 64);
-                const uint32_t pb = (uint32_t)__shfl_xor((int)b, j, 64);
-                const bool low = (lane & j) == 0;
-                const bool asc_a = (lane & k) == 0, asc_b = ((lane + 64) & k) == 0;
-                a = (low == asc_a) ? min(a, pa) : max(a, pa);
-                b = (low == asc_b) ? min(b, pb) : max(b, pb);
-            }
-        }
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+__device__ inline uint32_t wave_rank16(uint32_t a, uint32_t b, bool has_b, int k, uint32_t top) {
+    uint32_t lo = 0;
+    const int msb = top ? 31 - __builtin_clz(top) : 0; /* keys <= top: higher bits are 0 */
+    for (int bit = msb; bit >= 0; --bit) {             /* wave-uniform */
+        const uint32_t t = lo | (1u << bit);
+        const int cnt = __popcll(__ballot(a < t)) + __popcll(__ballot(has_b && b < t));
+        if (cnt <= k) lo = t;
     }
+    return lo;
 }
 
 __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int nx, int ny, int nframes,
@@ -224,12 +222,12 @@ __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int n
     const uint16_t *src = frames + ((size_t)f * ny + yoff) * nx + xoff;
     const int i0 = lane, i1 = lane + 64;
     const uint32_t u0 = src[(size_t)(i0 / kN) * nx + (i0 % kN)];
-    const uint32_t u1 = i1 < kPix ? (uint32_t)src[(size_t)(i1 / kN) * nx + (i1 % kN)] : 0x10000u;
-    uint32_t a = u0, b = u1;
-    bitonic128(a, b, lane);
-    const float m = (float)__shfl((int)a, kPix / 2, 64);        /* sorted key 60  */
-    const float mx = (float)__shfl((int)b, kPix - 1 - 64, 64);  /* sorted key 120 */
-    const float v0 = (float)u0, v1 = (float)(u1 & 0xffffu);
+    const bool has1 = i1 < kPix;
+    const uint32_t u1 = has1 ? (uint32_t)src[(size_t)(i1 / kN) * nx + (i1 % kN)] : 0u;
+    const uint32_t top = __builtin_amdgcn_readfirstlane(wave_max16(max(u0, u1)));
+    const float m = (float)__builtin_amdgcn_readfirstlane(wave_rank16(u0, u1, has1, kPix / 2, top)); /* key 60 */
+    const float mx = (float)top;                                                                    /* key 120 */
+    const float v0 = (float)u0, v1 = (float)u1;
     float scale = mx - m;
     if (!(scale >= 1.f)) scale = 1.f;
     row[i0] = f2bf((v0 - m) / scale);

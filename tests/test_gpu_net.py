@@ -169,16 +169,20 @@ def _net_runs(params, frames, cands, refit_rejected=False):
     return out
 
 
-def _check_start_residuals(work, examined, chi0):
+def _check_start_residuals(work, examined, chi0, chi):
     """The device's start-point residuals (left in the work buffer's amplitude) equal the
-    oracle's pixel-by-pixel |f| for every examined fit; NaN for the others."""
+    oracle's pixel-by-pixel |f| wherever the device summed them; the examined fits it cleared
+    without pixels (NaN) do satisfy |f| <= 3 |f_start|; NaN for every other candidate."""
     got = work["amplitude"]
     assert np.all(np.isnan(got[~examined]))
-    ok = examined & ~np.isnan(chi0)
-    assert np.all(np.isnan(got[examined & np.isnan(chi0)]))
+    summed = examined & ~np.isnan(got)
+    cleared = examined & np.isnan(got) & ~np.isnan(chi0)
+    assert np.all(chi[cleared] <= 3.0 * chi0[cleared])
+    assert np.all(np.isnan(chi0[summed]) == np.isnan(got[summed]))
+    ok = summed & ~np.isnan(chi0)
     rel = np.abs(got[ok] - chi0[ok]) / chi0[ok]
     assert ok.sum() == 0 or rel.max() <= 1e-10, rel.max()
-    return int(ok.sum()), float(rel.max()) if ok.any() else 0.0
+    return int(ok.sum()), float(rel.max()) if ok.any() else 0.0, int(cleared.sum())
 
 
 def _check_fallback(r, chi0, refit_rejected):
@@ -197,7 +201,8 @@ def _check_fallback(r, chi0, refit_rejected):
     tie = np.abs(first["chi"] - 3.0 * chi0) <= 1e-9 * np.abs(first["chi"])
     assert np.all((refit == want) | tie), np.flatnonzero((refit != want) & ~tie)[:10]
     examined = used & (first["status"] == abi.ACCEPTED) & ~(refit_rejected & failed)
-    _check_start_residuals(r["work"], examined, chi0)
+    _, _, ncleared = _check_start_residuals(r["work"], examined, chi0, first["chi"])
+    _say(f"fallback test: {int(examined.sum())} fits examined, {ncleared} cleared without pixel reads")
     return failed, high
 
 
@@ -301,6 +306,7 @@ def test_gpu_fallback_start_residual_all_window_classes(h):
                         data, nwin, xoff, yoff)
         chi0[i] = np.sqrt(np.sum(f * f))
     examined = (starts["width"] > 0.0) & (first["status"] == abi.ACCEPTED)
-    m, rel = _check_start_residuals(work, examined, chi0)
-    _say(f"window {nwin}x{nwin}: {m} start residuals compared, max rel err {rel:.3g}")
+    m, rel, ncleared = _check_start_residuals(work, examined, chi0, first["chi"])
+    _say(f"window {nwin}x{nwin}: {m} start residuals compared, max rel err {rel:.3g}; {ncleared} cleared "
+         f"without pixel reads")
     assert m >= 50
