@@ -679,7 +679,7 @@ def main():
     ap.add_argument("--dense-pipelines", type=int, default=DENSE_PIPELINES, help="frame blocks in flight")
     ap.add_argument("--net-train-rois", type=int, default=4 << 20)
     ap.add_argument("--net-eval-rois", type=int, default=4000000)
-    ap.add_argument("--net-epochs", type=float, default=2.0)
+    ap.add_argument("--net-epochs", type=float, default=8.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()

@@ -162,35 +162,25 @@ struct TrainStore {
 
 /* ---- prep: patch gather, median / scale, normalized BF16 input ------------------- */
 
-/* One wave per candidate, two pixels per lane (index l and l + 64; the second only for
- * l + 64 < 121).  The median (sorted key 60) is selected bit by bit from the top with wave
- * ballots: the largest t < 2^16 with #{keys < t} <= 60 is the key of rank 60, exactly --
- * about a dozen uniform steps of two compares and two ballots instead of the 28
- * exchange stages of a 128-key sorting network. */
-__device__ inline uint32_t wave_max16(uint32_t v) {
+/* Two candidates per wave, one per half-wave of 32 lanes, four pixels per lane (index
+ * l + 32 q, q < 4, for l + 32 q < 121).  The median (sorted key 60) is selected bit by bit
+ * from the top with wave ballots, both halves at once: the largest t < 2^16 with
+ * #{keys < t} <= 60 is the key of rank 60, exactly -- about a dozen uniform steps of four
+ * compares and four ballots instead of the 28 exchange stages of a 128-key sorting network. */
+__device__ inline uint32_t half_max16(uint32_t v) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    for (int off = 16; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
     return v;
-}
-
-__device__ inline uint32_t wave_rank16(uint32_t a, uint32_t b, bool has_b, int k, uint32_t top) {
-    uint32_t lo = 0;
-    const int msb = top ? 31 - __builtin_clz(top) : 0; /* keys <= top: higher bits are 0 */
-    for (int bit = msb; bit >= 0; --bit) {             /* wave-uniform */
-        const uint32_t t = lo | (1u << bit);
-        const int cnt = __popcll(__ballot(a < t)) + __popcll(__ballot(has_b && b < t));
-        if (cnt <= k) lo = t;
-    }
-    return lo;
 }
 
 __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int nx, int ny, int nframes,
                                                    const LzrCandidate *cands, int ncands, int nrows,
                                                    uint16_t *xin, PatchStat *stats) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + wave;
-    if (r >= nrows) return; /* wave-uniform */
-    bool valid = r < ncands;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+    const int r = (blockIdx.x * 4 + wave) * 2 + half;
+    const bool live = r < nrows;
+    if (__ballot(live) == 0) return; /* wave-uniform */
+    bool valid = live && r < ncands;
     int xoff = 0, yoff = 0, f = 0;
     if (valid) {
         const LzrCandidate c = cands[r];
@@ -204,42 +194,62 @@ __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int n
             valid = xoff + kN - 1 <= nx - 1 && yoff + kN - 1 <= ny - 1;
         }
     }
-    uint16_t *row = xin + (size_t)r * kXRow;
+    uint32_t u[4];
+    bool has[4];
+    const uint16_t *src = frames + ((size_t)f * ny + yoff) * nx + xoff;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = hl + 32 * q;
+        has[q] = valid && i < kPix;
+        u[q] = has[q] ? (uint32_t)src[(size_t)(i / kN) * nx + (i % kN)] : 0u;
+    }
+    const uint32_t top = half_max16(max(max(u[0], u[1]), max(u[2], u[3])));
+    /* rank 60 in each half (uniform loop over the larger top bit of the two halves) */
+    const uint32_t top2 = __builtin_amdgcn_readfirstlane(half_max16(top) | (uint32_t)__shfl_xor((int)top, 32, 64));
+    const int msb = top2 ? 31 - __builtin_clz(top2) : 0;
+    uint32_t lo = 0;
+    for (int bit = msb; bit >= 0; --bit) {
+        const uint32_t t = lo | (1u << bit);
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const unsigned long long m = __ballot(has[q] && u[q] < t);
+            cnt += half ? __popcll(m >> 32) : __popcll(m & 0xffffffffull);
+        }
+        if (cnt <= kPix / 2) lo = t;
+    }
+    uint16_t *row = xin + (size_t)(live ? r : 0) * kXRow;
+    if (!live) return;
     if (!valid) {
-        row[lane] = 0;
-        row[lane + 64] = 0;
-        if (lane == 0) {
-            PatchStat s;
-            s.median = 0.f;
-            s.scale = 1.f;
-            s.xoff = xoff;
-            s.yoff = yoff;
-            s.valid = 0;
-            stats[r] = s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) row[hl + 32 * q] = 0;
+        if (hl == 0) {
+            PatchStat st;
+            st.median = 0.f;
+            st.scale = 1.f;
+            st.xoff = xoff;
+            st.yoff = yoff;
+            st.valid = 0;
+            stats[r] = st;
         }
         return;
     }
-    const uint16_t *src = frames + ((size_t)f * ny + yoff) * nx + xoff;
-    const int i0 = lane, i1 = lane + 64;
-    const uint32_t u0 = src[(size_t)(i0 / kN) * nx + (i0 % kN)];
-    const bool has1 = i1 < kPix;
-    const uint32_t u1 = has1 ? (uint32_t)src[(size_t)(i1 / kN) * nx + (i1 % kN)] : 0u;
-    const uint32_t top = __builtin_amdgcn_readfirstlane(wave_max16(max(u0, u1)));
-    const float m = (float)__builtin_amdgcn_readfirstlane(wave_rank16(u0, u1, has1, kPix / 2, top)); /* key 60 */
-    const float mx = (float)top;                                                                    /* key 120 */
-    const float v0 = (float)u0, v1 = (float)u1;
+    const float m = (float)lo, mx = (float)top; /* keys 60 and 120 */
     float scale = mx - m;
     if (!(scale >= 1.f)) scale = 1.f;
-    row[i0] = f2bf((v0 - m) / scale);
-    row[i1] = i1 < kPix ? f2bf((v1 - m) / scale) : (uint16_t)0;
-    if (lane == 0) {
-        PatchStat s;
-        s.median = m;
-        s.scale = scale;
-        s.xoff = xoff;
-        s.yoff = yoff;
-        s.valid = 1;
-        stats[r] = s;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = hl + 32 * q;
+        row[i] = i < kPix ? f2bf(((float)u[q] - m) / scale) : (uint16_t)0;
+    }
+    if (hl == 0) {
+        PatchStat st;
+        st.median = m;
+        st.scale = scale;
+        st.xoff = xoff;
+        st.yoff = yoff;
+        st.valid = 1;
+        stats[r] = st;
     }
 }
 
@@ -1014,7 +1024,7 @@ template <bool kTrain>
 static void run_forward(LzrNet *net, const uint16_t *frames, int nx, int ny, int nframes, const LzrCandidate *cands,
                         int ncands, int nrows, hipStream_t s) {
     ensure_rows(net, (size_t)nrows);
-    hipLaunchKernelGGL(prep_kernel, dim3((nrows + 3) / 4), dim3(256), 0, s, frames, nx, ny, nframes, cands, ncands,
+    hipLaunchKernelGGL(prep_kernel, dim3((nrows + 7) / 8), dim3(256), 0, s, frames, nx, ny, nframes, cands, ncands,
                        nrows, net->d_xin, net->d_stats);
     TrainStore T{};
     if (kTrain) {
