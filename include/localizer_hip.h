@@ -162,6 +162,17 @@ LZR_EXPORT int lzr_fit_batch_device(LzrContext *ctx, const uint16_t *d_frames, i
                                     int nframes, const LzrCandidate *d_candidates, int ncandidates,
                                     LzrFit *d_results, void *stream);
 
+/* lzr_fit_batch / lzr_fit_batch_device on float frames: pixel values that are not camera
+ * counts (the reference fits any FP64 Image, source/Localization.cpp:126-130; FP64 values
+ * are stored as FP32 here, every sum is FP64).  Integer frames give the same records as the
+ * uint16 entry points.  The pixel tile is twice as large in LDS: windows up to 23x23
+ * (half-width 11).  Same return values. */
+LZR_EXPORT int lzr_fit_batch_f32(LzrContext *ctx, const float *frames, int nx, int ny, int nframes,
+                                 const LzrCandidate *candidates, int ncandidates, LzrFit *results);
+LZR_EXPORT int lzr_fit_batch_device_f32(LzrContext *ctx, const float *d_frames, int nx, int ny, int nframes,
+                                        const LzrCandidate *d_candidates, int ncandidates, LzrFit *d_results,
+                                        void *stream);
+
 /* As lzr_fit_batch_device, but allocating nothing: no device allocation, no allocating
  * library call.  No counterpart in the reference; it is for callers that cannot allocate
  * (code under stream capture, tight serving loops, harnesses that forbid hipMalloc).  Its

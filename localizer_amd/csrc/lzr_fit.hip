@@ -47,7 +47,7 @@
 namespace lzr {
 
 struct KernelArgs {
-    const uint16_t *frames;
+    const void *frames;    /* uint16_t or float pixels (the kernels' PixT) */
     int nx, ny, nframes;
     const LzrCandidate *cands;
     const LzrStart *starts; /* per-candidate LM start, or null     */
@@ -66,6 +66,17 @@ struct KernelArgs {
     unsigned *strag_ctr;
     unsigned strag_entries; /* capacity of strag[] in entries; a class's region is its share */
     int strag_lk;           /* log2 of the lanes per handed-off fit, -1: by the hand-off count */
+};
+
+/* What a pixel accessor returns for a frame element type: camera counts as unsigned, other
+ * (non-integer) frames as float; the passes convert either to FP64. */
+template <typename PixT>
+struct PixRaw {
+    using type = unsigned;
+};
+template <>
+struct PixRaw<float> {
+    using type = float;
 };
 
 /* Queue class of a model: 0 symmetric (5 parameters), 1 ellipsoidal and the
@@ -246,14 +257,15 @@ struct RoundArgs {
 };
 
 /* W = minimum waves per SIMD the register allocation must allow (occupancy target). */
-template <int P, typename Acc, int W>
+template <int P, typename Acc, int W, typename PixT = uint16_t>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     extern __shared__ double lds_words[];
     /* LDS: per-column FP64 factors [nmax][64], then the pixel tile [nmax*nmax][64] */
     const int nmax = 2 * args.cfg.max_half_width + 1;
     double *xcol = lds_words;
-    uint16_t *tile = reinterpret_cast<uint16_t *>(lds_words + nmax * 64);
+    PixT *tile = reinterpret_cast<PixT *>(lds_words + nmax * 64);
+    const PixT *frames = static_cast<const PixT *>(args.frames);
     const int lane = threadIdx.x;
     const unsigned long long below = (1ull << lane) - 1ull;
     const unsigned count = r.resume ? *r.in_count : args.count[cls];
@@ -330,7 +342,7 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             constexpr int kGatherLanes = decltype(gl)::value, R = decltype(rr)::value;
             while (gmask) {
                 int Lb[kGatherLanes], nb[kGatherLanes], inv[kGatherLanes];
-                const uint16_t *src[kGatherLanes];
+                const PixT *src[kGatherLanes];
                 int most = 0;
 #pragma unroll
                 for (int b = 0; b < kGatherLanes; ++b) {
@@ -345,18 +357,18 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
                     nb[b] = nL;
                     /* j = k / n as (k * inv) >> 16: exact for k < n^2 when n <= 31 (half-width <= 15) */
                     inv[b] = nL > 0 ? 65536 / nL + 1 : 0;
-                    src[b] = args.frames + ((size_t)fL * args.ny + yL) * args.nx + xL;
+                    src[b] = frames + ((size_t)fL * args.ny + yL) * args.nx + xL;
                     most = max(most, nL * nL);
                 }
                 for (int k0 = 0; k0 < most; k0 += R * 64) {
-                    uint16_t v[kGatherLanes][R];
+                    PixT v[kGatherLanes][R];
 #pragma unroll
                     for (int b = 0; b < kGatherLanes; ++b)
 #pragma unroll
                         for (int r = 0; r < R; ++r) {
                             const int k = k0 + lane + 64 * r;
                             const int j = (k * inv[b]) >> 16, i = k - j * nb[b];
-                            v[b][r] = k < nb[b] * nb[b] ? src[b][(size_t)j * args.nx + i] : (uint16_t)0;
+                            v[b][r] = k < nb[b] * nb[b] ? src[b][(size_t)j * args.nx + i] : (PixT)0;
                         }
 #pragma unroll
                     for (int b = 0; b < kGatherLanes; ++b)
@@ -402,10 +414,10 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         const int nw = __builtin_amdgcn_readfirstlane(wave_max(n)); /* wave-uniform loop bound */
         const bool mixed = __any(active && n != nw);
         struct {
-            const uint16_t *t;
+            const PixT *t;
             int n;
             __device__ float operator()(int i, int j) const { return (float)t[(i + j * n) * 64]; }
-            __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
+            __device__ typename PixRaw<PixT>::type raw(int i, int j) const { return t[(i + j * n) * 64]; }
         } pix{tile + lane, active ? n : nw};
         struct {
             double *c;
@@ -492,7 +504,7 @@ constexpr int kEllXchg = 15; /* w, P[5], R[3], W[3], rd, rdd, bad */
 __host__ __device__ constexpr int xchg_slots(int P) { return P == 7 ? kEllXchg : kSepXchg; }
 
 /* The straggler loop for groups of 2^LK lanes. */
-template <int P, typename Acc, int LK>
+template <int P, typename Acc, int LK, typename PixT>
 __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
                                                  const Parked<P> *park, double *lds_words) {
     /* LDS: per-lane FP64 column factors [nmax][64], the row exchange [slots][64], then the
@@ -501,7 +513,7 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
     const int nmax = 2 * args.cfg.max_half_width + 1;
     double *xcol = lds_words;
     double *xchg = lds_words + nmax * 64;
-    uint16_t *tile = reinterpret_cast<uint16_t *>(xchg + xchg_slots(P) * 64);
+    PixT *tile = reinterpret_cast<PixT *>(xchg + xchg_slots(P) * 64);
     constexpr int lk = LK, kGroup = 1 << LK, kMask = kGroup - 1;
     const int lane = threadIdx.x, grp = lane >> lk, r = lane & kMask, gbase = grp * kGroup;
 
@@ -546,14 +558,15 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
             const int n = s.n, nr = (n - r + kMask) >> lk; /* rows of this lane */
             const int cnt = nr * n;
             const int inv = 65536 / n + 1; /* m = t / n as (t * inv) >> 16, exact for t < n^2, n <= 31 */
-            const uint16_t *src = args.frames + ((size_t)s.frame * args.ny + s.yoff + r) * args.nx + s.xoff;
+            const PixT *src = static_cast<const PixT *>(args.frames) +
+                              ((size_t)s.frame * args.ny + s.yoff + r) * args.nx + s.xoff;
             for (int t0 = 0; t0 < cnt; t0 += 8) {
-                uint16_t v[8];
+                PixT v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const int t = t0 + u;
                     const int m = (t * inv) >> 16, i = t - m * n;
-                    v[u] = t < cnt ? src[(size_t)(m * kGroup) * args.nx + i] : (uint16_t)0;
+                    v[u] = t < cnt ? src[(size_t)(m * kGroup) * args.nx + i] : (PixT)0;
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
@@ -571,9 +584,9 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
         const int nw = __builtin_amdgcn_readfirstlane(wave_max(n));
         const int rounds = (nw + kMask) >> lk; /* wave-uniform: the exchange's barriers */
         struct {
-            const uint16_t *t;
+            const PixT *t;
             int n;
-            __device__ unsigned raw(int i, int j) const { return t[(i + j * n) * 64]; }
+            __device__ typename PixRaw<PixT>::type raw(int i, int j) const { return t[(i + j * n) * 64]; }
         } pix{tile + lane, n};
         if constexpr (P == 7) {
             EllIn q;
@@ -609,8 +622,8 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                     EllRow ro{{0.0, 0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
                     if (jo < n) {
                         struct {
-                            const uint16_t *t;
-                            __device__ unsigned raw(int i, int) const { return t[i * 64]; }
+                            const PixT *t;
+                            __device__ typename PixRaw<PixT>::type raw(int i, int) const { return t[i * 64]; }
                         } prow{tile + (m * n) * 64 + lane};
                         ro = ell_row_sums<false>(q, n, n, rw.fac, true, go, co, prow, xb, jo);
                     }
@@ -675,8 +688,8 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                     SepRow ro{0.0, 0.0, 0.0, 0.0, 0.0};
                     if (jo < n) {
                         struct {
-                            const uint16_t *t;
-                            __device__ unsigned raw(int i, int) const { return t[i * 64]; }
+                            const PixT *t;
+                            __device__ typename PixRaw<PixT>::type raw(int i, int) const { return t[i * 64]; }
                         } prow{tile + (m * n) * 64 + lane};
                         ro = sep_row<false>(q, n, n, true, prow, xb, jo);
                     }
@@ -716,7 +729,7 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
  * apply exits at once).  Both derive the group size from the same lane count. */
 enum StragglerRole : int { kAllGroups = 0, kMultiLane = 1, kOneLane = 2 };
 
-template <int P, typename Acc, int W, int Role>
+template <int P, typename Acc, int W, int Role, typename PixT = uint16_t>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     extern __shared__ double lds_words[];
@@ -735,24 +748,24 @@ lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     /* LZR_OPT_STRAGGLER_GROUP 1 runs one lane per fit in the one-wave-per-SIMD build */
     const bool one_wave_single = args.strag_lk == 0;
     if constexpr (Role == kOneLane) {
-        if (lk == 0 && !one_wave_single) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
+        if (lk == 0 && !one_wave_single) straggler_groups<P, Acc, 0, PixT>(args, count, head, park, lds_words);
         return;
     }
     if constexpr (Role == kMultiLane) {
         if (lk == 0 && one_wave_single) {
-            straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
+            straggler_groups<P, Acc, 0, PixT>(args, count, head, park, lds_words);
             return;
         }
     }
     switch (lk) {
-    case 4: straggler_groups<P, Acc, 4>(args, count, head, park, lds_words); break;
-    case 3: straggler_groups<P, Acc, 3>(args, count, head, park, lds_words); break;
-    case 2: straggler_groups<P, Acc, 2>(args, count, head, park, lds_words); break;
+    case 4: straggler_groups<P, Acc, 4, PixT>(args, count, head, park, lds_words); break;
+    case 3: straggler_groups<P, Acc, 3, PixT>(args, count, head, park, lds_words); break;
+    case 2: straggler_groups<P, Acc, 2, PixT>(args, count, head, park, lds_words); break;
     case 1:
-        if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 1>(args, count, head, park, lds_words);
+        if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 1, PixT>(args, count, head, park, lds_words);
         break;
     default:
-        if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 0>(args, count, head, park, lds_words);
+        if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 0, PixT>(args, count, head, park, lds_words);
         break;
     }
 }
@@ -808,6 +821,8 @@ struct LzrContext {
     std::mutex mu;
     /* staging of the host-buffer entry point */
     uint16_t *d_frames = nullptr;
+    float *d_framesf = nullptr; /* staging of float frames (lzr_fit_batch_f32) */
+    size_t framesf_cap = 0;
     size_t frames_cap = 0;
     LzrCandidate *d_cands = nullptr;
     size_t cands_cap = 0;
@@ -859,13 +874,13 @@ static void ensure(T *&ptr, size_t &cap, size_t need) {
 }
 
 struct Workspace;
-template <int P, typename Acc, int W>
+template <int P, typename Acc, int W, typename PixT>
 static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream);
 
-template <int P, typename Acc, int W>
+template <int P, typename Acc, int W, typename PixT = uint16_t>
 static int grid_for(LzrContext *ctx, size_t lds) {
     int per_cu = 0;
-    LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, W>, 64, lds));
+    LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, W, PixT>, 64, lds));
     if (per_cu < 1) throw std::string("fitting kernel cannot be resident with this window size");
     if (ctx->waves_per_cu > 0) per_cu = std::min(per_cu, ctx->waves_per_cu);
     return per_cu * ctx->num_cus;
@@ -887,9 +902,9 @@ static void reserve_park(LzrContext *ctx, Workspace &ws, int cls, size_t lds) {
     }
 }
 
-template <int P, typename Acc, int W>
+template <int P, typename Acc, int W, typename PixT>
 static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
-    const int grid = grid_for<P, Acc, W>(ctx, lds);
+    const int grid = grid_for<P, Acc, W, PixT>(ctx, lds);
     reserve_park<P, Acc, W>(ctx, ws, cls, lds);
     unsigned *ctr = ws.counters + kCounterWords + 2 * kClasses + 4 * cls; /* buffer b: count ctr[2b], head ctr[2b+1] */
     for (int round = 0; round < ctx->rounds; ++round) {
@@ -903,7 +918,7 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
         r.out = ws.park[cls][ob];
         r.out_count = ctr + 2 * ob;
         r.threshold = round == ctx->rounds - 1 ? 0 : ctx->park_threshold;
-        hipLaunchKernelGGL((lm_persistent<P, Acc, W>), dim3(grid), dim3(64), lds, stream, a, cls, r);
+        hipLaunchKernelGGL((lm_persistent<P, Acc, W, PixT>), dim3(grid), dim3(64), lds, stream, a, cls, r);
     }
     if (a.strag_trials > 0) {
         const size_t slds = lds + (size_t)xchg_slots(P) * 64 * sizeof(double); /* + the row exchange */
@@ -914,17 +929,18 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
          * interleaved waves take ~1.4x longer per trial) */
         const size_t glds = std::max(slds, ctx->lds_per_cu / 4);
         int per_cu = 0;
-        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1, kRole>, 64, glds));
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1, kRole, PixT>, 64, glds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
         per_cu = std::min(per_cu, 4);
         const unsigned lanes = (unsigned)(per_cu * ctx->num_cus) * 64u; /* sets the group size in both launches */
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole>), dim3(per_cu * ctx->num_cus), dim3(64), glds, stream, a,
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole, PixT>), dim3(per_cu * ctx->num_cus), dim3(64), glds, stream, a,
                            cls, lanes);
         if constexpr (P != 7) {
             int per_cu2 = 0;
-            LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, lm_straggler<P, Acc, 2, kOneLane>, 64, slds));
+            LZR_CHECK(
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, lm_straggler<P, Acc, 2, kOneLane, PixT>, 64, slds));
             if (per_cu2 < 1) throw std::string("straggler kernel cannot be resident with this window size");
-            hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane>), dim3(per_cu2 * ctx->num_cus), dim3(64), slds,
+            hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane, PixT>), dim3(per_cu2 * ctx->num_cus), dim3(64), slds,
                                stream, a, cls, lanes);
         }
     }
@@ -932,9 +948,11 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
 
 typedef double AccT;           /* normal-equation accumulation type      */
 
-static size_t launch_lds(const LzrContext *ctx) {
+/* LDS of the first kernel: the column factors and the pixel tile (pix_bytes per pixel: 2 for
+ * camera counts, 4 for float frames). */
+static size_t launch_lds(const LzrContext *ctx, size_t pix_bytes = sizeof(uint16_t)) {
     const int nmax = 2 * ctx->cfg.max_half_width + 1;
-    return (size_t)nmax * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * sizeof(uint16_t);
+    return (size_t)nmax * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * pix_bytes;
 }
 
 /* Waves per SIMD the 5-parameter kernel is built for.  Automatic: 2 when the LDS tiles
@@ -942,9 +960,9 @@ static size_t launch_lds(const LzrContext *ctx) {
  * two-wave build spills a few refill-path values but hides the pass's latencies (500k
  * 9x9 fits: 6.57 -> 5.91 ms); with larger windows LDS allows about one wave per SIMD and
  * the one-wave build has no spills. */
-static int occupancy5(const LzrContext *ctx) {
+static int occupancy5(const LzrContext *ctx, size_t pix_bytes = sizeof(uint16_t)) {
     if (ctx->occupancy5 > 0) return ctx->occupancy5;
-    return launch_lds(ctx) * 6 <= ctx->lds_per_cu ? 2 : 1;
+    return launch_lds(ctx, pix_bytes) * 6 <= ctx->lds_per_cu ? 2 : 1;
 }
 
 /* Makes ws hold a launch of up to ncands candidates (allocating, for an owned workspace), so
@@ -986,9 +1004,30 @@ static void release(Workspace &ws) {
 /* Enqueues one fit of ncands candidates (d_count, when not null, holds the live count
  * on the device, at most ncands) on workspace ws.  Allocates nothing once reserve() covered
  * ncands; order (eager calls) waits for the previous call on ws when the stream differs. */
-static void launch(LzrContext *ctx, Workspace &ws, const uint16_t *frames, int nx, int ny, int nframes,
+template <typename PixT>
+static void launch_classes(LzrContext *ctx, Workspace &ws, const KernelArgs &a, hipStream_t stream) {
+    const size_t lds = launch_lds(ctx, sizeof(PixT));
+    /* The classes share nothing but the sorted queue: classes 1 and 2 run on the
+     * context's side streams beside class 0, so one class's straggler tail overlaps
+     * another class's main kernel (fork/join by events; also inside a graph capture). */
+    LZR_CHECK(hipEventRecord(ctx->fork, stream));
+    for (int c = 0; c < kClasses - 1; ++c) LZR_CHECK(hipStreamWaitEvent(ctx->side[c], ctx->fork, 0));
+    run_class<7, AccT, 1, PixT>(ctx, ws, a, 1, lds, ctx->side[0]);
+    run_class<4, AccT, 2, PixT>(ctx, ws, a, 2, lds, ctx->side[1]);
+    if (occupancy5(ctx, sizeof(PixT)) == 1)
+        run_class<5, AccT, 1, PixT>(ctx, ws, a, 0, lds, stream);
+    else
+        run_class<5, AccT, 2, PixT>(ctx, ws, a, 0, lds, stream);
+    for (int c = 0; c < kClasses - 1; ++c) {
+        LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
+        LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
+    }
+}
+
+/* frames: uint16 camera counts, or float pixels when float_frames (non-integer data). */
+static void launch(LzrContext *ctx, Workspace &ws, const void *frames, int nx, int ny, int nframes,
                    const LzrCandidate *cands, const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream,
-                   const int *d_count = nullptr, LzrStreamOrder *order = nullptr) {
+                   const int *d_count = nullptr, LzrStreamOrder *order = nullptr, bool float_frames = false) {
     reserve(ctx, ws, ncands);
     /* the workspace's queue, counters and hand-off buffers are reused by every call on it */
     if (order) LZR_CHECK(order->begin(stream));
@@ -1017,22 +1056,10 @@ static void launch(LzrContext *ctx, Workspace &ws, const uint16_t *frames, int n
     a.strag_ctr = strag_ctr;
     a.strag_entries = (unsigned)std::min(ws.strag_entries, (size_t)0xffffffffu);
     a.strag_lk = ctx->strag_group > 0 ? __builtin_ctz((unsigned)ctx->strag_group) : -1;
-    const size_t lds = launch_lds(ctx);
-    /* The classes share nothing but the sorted queue: classes 1 and 2 run on the
-     * context's side streams beside class 0, so one class's straggler tail overlaps
-     * another class's main kernel (fork/join by events; also inside a graph capture). */
-    LZR_CHECK(hipEventRecord(ctx->fork, stream));
-    for (int c = 0; c < kClasses - 1; ++c) LZR_CHECK(hipStreamWaitEvent(ctx->side[c], ctx->fork, 0));
-    run_class<7, AccT, 1>(ctx, ws, a, 1, lds, ctx->side[0]);
-    run_class<4, AccT, 2>(ctx, ws, a, 2, lds, ctx->side[1]);
-    if (occupancy5(ctx) == 1)
-        run_class<5, AccT, 1>(ctx, ws, a, 0, lds, stream);
+    if (float_frames)
+        launch_classes<float>(ctx, ws, a, stream);
     else
-        run_class<5, AccT, 2>(ctx, ws, a, 0, lds, stream);
-    for (int c = 0; c < kClasses - 1; ++c) {
-        LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
-        LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
-    }
+        launch_classes<uint16_t>(ctx, ws, a, stream);
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
                        ctx->cfg);
     LZR_CHECK(hipGetLastError());
@@ -1150,6 +1177,7 @@ LZR_EXPORT int lzr_clone(LzrContext *src, LzrContext **out) {
 LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     if (!ctx) return -1;
     (void)hipFree(ctx->d_frames);
+    (void)hipFree(ctx->d_framesf);
     (void)hipFree(ctx->d_cands);
     (void)hipFree(ctx->d_out);
     (void)hipFree(ctx->netinit_words);
@@ -1316,6 +1344,53 @@ LZR_EXPORT int lzr_fit_batch(LzrContext *ctx, const uint16_t *frames, int nx, in
     int accepted = 0;
     for (int i = 0; i < ncands; ++i) accepted += results[i].status == LZR_ACCEPTED;
     return accepted;
+}
+
+LZR_EXPORT int lzr_fit_batch_f32(LzrContext *ctx, const float *frames, int nx, int ny, int nframes,
+                                 const LzrCandidate *cands, int ncands, LzrFit *results) {
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+    if (ncands == 0) return 0;
+    if (!frames || !cands || !results) return fail(ctx, "null host pointer");
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        if (!ctx->stream) LZR_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        const size_t npx = (size_t)nx * ny * nframes;
+        ensure(ctx->d_framesf, ctx->framesf_cap, npx);
+        ensure(ctx->d_cands, ctx->cands_cap, (size_t)ncands);
+        ensure(ctx->d_out, ctx->out_cap, (size_t)ncands);
+        LZR_CHECK(hipMemcpyAsync(ctx->d_framesf, frames, npx * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        LZR_CHECK(hipMemcpyAsync(ctx->d_cands, cands, (size_t)ncands * sizeof(LzrCandidate), hipMemcpyHostToDevice,
+                                 ctx->stream));
+        launch(ctx, ctx->ws, ctx->d_framesf, nx, ny, nframes, ctx->d_cands, nullptr, ncands, ctx->d_out, ctx->stream,
+               nullptr, &ctx->order, true);
+        LZR_CHECK(hipMemcpyAsync(results, ctx->d_out, (size_t)ncands * sizeof(LzrFit), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        LZR_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (const std::string &e) {
+        return fail(ctx, e);
+    }
+    int accepted = 0;
+    for (int i = 0; i < ncands; ++i) accepted += results[i].status == LZR_ACCEPTED;
+    return accepted;
+}
+
+LZR_EXPORT int lzr_fit_batch_device_f32(LzrContext *ctx, const float *d_frames, int nx, int ny, int nframes,
+                                        const LzrCandidate *d_cands, int ncands, LzrFit *d_out, void *stream) {
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+    if (ncands == 0) return 0;
+    if (!d_frames || !d_cands || !d_out) return fail(ctx, "null device pointer");
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        launch(ctx, ctx->ws, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_out, (hipStream_t)stream, nullptr,
+               &ctx->order, true);
+    } catch (const std::string &e) {
+        return fail(ctx, e);
+    }
+    return 0;
 }
 
 LZR_EXPORT int lzr_fit_batch_device_start(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,

@@ -52,19 +52,26 @@ inline std::string last_error(const LzrContext *ctx) {
     return std::string(buf);
 }
 
-/* Copies an FP64 frame (column-major, rows = x) into a uint16 buffer; the GPU path
- * fits camera counts, so every pixel must be an integer in [0, 65535]. */
-inline void append_frame(const Image &img, std::vector<uint16_t> &dst) {
+/* True when every pixel of an FP64 frame is an integer camera count in [0, 65535]: such
+ * frames travel as uint16, others as float (lzr_fit_batch_f32). */
+inline bool counts_only(const Image &img) {
+    const long nx = img.rows(), ny = img.cols();
+    for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) {
+            const double v = img(x, y);
+            if (!(v >= 0.0 && v <= 65535.0) || v != std::floor(v)) return false;
+        }
+    return true;
+}
+
+/* Appends an FP64 frame (column-major, rows = x, i.e. x fast) to a frame stack of T. */
+template <class T>
+inline void append_frame(const Image &img, std::vector<T> &dst) {
     const long nx = img.rows(), ny = img.cols();
     const size_t base = dst.size();
     dst.resize(base + (size_t)nx * ny);
     for (long y = 0; y < ny; ++y)
-        for (long x = 0; x < nx; ++x) {
-            const double v = img(x, y);
-            if (!(v >= 0.0 && v <= 65535.0) || v != std::floor(v))
-                throw std::runtime_error("LocalizationHIP: frame pixels must be integer counts in [0, 65535]");
-            dst[base + (size_t)y * nx + x] = (uint16_t)v;
-        }
+        for (long x = 0; x < nx; ++x) dst[base + (size_t)y * nx + x] = (T)img(x, y);
 }
 
 class FitterCore {
@@ -89,8 +96,12 @@ public:
         if (images.size() != lists.size()) throw std::runtime_error("LocalizationHIP: frames and lists differ in count");
         std::vector<std::shared_ptr<LocalizedPositionsContainer> > out;
         std::vector<uint16_t> stack;
+        std::vector<float> stackf;
         std::vector<LzrCandidate> cands;
         long nx = -1, ny = -1;
+        bool counts = true;
+        for (size_t f = 0; f < images.size(); ++f)
+            if (!lists[f]->empty() && !counts_only(*images[f])) counts = false;
         for (size_t f = 0; f < images.size(); ++f) {
             out.push_back(std::shared_ptr<LocalizedPositionsContainer>(new Container()));
             if (lists[f]->empty()) continue;
@@ -100,16 +111,21 @@ public:
             } else if (images[f]->rows() != nx || images[f]->cols() != ny) {
                 throw std::runtime_error("LocalizationHIP: frames in one batch must share their size");
             }
-            const int fi = (int)(stack.size() / ((size_t)nx * ny));
-            append_frame(*images[f], stack);
+            const int fi = (int)((counts ? stack.size() : stackf.size()) / ((size_t)nx * ny));
+            if (counts)
+                append_frame(*images[f], stack);
+            else
+                append_frame(*images[f], stackf);
             for (const Particle &p : *lists[f]) cands.push_back(candidate(p, fi));
         }
         if (cands.empty()) return out;
         std::vector<LzrFit> res(cands.size());
-        const int nframes = (int)(stack.size() / ((size_t)nx * ny));
-        if (lzr_fit_batch(ctx_, stack.data(), (int)nx, (int)ny, nframes, cands.data(), (int)cands.size(),
-                          res.data()) < 0)
-            throw std::runtime_error("LocalizationHIP: " + last_error(ctx_));
+        const int nframes = (int)((counts ? stack.size() : stackf.size()) / ((size_t)nx * ny));
+        const int rc = counts ? lzr_fit_batch(ctx_, stack.data(), (int)nx, (int)ny, nframes, cands.data(),
+                                              (int)cands.size(), res.data())
+                              : lzr_fit_batch_f32(ctx_, stackf.data(), (int)nx, (int)ny, nframes, cands.data(),
+                                                  (int)cands.size(), res.data());
+        if (rc < 0) throw std::runtime_error("LocalizationHIP: " + last_error(ctx_));
         size_t k = 0;
         for (size_t f = 0; f < images.size(); ++f) {
             auto it = lists[f]->begin();
@@ -129,8 +145,8 @@ public:
     }
 
     /* One frame (the reference's fit_positions): through the batcher, created for the
-     * first frame's size; frames of another size, or more particles than one batch holds,
-     * take the one-call path. */
+     * first frame's size; frames of another size, more particles than one batch holds, or
+     * windows holding non-integer pixel values take the one-call path (float frames). */
     template <class Container>
     std::shared_ptr<LocalizedPositionsContainer> fit_one(const ImagePtr image,
                                                          std::shared_ptr<std::list<Particle> > particles) {
@@ -148,9 +164,9 @@ public:
         std::vector<LzrFit> res(cands.size());
         if (lzr_batcher_fit_frame_f64(b, image->data(), (int)image->rows(), (int)image->cols(), cands.data(),
                                       (int)cands.size(), res.data()) < 0) {
-            char buf[512];
-            lzr_batcher_last_error(b, buf, (int)sizeof buf);
-            throw std::runtime_error(std::string("LocalizationHIP: ") + buf);
+            std::vector<ImagePtr> im(1, image);
+            std::vector<std::shared_ptr<std::list<Particle> > > li(1, particles);
+            return fit_many<Container>(im, li)[0];
         }
         size_t k = 0;
         for (auto it = particles->begin(); it != particles->end();) {

@@ -22,6 +22,8 @@ EXPORTED = (
     "lzr_set_option",
     "lzr_fit_batch",
     "lzr_fit_batch_device",
+    "lzr_fit_batch_f32",
+    "lzr_fit_batch_device_f32",
     "lzr_abi_version",
     "lzr_fit_batch_device_start",
     "lzr_fit_batch_device_netinit",
@@ -93,6 +95,10 @@ def lib():
         L.lzr_fit_batch.restype = ctypes.c_int
         L.lzr_fit_batch_device.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp]
         L.lzr_fit_batch_device.restype = ctypes.c_int
+        L.lzr_fit_batch_f32.argtypes = L.lzr_fit_batch.argtypes
+        L.lzr_fit_batch_f32.restype = ctypes.c_int
+        L.lzr_fit_batch_device_f32.argtypes = L.lzr_fit_batch_device.argtypes
+        L.lzr_fit_batch_device_f32.restype = ctypes.c_int
         L.lzr_abi_version.argtypes = []
         L.lzr_abi_version.restype = ctypes.c_int
         i, d, f = ctypes.c_int, ctypes.c_double, ctypes.c_float
@@ -181,18 +187,29 @@ class Context:
             raise LocalizerError(_error(self._h))
 
     def fit(self, frames, candidates):
-        """Host arrays in, FIT_DTYPE array out (lzr_fit_batch)."""
-        frames = np.ascontiguousarray(frames, np.uint16)
+        """Host arrays in, FIT_DTYPE array out: uint16 frames through lzr_fit_batch, floating-
+        point frames (any pixel values) through lzr_fit_batch_f32."""
+        flt = np.issubdtype(np.asarray(frames).dtype, np.floating)
+        frames = np.ascontiguousarray(frames, np.float32 if flt else np.uint16)
         if frames.ndim == 2:
             frames = frames[None]
         nf, ny, nx = frames.shape
         cands = np.ascontiguousarray(candidates, abi.CANDIDATE_DTYPE)
         out = np.zeros(cands.size, abi.FIT_DTYPE)
-        rc = lib().lzr_fit_batch(self._h, frames.ctypes.data, nx, ny, nf, cands.ctypes.data, cands.size,
-                                 out.ctypes.data)
+        fn = lib().lzr_fit_batch_f32 if flt else lib().lzr_fit_batch
+        rc = fn(self._h, frames.ctypes.data, nx, ny, nf, cands.ctypes.data, cands.size, out.ctypes.data)
         if rc < 0:
             raise LocalizerError(_error(self._h))
         return out
+
+    def fit_device_f32(self, d_frames, nx, ny, nframes, d_cands, ncands, d_out, stream=0):
+        """lzr_fit_batch_device_f32: float frames in device memory."""
+        rc = lib().lzr_fit_batch_device_f32(self._h, ctypes.c_void_p(d_frames), int(nx), int(ny), int(nframes),
+                                            ctypes.c_void_p(d_cands), int(ncands), ctypes.c_void_p(d_out),
+                                            ctypes.c_void_p(stream))
+        if rc < 0:
+            raise LocalizerError(_error(self._h))
+        return rc
 
     def fit_device_start(self, d_frames, nx, ny, nframes, d_cands, d_starts, ncands, d_out, stream=0):
         """lzr_fit_batch_device_start: per-candidate LM start points (d_starts may be 0)."""

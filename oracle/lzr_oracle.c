@@ -737,8 +737,10 @@ int orc_eigen2(double sx, double sy, double rho, double *sd1, double *sd2, doubl
 
 static double gsl_max(double a, double b) { return a > b ? a : b; } /* GSL_MAX_DBL */
 
-void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
-                 const LzrCandidate *cand, LzrFit *out) {
+/* One candidate's fit; the frame is uint16 counts (frame16) or FP64 values (frame64), the
+ * reference's Image (source/Localization.cpp:126-130 reads it as double). */
+static void fit_roi(const OrcSettings *s, const uint16_t *frame16, const double *frame64, int nx, int ny,
+                    const LzrCandidate *cand, LzrFit *out) {
     memset(out, 0, sizeof *out);
     const int model = cand->model >= 0 ? cand->model : s->model;
     out->model = (int16_t)model;
@@ -765,7 +767,10 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
     const int p = sym ? 5 : (fixed ? 4 : 7);
     double data[ORC_MAXN];
     for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) data[i + n * j] = (double)frame[(long)(yoff + j) * nx + xoff + i];
+        for (int i = 0; i < n; ++i) {
+            const long k = (long)(yoff + j) * nx + xoff + i;
+            data[i + n * j] = frame64 ? frame64[k] : (double)frame16[k];
+        }
 
     double t0[ORC_MAXP];
     if (fixed) {
@@ -910,6 +915,11 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
     free(lm);
 }
 
+void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny, const LzrCandidate *cand,
+                 LzrFit *out) {
+    fit_roi(s, frame, NULL, nx, ny, cand, out);
+}
+
 /* ------------------------------------------------------------------------------ */
 /* Frame-parallel driver (PALMAnalysisController, source/PALMAnalysis.cpp:79-280)   */
 /* ------------------------------------------------------------------------------ */
@@ -917,6 +927,7 @@ void orc_fit_roi(const OrcSettings *s, const uint16_t *frame, int nx, int ny,
 typedef struct Pool {
     const OrcSettings *s;
     const uint16_t *frames;
+    const double *frames64; /* FP64 frames instead of frames (NULL: uint16) */
     int nx, ny, nframes;
     const LzrCandidate *cands;
     const int *order;        /* candidate indices grouped by frame */
@@ -939,7 +950,9 @@ static void *pool_worker(void *arg) {
                 P->out[ci].status = LZR_REJECT_INVALID;
                 continue;
             }
-            orc_fit_roi(P->s, P->frames + (long)c->frame * P->nx * P->ny, P->nx, P->ny, c, &P->out[ci]);
+            const long base = (long)c->frame * P->nx * P->ny;
+            fit_roi(P->s, P->frames64 ? NULL : P->frames + base, P->frames64 ? P->frames64 + base : NULL, P->nx,
+                    P->ny, c, &P->out[ci]);
         }
     }
     return NULL;
@@ -952,8 +965,8 @@ static int cmp_frame_idx(const void *a, const void *b, void *ctx) {
     return ia < ib ? -1 : (ia > ib);
 }
 
-int orc_fit_batch(const OrcSettings *s, const uint16_t *frames, int nx, int ny, int nframes,
-                  const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads) {
+static int fit_batch(const OrcSettings *s, const uint16_t *frames, const double *frames64, int nx, int ny,
+                     int nframes, const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads) {
     if (ncands <= 0) return 0;
     int *order = (int *)malloc(sizeof(int) * ncands);
     int *starts = (int *)malloc(sizeof(int) * (ncands + 1));
@@ -963,7 +976,7 @@ int orc_fit_batch(const OrcSettings *s, const uint16_t *frames, int nx, int ny, 
     for (int i = 0; i < ncands; ++i)
         if (i == 0 || cands[order[i]].frame != cands[order[i - 1]].frame) starts[ng++] = i;
     starts[ng] = ncands;
-    Pool P = {s, frames, nx, ny, nframes, cands, order, starts, ng, 0, out};
+    Pool P = {s, frames, frames64, nx, ny, nframes, cands, order, starts, ng, 0, out};
     if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
     if (nthreads < 1) nthreads = 1;
     if (nthreads > ng) nthreads = ng;
@@ -977,4 +990,14 @@ int orc_fit_batch(const OrcSettings *s, const uint16_t *frames, int nx, int ny, 
     int acc = 0;
     for (int i = 0; i < ncands; ++i) acc += out[i].status == LZR_ACCEPTED;
     return acc;
+}
+
+int orc_fit_batch(const OrcSettings *s, const uint16_t *frames, int nx, int ny, int nframes,
+                  const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads) {
+    return fit_batch(s, frames, NULL, nx, ny, nframes, cands, ncands, out, nthreads);
+}
+
+int orc_fit_batch_f64(const OrcSettings *s, const double *frames, int nx, int ny, int nframes,
+                      const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads) {
+    return fit_batch(s, NULL, frames, nx, ny, nframes, cands, ncands, out, nthreads);
 }

@@ -131,6 +131,9 @@ int orc_render(const double *pos, int n, int x_size, int y_size, double scale, i
 
 int orc_fit_batch(const OrcSettings *s, const uint16_t *frames, int nx, int ny, int nframes,
                   const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads);
+/* The same on FP64 frames (the reference's Image; any pixel values). */
+int orc_fit_batch_f64(const OrcSettings *s, const double *frames, int nx, int ny, int nframes,
+                      const LzrCandidate *cands, int ncands, LzrFit *out, int nthreads);
 
 #ifdef __cplusplus
 }

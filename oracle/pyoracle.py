@@ -75,6 +75,8 @@ def lib():
             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
         ]
         L.orc_fit_batch.restype = ctypes.c_int
+        L.orc_fit_batch_f64.argtypes = L.orc_fit_batch.argtypes
+        L.orc_fit_batch_f64.restype = ctypes.c_int
         L.orc_glrt_window.argtypes = [ctypes.c_double]
         L.orc_glrt_window.restype = ctypes.c_int
         L.orc_glrt_mask.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -208,16 +210,19 @@ def eigen2(sx, sy, rho):
 
 def fit_batch(frames, candidates, model_code, initial_psf_width, sigma=1.0, default_half_width=0,
               nthreads=0, options=None):
-    """Fits every candidate with the frame-parallel pool; returns an array of FIT_DTYPE."""
-    frames = np.ascontiguousarray(frames, np.uint16)
+    """Fits every candidate with the frame-parallel pool; returns an array of FIT_DTYPE.
+    Floating-point frames are fitted as FP64 values (orc_fit_batch_f64), others as uint16."""
+    f64 = np.issubdtype(np.asarray(frames).dtype, np.floating)
+    frames = np.ascontiguousarray(frames, np.float64 if f64 else np.uint16)
     if frames.ndim == 2:
         frames = frames[None]
     nframes, ny, nx = frames.shape
     cands = np.ascontiguousarray(candidates, abi.CANDIDATE_DTYPE)
     out = np.zeros(cands.size, abi.FIT_DTYPE)
     s = settings(model_code, initial_psf_width, sigma, default_half_width, options)
-    lib().orc_fit_batch(ctypes.byref(s), frames.ctypes.data, nx, ny, nframes, cands.ctypes.data,
-                        cands.size, out.ctypes.data, int(nthreads))
+    fn = lib().orc_fit_batch_f64 if f64 else lib().orc_fit_batch
+    fn(ctypes.byref(s), frames.ctypes.data, nx, ny, nframes, cands.ctypes.data, cands.size, out.ctypes.data,
+       int(nthreads))
     return out
 
 

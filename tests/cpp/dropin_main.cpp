@@ -3,7 +3,8 @@
 //
 // usage: dropin_main <model> <initialPSFWidth> <sigma> <mode> <in.bin> <out.bin>
 //   mode: single (one fit_positions call per frame), threads (4 threads sharing one
-//         fitter instance), batch (fit_positions_batch over all frames)
+//         fitter instance), batch (fit_positions_batch over all frames); a "scaled_" prefix
+//         fits the frames with pixel values 1.375 v + 0.25 (non-integer: the float path)
 //   in.bin : int32 nx, ny, nframes; uint16 frames[nframes][ny][nx];
 //            per frame: int32 count, count x (x, y, intensity, background) doubles
 //   out.bin: per frame: int32 rows, cols, doubles matrix (column-major);
@@ -29,7 +30,9 @@ int main(int argc, char **argv) {
     }
     const int model = atoi(argv[1]);
     const double w = atof(argv[2]), sigma = atof(argv[3]);
-    const std::string mode = argv[4];
+    std::string mode = argv[4];
+    const bool scaled = mode.rfind("scaled_", 0) == 0;
+    if (scaled) mode = mode.substr(7);
     FILE *in = fopen(argv[5], "rb");
     if (!in) return 2;
     int32_t nx, ny, nf;
@@ -43,7 +46,10 @@ int main(int argc, char **argv) {
     for (int f = 0; f < nf; ++f) {
         ImagePtr im(new Image(nx, ny));
         for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) (*im)(x, y) = raw[((size_t)f * ny + y) * nx + x];
+            for (int x = 0; x < nx; ++x) {
+                const double v = raw[((size_t)f * ny + y) * nx + x];
+                (*im)(x, y) = scaled ? 1.375 * v + 0.25 : v;
+            }
         images.push_back(im);
         int32_t cnt;
         rd(in, &cnt, 4);

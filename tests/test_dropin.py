@@ -125,6 +125,41 @@ def test_dropin_matches_c_abi_and_oracle(tmp_path, model, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["scaled_single", "scaled_threads", "scaled_batch"])
+def test_dropin_non_integer_frames(tmp_path, mode):
+    """Frames whose pixels are not camera counts (1.375 v + 0.25, exact in FP32) are fitted,
+    not refused: the drop-in returns the records of the C ABI's float entry point, and those
+    meet the parity bar against the FP64 oracle on the same FP64 frames."""
+    from localizer_amd import native
+
+    print(f"case: drop-in non-integer frames, mode={mode}", flush=True)
+    w = synth.PLAIN_GPU_256.scaled(12)
+    frames, _, cands = synth.generate(w)
+    cands["half_width"] = 0
+    # the candidates a finder would report on the scaled frames
+    cands["intensity"] *= 1.375
+    cands["background"] = cands["background"] * 1.375 + 0.25
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    _write_input(fin, frames, cands)
+    w_psf = 1.35
+    subprocess.run([DRIVER, "0", str(w_psf), "1.0", mode, fin, fout], check=True, timeout=120)
+    got = _read_output(fout, frames.shape[0])
+    ff = frames.astype(np.float64) * 1.375 + 0.25
+    with native.Context(0, w_psf) as ctx:
+        capi = ctx.fit(ff.astype(np.float32), cands)
+    for fi in range(frames.shape[0]):
+        r = capi[cands["frame"] == fi]
+        acc = r["status"] == 0
+        m, _ = got[fi]
+        assert m.shape[0] == acc.sum()
+        np.testing.assert_array_equal(m[:, 3], r["params"][acc][:, 2])
+    ref = po.fit_batch(ff, cands, 0, w_psf)
+    st = compare(ref, capi, 0)
+    print(format_stats("drop-in non-integer frames", st))
+    assert_parity(st, 0, slack=1)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("method,params,weighing", [(0, (0.2, 0, 0), 1), (1, (4.0, 0.08, 0), 0), (2, (1.3, 100.0, 2.0), 1)])
 def test_render_dropin_matches_oracle(tmp_path, method, params, weighing):
     """PALMBitmapImageCalculatorHIP::CalculateImage on a 2DGauss container equals the

@@ -312,3 +312,31 @@ def test_gpu_mixed_100k_matches_oracle(mixed_100k):
         st = compare(ref[sel], got[sel], m)
         _say(format_stats(f"gpu vs oracle mixed 100k model {m}", st))
         assert_parity(st, m, spread=compare(ref[sel], other[sel], m))
+
+
+def test_gpu_float_frames(mixed_100k):
+    """The float entry points (lzr_fit_batch_f32): integer frames passed as float give the
+    uint16 records bit for bit (mixed models, windows 5x5..15x15, straggler hand-offs
+    included); non-integer frames (1.375 v + 0.25, exact in FP32) meet the bar against the
+    FP64 oracle fitting the same FP64 frames."""
+    frames, cands, _, _ = mixed_100k
+    frames, cands = frames[:400], cands[cands["frame"] < 400]
+    _say(f"case: float frames, {cands.size} mixed fits")
+    with native.Context(abi.MODEL_SYMMETRIC, 1.5) as ctx:
+        ctx.set_option(abi.OPT_MAX_HALF_WIDTH, 7)
+        u16 = ctx.fit(frames, cands)
+        f32 = ctx.fit(frames.astype(np.float32), cands)
+        assert u16.tobytes() == f32.tobytes()
+        # the same spots on an affine pixel scale, with the candidates a finder would report
+        ff = frames.astype(np.float64) * 1.375 + 0.25
+        cs = cands.copy()
+        cs["intensity"] *= 1.375
+        cs["background"] = cs["background"] * 1.375 + 0.25
+        got = ctx.fit(ff.astype(np.float32), cs)
+    ref = po.fit_batch(ff, cs, abi.MODEL_SYMMETRIC, 1.5)
+    other = oracle_sigma_ulp(po, ff, cs, abi.MODEL_SYMMETRIC, 1.5)
+    for m in (abi.MODEL_SYMMETRIC, abi.MODEL_ELLIPSOIDAL):
+        sel = got["model"] == m
+        st = compare(ref[sel], got[sel], m)
+        _say(format_stats(f"gpu float frames vs FP64 oracle, model {m}", st))
+        assert_parity(st, m, spread=compare(ref[sel], other[sel], m), slack=1)

@@ -109,3 +109,20 @@ def test_ellipsoidal_integral_is_volume_times_sqrt_one_minus_rho2(rho):
     assert r["integral"] == pytest.approx(vol * math.sqrt(1 - rho1 * rho1), rel=1e-8)
     # sd1 sd2 is the square root of the covariance determinant
     assert r["sd1"] * r["sd2"] == pytest.approx(sx1 * sy1 * math.sqrt(1 - rho1 * rho1), rel=1e-12)
+
+
+def test_fp64_frames_of_counts_fit_like_uint16_frames():
+    """orc_fit_batch_f64 reads the frame as the reference's FP64 Image: integer-valued frames
+    give the uint16 driver's records bit for bit; a constant offset added to every pixel moves
+    only the fitted background, by that offset (to the LM's tolerance)."""
+    from localizer_amd import synth
+
+    frames, _, cands = synth.generate(synth.PLAIN_CPU_512.scaled(4))
+    a = po.fit_batch(frames, cands, abi.MODEL_SYMMETRIC, 1.3)
+    b = po.fit_batch(frames.astype(np.float64), cands, abi.MODEL_SYMMETRIC, 1.3)
+    assert a.tobytes() == b.tobytes()
+    c = po.fit_batch(frames.astype(np.float64) + 0.375, cands, abi.MODEL_SYMMETRIC, 1.3)
+    both = (a["status"] == 0) & (c["status"] == 0)
+    assert both.sum() > 0.9 * (a["status"] == 0).sum()
+    assert np.allclose(c["params"][both, 4] - a["params"][both, 4], 0.375, atol=2e-3)
+    assert np.allclose(c["params"][both, 2:4], a["params"][both, 2:4], atol=1e-3)
