@@ -261,10 +261,12 @@ template <int P, typename Acc, int W, typename PixT = uint16_t>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     extern __shared__ double lds_words[];
-    /* LDS: per-column FP64 factors [nmax][64], then the pixel tile [nmax*nmax][64] */
+    /* LDS: per-lane FP64 column buffer [nmax + 2][64] (column factors, then the first two row
+     * factors), then the pixel tile [nmax*nmax][64] */
     const int nmax = 2 * args.cfg.max_half_width + 1;
+    const int ey_slot = xb_ey_slot(args.cfg);
     double *xcol = lds_words;
-    PixT *tile = reinterpret_cast<PixT *>(lds_words + nmax * 64);
+    PixT *tile = reinterpret_cast<PixT *>(lds_words + (nmax + kXbEySlots) * 64);
     const PixT *frames = static_cast<const PixT *>(args.frames);
     const int lane = threadIdx.x;
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -447,14 +449,14 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             else
                 make_sep_in(z, z, 1.0, 1.0, q);
             double v[kSepSums];
-            sep_pass(q, n, nw, mixed, pix, xb, v);
+            sep_pass(q, n, nw, mixed, pix, xb, v, ey_slot);
             sep_finish<P, Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, acc);
         }
 
         /* ---- LM logic ---- */
         bool hand_off = false;
         if (active) {
-            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, reinterpret_cast<LzrRawFit *>(args.out + s.roi)))
+            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, xb, model, reinterpret_cast<LzrRawFit *>(args.out + s.roi)))
                 s.phase = open ? kIdle : kFinished;
             else
                 hand_off = args.strag_trials > 0 && !stay && s.trials >= args.strag_trials;
@@ -499,20 +501,69 @@ constexpr int kMaxGroup = LZR_STRAGGLER_GROUP;
 static_assert(kMaxGroup == 8 || kMaxGroup == 16, "group sizes are 1..16 lanes");
 
 /* Row-exchange slots per lane in LDS: a row's factor and inner sums. */
-constexpr int kSepXchg = 6;  /* ey, r1, ru, ru2, rd, rdd */
+constexpr int kSepXchg = 5;  /* r1, ru, ru2, rd, rdd */
+constexpr int kSepRowsOne = 15; /* windows up to this side fold every row after one exchange */
+constexpr int kSepXchgSlots = kSepXchg * ((kSepRowsOne + 3) / 4); /* groups of >= 4 lanes */
 constexpr int kEllXchg = 15; /* w, P[5], R[3], W[3], rd, rdd, bad */
-__host__ __device__ constexpr int xchg_slots(int P) { return P == 7 ? kEllXchg : kSepXchg; }
+__host__ __device__ constexpr int xchg_slots(int P) { return P == 7 ? kEllXchg : kSepXchgSlots; }
+
+/* The rows of a group's separable pass (windows up to K x K, groups of 2^LK >= 4 lanes) with a
+ * single exchange: every lane steps all K row factors (rows at or beyond n are 0), lane r sums
+ * the pixels of rows r, r + k, ..., writes their inner sums to the exchange, and after one LDS
+ * wait every lane folds rows 0..K-1 in row order, exactly the one-lane pass's sums. */
+template <int K, int LK, typename PixT, typename XBuf>
+__device__ __forceinline__ void sep_group_rows(const SepIn &q, int n, int nw, bool guard, GaussRec &gy,
+                                               const PixT *tile, XBuf &xb, int ey_slot, double *xchg, int lane,
+                                               double (&v)[kSepSums]) {
+    constexpr int kGroup = 1 << LK, kMask = kGroup - 1, M = (K + kMask) >> LK;
+    static_assert(LK >= 2 && M * kSepXchg <= kSepXchgSlots, "exchange slots");
+    const int r = lane & kMask, gbase = lane & ~kMask;
+    double eyr[K];
+    sep_row_factors<K>(q, gy, 0, n, v, eyr);
+    xb.set(ey_slot, eyr[0]);
+    xb.set(ey_slot + 1, eyr[1]);
+    double *x = xchg + lane;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int jo = m * kGroup + r;
+        SepRow ro{0.0, 0.0, 0.0, 0.0, 0.0};
+        if (jo < n) {
+            struct {
+                const PixT *t;
+                __device__ typename PixRaw<PixT>::type raw(int i, int) const { return t[i * 64]; }
+            } prow{tile + (m * n) * 64 + lane};
+            ro = guard ? sep_row<true>(q, n, nw, true, prow, xb, jo) : sep_row<false>(q, n, nw, true, prow, xb, jo);
+        }
+        x[(m * kSepXchg + 0) * 64] = ro.r1;
+        x[(m * kSepXchg + 1) * 64] = ro.ru;
+        x[(m * kSepXchg + 2) * 64] = ro.ru2;
+        x[(m * kSepXchg + 3) * 64] = ro.rd;
+        x[(m * kSepXchg + 4) * 64] = ro.rdd;
+    }
+    __syncthreads();
+    double yv[K][kSepXchg];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const double *y = xchg + ((j >> LK) * kSepXchg) * 64 + gbase + (j & kMask);
+#pragma unroll
+        for (int a = 0; a < kSepXchg; ++a) yv[j][a] = y[a * 64];
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) row_sums(eyr[j], (double)j - q.y1, yv[j][0], yv[j][1], yv[j][2], yv[j][3], yv[j][4], v);
+    __syncthreads();
+}
 
 /* The straggler loop for groups of 2^LK lanes. */
 template <int P, typename Acc, int LK, typename PixT>
 __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
                                                  const Parked<P> *park, double *lds_words) {
-    /* LDS: per-lane FP64 column factors [nmax][64], the row exchange [slots][64], then the
-     * windows: lane r of a group holds rows r, r + k, ...: pixel (i, r + k m) at
-     * [(i + n m) * 64 + lane] (for k = 1 the first kernel's layout) */
+    /* LDS: per-lane FP64 column buffer [nmax + 2][64] (as in the first kernel), the row exchange
+     * [slots][64], then the windows: lane r of a group holds rows r, r + k, ...: pixel
+     * (i, r + k m) at [(i + n m) * 64 + lane] (for k = 1 the first kernel's layout) */
     const int nmax = 2 * args.cfg.max_half_width + 1;
+    const int ey_slot = xb_ey_slot(args.cfg);
     double *xcol = lds_words;
-    double *xchg = lds_words + nmax * 64;
+    double *xchg = lds_words + (nmax + kXbEySlots) * 64;
     PixT *tile = reinterpret_cast<PixT *>(xchg + xchg_slots(P) * 64);
     constexpr int lk = LK, kGroup = 1 << LK, kMask = kGroup - 1;
     const int lane = threadIdx.x, grp = lane >> lk, r = lane & kMask, gbase = grp * kGroup;
@@ -671,18 +722,26 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
             double v[kSepSums];
             if constexpr (kGroup == 1) {
                 /* one lane per fit: the first kernel's pass over the same layout */
-                sep_pass(q, n, nw, __any(active && n != nw), pix, xb, v);
+                sep_pass(q, n, nw, __any(active && n != nw), pix, xb, v, ey_slot);
             } else {
+                /* Loop bounds are the wave's largest window (nw): columns and rows at or beyond a
+                 * lane's own n carry zero factors and zero sums and leave every sum unchanged, so
+                 * the pass is the one-lane pass bit for bit and its loops need no per-lane branch. */
                 GaussRec gy;
-                sep_columns(q, n, n, xb, v, gy);
-                for (int m = 0; m < rounds; ++m) {
-                    double eo = 0.0;
-                    for (int rr = 0; rr < kGroup; ++rr) {
-                        const int j = m * kGroup + rr;
-                        if (j < n) {
-                            const double ey = sep_row_factor(q, gy, j, n, v);
-                            if (rr == r) eo = ey;
-                        }
+                sep_columns(q, n, nw, xb, v, gy);
+                const bool guard = __any(active && n != nw);
+                if (nw <= 9) {
+                    sep_group_rows<9, lk>(q, n, nw, guard, gy, tile, xb, ey_slot, xchg, lane, v);
+                } else if (nw <= kSepRowsOne) {
+                    sep_group_rows<kSepRowsOne, lk>(q, n, nw, guard, gy, tile, xb, ey_slot, xchg, lane, v);
+                } else for (int m = 0; m < rounds; ++m) {
+                    /* the round's row factors in row order: every lane steps the recurrence and
+                     * keeps them, so only the rows' inner sums are exchanged */
+                    double eyr[kGroup];
+                    sep_row_factors<kGroup>(q, gy, m * kGroup, n, v, eyr);
+                    if (m == 0) {
+                        xb.set(ey_slot, eyr[0]);
+                        xb.set(ey_slot + 1, eyr[1]);
                     }
                     const int jo = m * kGroup + r;
                     SepRow ro{0.0, 0.0, 0.0, 0.0, 0.0};
@@ -691,22 +750,28 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                             const PixT *t;
                             __device__ typename PixRaw<PixT>::type raw(int i, int) const { return t[i * 64]; }
                         } prow{tile + (m * n) * 64 + lane};
-                        ro = sep_row<false>(q, n, n, true, prow, xb, jo);
+                        ro = guard ? sep_row<true>(q, n, nw, true, prow, xb, jo) : sep_row<false>(q, n, nw, true, prow, xb, jo);
                     }
                     double *x = xchg + lane;
-                    x[0] = eo;
-                    x[64] = ro.r1;
-                    x[128] = ro.ru;
-                    x[192] = ro.ru2;
-                    x[256] = ro.rd;
-                    x[320] = ro.rdd;
+                    x[0] = ro.r1;
+                    x[64] = ro.ru;
+                    x[128] = ro.ru2;
+                    x[192] = ro.rd;
+                    x[256] = ro.rdd;
                     __syncthreads();
+                    /* every row of the round read at once (one LDS wait), then folded in row order;
+                     * rows at or beyond n have zero factors and sums */
+                    double yv[kGroup][kSepXchg];
+#pragma unroll
+                    for (int rr = 0; rr < kGroup; ++rr) {
+                        const double *y = xchg + gbase + rr;
+#pragma unroll
+                        for (int a = 0; a < kSepXchg; ++a) yv[rr][a] = y[a * 64];
+                    }
+#pragma unroll
                     for (int rr = 0; rr < kGroup; ++rr) {
                         const int j = m * kGroup + rr;
-                        if (j < n) {
-                            const double *y = xchg + gbase + rr;
-                            row_sums(y[0], (double)j - q.y1, y[64], y[128], y[192], y[256], y[320], v);
-                        }
+                        row_sums(eyr[rr], (double)j - q.y1, yv[rr][0], yv[rr][1], yv[rr][2], yv[rr][3], yv[rr][4], v);
                     }
                     __syncthreads();
                 }
@@ -716,7 +781,7 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
 
         if (active) {
             LzrRawFit *o = r == 0 ? reinterpret_cast<LzrRawFit *>(args.out + s.roi) : nullptr;
-            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, model, o)) s.phase = open ? kIdle : kFinished;
+            if (!lane_after_pass<P, Acc>(s, args.cfg, acc, xb, model, o)) s.phase = open ? kIdle : kFinished;
         }
         __syncthreads();
     }
@@ -952,7 +1017,7 @@ typedef double AccT;           /* normal-equation accumulation type      */
  * camera counts, 4 for float frames). */
 static size_t launch_lds(const LzrContext *ctx, size_t pix_bytes = sizeof(uint16_t)) {
     const int nmax = 2 * ctx->cfg.max_half_width + 1;
-    return (size_t)nmax * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * pix_bytes;
+    return (size_t)(nmax + kXbEySlots) * 64 * sizeof(double) + (size_t)nmax * nmax * 64 * pix_bytes;
 }
 
 /* Waves per SIMD the 5-parameter kernel is built for.  Automatic: 2 when the LDS tiles

@@ -535,6 +535,13 @@ struct GaussRec {
     }
 };
 
+/* A lane's column buffer holds the column factors Ex(0..nmax-1) of its last separable pass,
+ * then Ey(0) and Ey(1) at slots ey_slot, ey_slot + 1 (ey_slot = nmax, the launch's largest
+ * window side): the scale update of an accepted step reads the first rows of J from them
+ * (sep_diag_norms). */
+constexpr int kXbEySlots = 2;
+LZR_HD LZR_INLINE int xb_ey_slot(const LaneConfig &cfg) { return 2 * cfg.max_half_width + 1; }
+
 /* Pixels per step of the separable pass's row loop (odd windows of 9 and 15 split evenly). */
 constexpr int kSepChunk = 3;
 
@@ -593,13 +600,24 @@ LZR_HD LZR_INLINE void sep_columns(const SepIn &q, int n, int nloop, XBuf &xb, d
     GaussRec gx;
     gx.start(q.c1, q.x1, n, qexp);
     gy.start(q.c1, q.y1, n, qexp);
-    for (int i = 0; i < nloop; ++i) {
-        const double u = (double)i - q.x1;
-        double e = gx.next();
-        if (!gx.ok) e = dexp(-(q.c1 * u) * u);
-        e = i < n ? e : 0.0;
-        xb.set(i, e);
-        axis_moments(e, u, v + kMx0);
+    /* the lane's branch is taken once, outside the column loop: the common recurrence loop
+     * has no branch in its body (a lane that needs direct exponentials runs the second loop) */
+    if (gx.ok) {
+        for (int i = 0; i < nloop; ++i) {
+            const double u = (double)i - q.x1;
+            double e = gx.next();
+            e = i < n ? e : 0.0;
+            xb.set(i, e);
+            axis_moments(e, u, v + kMx0);
+        }
+    } else {
+        for (int i = 0; i < nloop; ++i) {
+            const double u = (double)i - q.x1;
+            double e = dexp(-(q.c1 * u) * u);
+            e = i < n ? e : 0.0;
+            xb.set(i, e);
+            axis_moments(e, u, v + kMx0);
+        }
     }
 }
 
@@ -613,6 +631,37 @@ LZR_HD LZR_INLINE double sep_row_factor(const SepIn &q, GaussRec &gy, int j, int
     ey = j < n ? ey : 0.0;
     axis_moments(ey, w, v + kMy0);
     return ey;
+}
+
+/* K consecutive row factors Ey(j0..j0+K-1) (0 at and beyond n), added to the row moments in
+ * row order: the same operations as K calls of sep_row_factor, with the lane's branch between
+ * the recurrence and direct exponentials taken once.  Rows at or beyond n add zeros, which
+ * leaves every moment unchanged, so K may run past the window. */
+template <int K>
+LZR_HD LZR_INLINE void sep_row_factors(const SepIn &q, GaussRec &gy, int j0, int n, double (&v)[kSepSums],
+                                       double (&ey)[K]) {
+    LZR_NO_CONTRACT
+    if (gy.ok) {
+#pragma unroll
+        for (int rr = 0; rr < K; ++rr) {
+            const int j = j0 + rr;
+            const double w = (double)j - q.y1;
+            double e = gy.next();
+            e = j < n ? e : 0.0;
+            axis_moments(e, w, v + kMy0);
+            ey[rr] = e;
+        }
+    } else {
+#pragma unroll
+        for (int rr = 0; rr < K; ++rr) {
+            const int j = j0 + rr;
+            const double w = (double)j - q.y1;
+            double e = dexp(-(q.c1 * w) * w);
+            e = j < n ? e : 0.0;
+            axis_moments(e, w, v + kMy0);
+            ey[rr] = e;
+        }
+    }
 }
 
 /* Inner sums of row j over columns 0..nloop-1 (Guard: pixels outside the lane's own window
@@ -634,12 +683,13 @@ LZR_HD LZR_INLINE SepRow sep_row(const SepIn &q, int n, int nloop, bool row_in, 
  * the same operations in the same order, so a fit's sums do not depend on who computes them. */
 template <typename Pix, typename XBuf>
 LZR_HD LZR_INLINE void sep_pass(const SepIn &q, int n, int nloop, bool guard, const Pix &pix, XBuf &xb,
-                                double (&v)[kSepSums]) {
+                                double (&v)[kSepSums], int ey_slot) {
     LZR_NO_CONTRACT
     GaussRec gy;
     sep_columns(q, n, nloop, xb, v, gy);
     for (int j = 0; j < nloop; ++j) {
         const double ey = sep_row_factor(q, gy, j, n, v);
+        if (j < kXbEySlots) xb.set(ey_slot + j, ey);
         const bool row_in = !guard || j < n;
         const SepRow r = guard ? sep_row<true>(q, n, nloop, row_in, pix, xb, j)
                                : sep_row<false>(q, n, nloop, row_in, pix, xb, j);
@@ -1108,6 +1158,45 @@ LZR_HD LZR_INLINE void partial_column_norms(const double (&t)[P], int n, double 
     for (int c = 0; c < P; ++c) cn[c] = dsqrt(cn[c] == 0.0 ? 1.0 : cn[c]);
 }
 
+/* The same column norms for the separable models (P = 5, 4) and rows = P, from the Gaussian
+ * factors the pass at t left in the lane's column buffer: J row k (pixel i = k mod n,
+ * j = k div n; n >= 3, so j <= 1) has e = Ex(i) Ey(j) and the pass's own column constants, so
+ * no exponential is evaluated here.  The factors are those of the pass's recurrence, a few
+ * ulps from a direct exponential. */
+template <int P, typename XBuf>
+LZR_HD LZR_INLINE void sep_diag_norms(const double (&t)[P], int n, double is, double w, const XBuf &xb, int ey_slot,
+                                      double (&cn)[P]) {
+    LZR_NO_CONTRACT
+    static_assert(P == 4 || P == 5, "separable models only");
+    constexpr int X = P == 5 ? 2 : 1; /* column of x0 */
+    double z[P];
+#pragma unroll
+    for (int c = 0; c < P; ++c) z[c] = 0.0;
+    SepIn q;
+    make_sep_in(t, z, is, w, q);
+    const double ey0 = xb.get(ey_slot), ey1 = xb.get(ey_slot + 1);
+    double ss[P];
+#pragma unroll
+    for (int c = 0; c < P; ++c) ss[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int j = k >= n ? 1 : 0, i = k - j * n;
+        const double e = xb.get(i) * (j == 0 ? ey0 : ey1);
+        const double u = (double)i - q.x1, v = (double)j - q.y1;
+        const double ae = q.A1 * e;
+        double row[P];
+        row[0] = e * is;
+        if constexpr (P == 5) row[1] = q.kr * ae * dfma(u, u, v * v);
+        row[X] = q.kx * ae * u;
+        row[X + 1] = q.kx * ae * v;
+        row[P - 1] = is;
+#pragma unroll
+        for (int c = 0; c < P; ++c) ss[c] = dfma(row[c], row[c], ss[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < P; ++c) cn[c] = dsqrt(ss[c] == 0.0 ? 1.0 : ss[c]);
+}
+
 /* Full normal equations from the pass accumulators. */
 template <int P, typename Acc>
 LZR_HD LZR_INLINE void assemble(const PassAcc<P, Acc> &a, int npix, double is, double (&H)[NP<P>::S], double (&g)[P]) {
@@ -1349,8 +1438,9 @@ LZR_HD LZR_INLINE void store_final(LaneState<P> &s, int lm_status, int model, Lz
 
 /* The bookkeeping of one pass; true when a new trial is to be proposed, otherwise false
  * with the fit's final status in *st_out. */
-template <int P, typename Acc>
-LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int *st_out) {
+template <int P, typename Acc, typename XBuf>
+LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, const XBuf &xb,
+                                 int *st_out) {
     LZR_NO_CONTRACT
     const int npix = s.n * s.n;
     if (s.phase == kInit) {
@@ -1447,7 +1537,14 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
                 cn[i] = dsqrt(hii == 0.0 ? 1.0 : hii);
             }
         } else {
-            partial_column_norms<P>(s.th, s.n, cfg.inv_sigma, cfg.initial_psf_width, rows, cn);
+            bool done = false;
+            if constexpr (P != 7) {
+                if (rows == P) {
+                    sep_diag_norms<P>(s.th, s.n, cfg.inv_sigma, cfg.initial_psf_width, xb, xb_ey_slot(cfg), cn);
+                    done = true;
+                }
+            }
+            if (!done) partial_column_norms<P>(s.th, s.n, cfg.inv_sigma, cfg.initial_psf_width, rows, cn);
         }
 #pragma unroll
         for (int i = 0; i < P; ++i)
@@ -1479,11 +1576,11 @@ LZR_HD LZR_INLINE bool lane_step(LaneState<P> &s, const LaneConfig &cfg, const P
  * when the fit is complete (*out written unless out is null).  Every path that continues ends in the one
  * call of propose (lmpar) at the bottom, so the lanes of a wave run lmpar together
  * whichever branch each of them took. */
-template <int P, typename Acc>
-LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a, int model,
-                                       LzrRawFit *out) {
+template <int P, typename Acc, typename XBuf>
+LZR_HD LZR_INLINE bool lane_after_pass(LaneState<P> &s, const LaneConfig &cfg, const PassAcc<P, Acc> &a,
+                                       const XBuf &xb, int model, LzrRawFit *out) {
     int st = LZR_LM_CONTINUE;
-    if (lane_step<P, Acc>(s, cfg, a, &st)) {
+    if (lane_step<P, Acc>(s, cfg, a, xb, &st)) {
         propose<P>(s);
         return true;
     }

@@ -29,7 +29,7 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             float raw(int i, int j) const { return D[i + n * j]; }
         } pix{D, n};
         struct {
-            double v[2 * LZR_MAX_HALF_WIDTH + 1];
+            double v[2 * LZR_MAX_HALF_WIDTH + 1 + kXbEySlots];
             void set(int i, double x) { v[i] = x; }
             double get(int i) const { return v[i]; }
         } xb;
@@ -43,10 +43,10 @@ static void fit_one(const LaneConfig &cfg, const uint16_t *frames, int nx, int n
             SepIn q;
             lane_sep_in<P>(s, cfg, q);
             double v[kSepSums];
-            sep_pass(q, n, n, false, pix, xb, v);
+            sep_pass(q, n, n, false, pix, xb, v, xb_ey_slot(cfg));
             sep_finish<P, double>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, a);
         }
-        if (!lane_after_pass<P, double>(s, cfg, a, model, reinterpret_cast<LzrRawFit *>(&out))) {
+        if (!lane_after_pass<P, double>(s, cfg, a, xb, model, reinterpret_cast<LzrRawFit *>(&out))) {
             finish_any(out, cfg, c);
             return;
         }
@@ -119,7 +119,7 @@ static void sep_one(const double *theta, const double *dx, double sigma, double 
         double get(int i) const { return v[i]; }
     } xb;
     double v[kSepSums];
-    sep_pass(q, n, nloop, nloop != n, pix, xb, v);
+    sep_pass(q, n, nloop, nloop != n, pix, xb, v, 62);
     PassAcc<P, double> a;
     a.zero();
     sep_finish<P, double>(q, n * n, v, true, 0.0, a);
