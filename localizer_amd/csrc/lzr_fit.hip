@@ -558,13 +558,14 @@ template <int P, typename Acc, int LK, typename PixT>
 __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
                                                  const Parked<P> *park, double *lds_words) {
     /* LDS: per-lane FP64 column buffer [nmax + 2][64] (as in the first kernel), the row exchange
-     * [slots][64], then the windows: lane r of a group holds rows r, r + k, ...: pixel
-     * (i, r + k m) at [(i + n m) * 64 + lane] (for k = 1 the first kernel's layout) */
+     * [slots][64] (groups only), then the windows: lane r of a group holds rows r, r + k, ...:
+     * pixel (i, r + k m) at [(i + n m) * 64 + lane] (for k = 1 the first kernel's layout, so the
+     * one-lane build needs the first kernel's LDS only) */
     const int nmax = 2 * args.cfg.max_half_width + 1;
     const int ey_slot = xb_ey_slot(args.cfg);
     double *xcol = lds_words;
     double *xchg = lds_words + (nmax + kXbEySlots) * 64;
-    PixT *tile = reinterpret_cast<PixT *>(xchg + xchg_slots(P) * 64);
+    PixT *tile = reinterpret_cast<PixT *>(xchg + (LK == 0 ? 0 : xchg_slots(P)) * 64);
     constexpr int lk = LK, kGroup = 1 << LK, kMask = kGroup - 1;
     const int lane = threadIdx.x, grp = lane >> lk, r = lane & kMask, gbase = grp * kGroup;
 
@@ -1003,9 +1004,9 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
         if constexpr (P != 7) {
             int per_cu2 = 0;
             LZR_CHECK(
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, lm_straggler<P, Acc, 2, kOneLane, PixT>, 64, slds));
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, lm_straggler<P, Acc, 2, kOneLane, PixT>, 64, lds));
             if (per_cu2 < 1) throw std::string("straggler kernel cannot be resident with this window size");
-            hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane, PixT>), dim3(per_cu2 * ctx->num_cus), dim3(64), slds,
+            hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane, PixT>), dim3(per_cu2 * ctx->num_cus), dim3(64), lds,
                                stream, a, cls, lanes);
         }
     }
