@@ -109,7 +109,8 @@ def test_gpu_autotuner_keeps_results():
                                                        ("waves_per_cu", abi.OPT_WAVES_PER_CU),
                                                        ("rounds", abi.OPT_ROUNDS))}
         _say(f"autotune: best {best:.3f} ms with {chosen}")
-        assert chosen["occupancy"] in (1, 2) and chosen["rounds"] in (1, 2)
+        # the context's own (automatic, 0) configuration is a candidate too
+        assert chosen["occupancy"] in (0, 1, 2) and chosen["rounds"] in (1, 2)
         got = ctx.fit(frames, cands)
     assert np.array_equal(got["status"], ref["status"])
     np.testing.assert_allclose(got["params"], ref["params"], rtol=1e-12, atol=0)
