@@ -263,6 +263,10 @@ __device__ inline uint32_t cvt_pk(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
 }
 
+/* ReLU as one integer max on the bit pattern (a float's bits order like the float for
+ * non-negative values; negative values, -0 included, have the sign bit set and map to +0). */
+__device__ inline float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 /* Every layer is computed transposed, C[feature][patch] = W * X^T: the weights are the
  * A operand (registers / L2) and the activations the B operand (LDS), so a lane's four
  * outputs are four consecutive features of one patch (one 8-byte LDS store), and the
@@ -307,9 +311,11 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
     }
     /* conv1 taps of this lane's k-slots 4*lg + i */
     int toff[4];
+    bool kval[4]; /* k-slots beyond the 9 taps read the patch's first pixel and are zeroed */
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int k = 4 * lg + i;
+        kval[i] = k < 9;
         toff[i] = k < 9 ? (k / 3) * kN + k % 3 : 0;
     }
 
@@ -350,7 +356,18 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
                 const int py = p / kS1, px = p - py * kS1;
                 const uint16_t *xr = xs + lr * kXsStride + py * kN + px;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) bx[u][i] = 4 * lg + i < 9 ? (short)xr[toff[i]] : (short)0;
+                for (int i = 0; i < 4; ++i) {
+                    const short v = (short)xr[toff[i]]; /* unconditional: no branch per load */
+                    bx[u][i] = kval[i] ? v : (short)0;
+                }
+            }
+            /* both positions' products first, so that neither waits on the other's results */
+            f32x4_t cc[2][2];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) cc[0][mt] = mfma16(w1r[mt], bx[0], b1v[mt]);
+            if (pp[1] >= 0) {
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) cc[1][mt] = mfma16(w1r[mt], bx[1], b1v[mt]);
             }
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
@@ -358,9 +375,8 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
                 const int p = pp[u];
 #pragma unroll
                 for (int mt = 0; mt < 2; ++mt) {
-                    const f32x4_t c = mfma16(w1r[mt], bx[u], b1v[mt]);
-                    const float v0 = fmaxf(c[0], 0.f), v1 = fmaxf(c[1], 0.f), v2 = fmaxf(c[2], 0.f),
-                                v3 = fmaxf(c[3], 0.f);
+                    const f32x4_t c = cc[u][mt];
+                    const float v0 = relu(c[0]), v1 = relu(c[1]), v2 = relu(c[2]), v3 = relu(c[3]);
                     u32x2_t pk;
                     pk[0] = cvt_pk(v0, v1);
                     pk[1] = cvt_pk(v2, v3);
@@ -401,10 +417,10 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
                 c1 = mfma32(w2r[t][1], b, c1);
             }
             u32x4_t bq;
-            bq[0] = cvt_pk(fmaxf(c0[0], 0.f), fmaxf(c0[1], 0.f));
-            bq[1] = cvt_pk(fmaxf(c0[2], 0.f), fmaxf(c0[3], 0.f));
-            bq[2] = cvt_pk(fmaxf(c1[0], 0.f), fmaxf(c1[1], 0.f));
-            bq[3] = cvt_pk(fmaxf(c1[2], 0.f), fmaxf(c1[3], 0.f));
+            bq[0] = cvt_pk(relu(c0[0]), relu(c0[1]));
+            bq[1] = cvt_pk(relu(c0[2]), relu(c0[3]));
+            bq[2] = cvt_pk(relu(c1[0]), relu(c1[1]));
+            bq[3] = cvt_pk(relu(c1[2]), relu(c1[3]));
             if (kTrain) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
@@ -413,22 +429,24 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
                     T.a2T[(size_t)(q * kC + o) * T.ldb + roi0 + lr] = v;
                 }
             }
-            u32x4_t cur[4];
+            /* fc1 K-chunk q, then the weights of this wave's next position into the same registers */
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) cur[mt] = wf[mt];
+            for (int mt = 0; mt < 4; ++mt) hacc[mt] = mfma32(wf[mt], bq, hacc[mt]);
             if (q + kWaves < kP2) {
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) wf[mt] = ld16(W.wf1 + (((q + kWaves) * 4 + mt) * 64 + lane) * 8);
             }
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt) hacc[mt] = mfma32(cur[mt], bq, hacc[mt]);
         }
         __syncthreads();
-        float *red = reinterpret_cast<float *>(a1s); /* [waves][64 j][16 patches], a1 no longer needed */
+        /* [waves][64 j][kRedStride]: a1 is no longer needed.  A row of 16 patches is padded to 20
+         * floats so that the four 16-lane quarters of a wave's stores fall on different banks. */
+        constexpr int kRedStride = kGroup + 4;
+        static_assert(kWaves * kHid * kRedStride * 4 <= kGroup * kA1Stride * 2, "reduction buffer");
+        float *red = reinterpret_cast<float *>(a1s);
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) red[(wave * kHid + mt * 16 + 4 * lg + i) * kGroup + lr] = hacc[mt][i];
+            for (int i = 0; i < 4; ++i) red[(wave * kHid + mt * 16 + 4 * lg + i) * kRedStride + lr] = hacc[mt][i];
         __syncthreads();
 
         /* phase 3: h = relu(sum over waves + b) */
@@ -436,8 +454,8 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
             const int r = e & (kGroup - 1), j = e >> 4;
             float s = W.params[OFF_BF1 + j];
 #pragma unroll
-            for (int w = 0; w < kWaves; ++w) s += red[(w * kHid + j) * kGroup + r];
-            const uint16_t hb = f2bf(fmaxf(s, 0.f));
+            for (int w = 0; w < kWaves; ++w) s += red[(w * kHid + j) * kRedStride + r];
+            const uint16_t hb = f2bf(relu(s));
             hs[r * kHsStride + j] = hb;
             if (kTrain) T.h[(size_t)(roi0 + r) * kHid + j] = hb;
         }
