@@ -279,7 +279,8 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
     __shared__ __attribute__((aligned(16))) uint16_t xs[kGroup * kXsStride];
     __shared__ __attribute__((aligned(16))) uint16_t a1s[kGroup * kA1Stride];
     __shared__ __attribute__((aligned(16))) uint16_t hs[kGroup * kHsStride];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6); /* uniform: position indices in scalar registers */
     const int lr = lane & 15, lg = lane >> 4;
 
     /* weights in MFMA A-operand order, once per wave */
@@ -409,12 +410,15 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
             const int qy = q / kS2, qx = q - qy * kS2;
             f32x4_t c0 = b2v[0], c1 = b2v[1];
             const uint16_t *ar = a1s + lr * kA1Stride + lg * 8;
+            /* the nine B operands read before the products: one LDS wait instead of one per tap */
+            u32x4_t b[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) b[t] = ld16(ar + ((qy + t / 3) * kS1 + qx + t % 3) * kC);
+            if constexpr (!kTrain) __builtin_amdgcn_sched_barrier(0); /* keep the reads ahead of the products */
 #pragma unroll
             for (int t = 0; t < 9; ++t) {
-                const int pos = (qy + t / 3) * kS1 + qx + t % 3;
-                const u32x4_t b = ld16(ar + pos * kC);
-                c0 = mfma32(w2r[t][0], b, c0);
-                c1 = mfma32(w2r[t][1], b, c1);
+                c0 = mfma32(w2r[t][0], b[t], c0);
+                c1 = mfma32(w2r[t][1], b[t], c1);
             }
             u32x4_t bq;
             bq[0] = cvt_pk(relu(c0[0]), relu(c0[1]));
