@@ -1505,12 +1505,19 @@ __device__ double mark_sse(uint16_t *buf, MarkMeta &meta, const uint16_t *src, i
     meta.src[lane] = src;
     meta.n[lane] = n;
     meta.inv[lane] = n > 0 ? 65536 / n + 1 : 0; /* r = t / n as (t * inv) >> 16, exact for t < n^2, n <= 31 */
+    /* Gaussian factors by the recurrence of the fit's own passes (three exponentials per axis),
+     * or directly where an exponent at the window's ends could leave +-690 */
+    const double qexp = dexp(-2.0 * k);
+    GaussRec gx, gy;
+    gx.start(k, x0, n, qexp);
+    gy.start(k, y0, n, qexp);
     double ex[NMAX];
     double sx = 0.0, sxx = 0.0;
 #pragma unroll
     for (int i = 0; i < NMAX; ++i) {
         const double u = (double)i - x0;
-        ex[i] = i < n ? dexp(-(u * u) * k) : 0.0;
+        const double e = gx.next();
+        ex[i] = i < n ? (gx.ok ? e : dexp(-(u * u) * k)) : 0.0;
         sx += ex[i];
         sxx = dfma(ex[i], ex[i], sxx);
     }
@@ -1546,7 +1553,8 @@ __device__ double mark_sse(uint16_t *buf, MarkMeta &meta, const uint16_t *src, i
         const int rows = min(R, n - j0);
         for (int r = 0; r < rows; ++r) {
             const double v = (double)(j0 + r) - y0;
-            const double ey = dexp(-(v * v) * k);
+            const double eyr = gy.next(); /* rows in order 0, 1, ... across the passes */
+            const double ey = gy.ok ? eyr : dexp(-(v * v) * k);
             sy += ey;
             syy = dfma(ey, ey, syy);
             const uint16_t *row = buf + lane * S + r * NMAX;
@@ -1593,12 +1601,19 @@ __device__ double start_distance_bound(const LzrFit &f, const LzrCandidate &c, i
     }
     const double k0 = 0.5 / (w0 * w0), k1 = 0.5 / (r1 * r1);
     double s00[2] = {0.0, 0.0}, s11[2] = {0.0, 0.0}, s01[2] = {0.0, 0.0}, t0[2] = {0.0, 0.0}, t1[2] = {0.0, 0.0};
+    /* the Gaussian factors by recurrence (three exponentials per axis and surface); their few
+     * ulps are far inside the 1e-12 margin below */
+    const double q0 = dexp(-2.0 * k0), q1 = dexp(-2.0 * k1);
     for (int a = 0; a < 2; ++a) {
         const double o = a == 0 ? (double)xoff : (double)yoff;
         const double p0 = a == 0 ? c.x : c.y, p1 = a == 0 ? x1 : y1;
+        GaussRec g0, g1;
+        g0.start(k0, p0 - o, n, q0);
+        g1.start(k1, p1 - o, n, q1);
         for (int i = 0; i < n; ++i) {
             const double u0 = o + (double)i - p0, u1 = o + (double)i - p1;
-            const double e0 = dexp(-(u0 * u0) * k0), e1 = dexp(-(u1 * u1) * k1);
+            const double r0 = g0.next(), r1 = g1.next();
+            const double e0 = g0.ok ? r0 : dexp(-(u0 * u0) * k0), e1 = g1.ok ? r1 : dexp(-(u1 * u1) * k1);
             s00[a] = dfma(e0, e0, s00[a]);
             s11[a] = dfma(e1, e1, s11[a]);
             s01[a] = dfma(e0, e1, s01[a]);
