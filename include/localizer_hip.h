@@ -117,6 +117,7 @@ typedef struct LzrContext LzrContext;
 #define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues in the straggler kernel, on 1..16 lanes by the count of such fits (default 48, 0 = off) */
 #define LZR_OPT_DEFAULT_HALF_WIDTH 11 /* window half-width of candidates with half_width 0 (read-only: set at lzr_create) */
 #define LZR_OPT_STRAGGLER_GROUP 12  /* lanes per fit in the straggler kernel (64/lanes ROIs per wavefront): 1, 2, 4, 8, 16, or 0 (default) = by the number of handed-off fits */
+#define LZR_OPT_NET_MAX_INTENSITY 13 /* lzr_fit_batch_device_netinit starts from the network only the candidates whose intensity is below this value, the others from the reference start (default +inf: every candidate) */
 
 /* Creates a fitting context on the current HIP device.
  * Replaces the constructors FitPositions_SymmetricGaussian(initialPSFWidth, sigma)

@@ -46,6 +46,7 @@ OPT_PARK_THRESHOLD = 9
 OPT_STRAGGLER_TRIALS = 10
 OPT_DEFAULT_HALF_WIDTH = 11
 OPT_STRAGGLER_GROUP = 12
+OPT_NET_MAX_INTENSITY = 13
 
 CANDIDATE_DTYPE = np.dtype(
     [

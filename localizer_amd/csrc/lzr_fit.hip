@@ -33,6 +33,7 @@
 
 #include "lzr_lane.h"
 #include "lzr_order.h"
+#include "lzr_netinit.h"
 
 #ifndef LZR_STRAGGLER_GROUP
 #define LZR_STRAGGLER_GROUP 16 /* lanes per fit in the straggler kernel */
@@ -876,6 +877,7 @@ struct LzrContext {
     int park_threshold = 16; /* park a wave's fits when this few lanes are still busy */
     int strag_trials = 48;   /* hand a fit to the straggler kernel after this many trials */
     int strag_group = 0;     /* lanes per handed-off fit (1, 2, 4, 8, 16), 0: by the hand-off count */
+    double net_max_intensity = __builtin_inf(); /* net-init: the network starts candidates below this intensity */
     Workspace ws;            /* the eager calls' work memory */
     hipStream_t side[kClasses - 1] = {}; /* classes 1, 2 run beside class 0 */
     hipEvent_t fork = nullptr, join[kClasses - 1] = {};
@@ -883,6 +885,10 @@ struct LzrContext {
     int num_cus = 0;
     void *scratch = nullptr; /* this device's fixed scratch array (allocation-free entry) */
     unsigned *netinit_words = nullptr; /* net-init: refit count, live count of the refit launch */
+    double *netinit_sse = nullptr;     /* net-init: start-point |f|^2 over the network's patches */
+    size_t netinit_sse_cap = 0;
+    int *netinit_sel = nullptr;        /* net-init: the candidates the network starts */
+    size_t netinit_sel_cap = 0;
     std::string err;
     std::mutex mu;
     /* staging of the host-buffer entry point */
@@ -1236,6 +1242,7 @@ LZR_EXPORT int lzr_clone(LzrContext *src, LzrContext **out) {
     c->park_threshold = src->park_threshold;
     c->strag_trials = src->strag_trials;
     c->strag_group = src->strag_group;
+    c->net_max_intensity = src->net_max_intensity;
     *out = c;
     return 0;
 }
@@ -1247,6 +1254,8 @@ LZR_EXPORT int lzr_destroy(LzrContext *ctx) {
     (void)hipFree(ctx->d_cands);
     (void)hipFree(ctx->d_out);
     (void)hipFree(ctx->netinit_words);
+    (void)hipFree(ctx->netinit_sse);
+    (void)hipFree(ctx->netinit_sel);
     release(ctx->ws);
     for (int c = 0; c < kClasses - 1; ++c) {
         if (ctx->side[c]) (void)hipStreamDestroy(ctx->side[c]);
@@ -1322,6 +1331,10 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
         ctx->strag_group = g;
         return 0;
     }
+    case LZR_OPT_NET_MAX_INTENSITY:
+        if (value != value) return fail(ctx, "net max intensity must not be NaN");
+        ctx->net_max_intensity = value;
+        return 0;
     default:
         return fail(ctx, "unknown option");
     }
@@ -1634,14 +1647,17 @@ __device__ double start_distance_bound(const LzrFit &f, const LzrCandidate &c, i
  * (refit) when the net start was used (width > 0) and the fit was rejected or did not
  * converge (with refit_rejected), or ended with a residual |f| above factor times the residual
  * at the reference's own start point (the normalized residuals chi/sqrt(dof) share the dof);
- * width -1 (keep) otherwise.  A fit that start_distance_bound already clears needs no pixel
- * reads; the others' start-point residual is summed over the window (mark_sse) and left in
- * amplitude (NaN for every other candidate).  *nrefit (zeroed before) counts the refits.
- * One wave per workgroup (the window staging of mark_sse). */
+ * width -1 (keep) otherwise.  A fit window that is the network's patch (half-width 5) takes
+ * the start-point |f|^2 the network's patch pass summed (patch_sse); of the others, a fit that
+ * start_distance_bound already clears needs no pixel reads, and the rest's start-point
+ * residual is summed over the window (mark_sse).  The residual is left in amplitude (NaN for
+ * every other candidate).  *nrefit (zeroed before) counts the refits.  One wave per
+ * workgroup (the window staging of mark_sse). */
 __global__ void __launch_bounds__(64) mark_fallback_kernel(const uint16_t *frames, int nx, int ny, int nframes,
                                                            const LzrCandidate *cands, const LzrFit *fits,
                                                            LzrStart *starts, int n, LaneConfig cfg, double factor,
-                                                           int refit_rejected, unsigned *nrefit) {
+                                                           int refit_rejected, const double *patch_sse,
+                                                           unsigned *nrefit) {
     LZR_NO_CONTRACT
     __shared__ uint16_t buf[kMarkLdsHalf];
     __shared__ MarkMeta meta;
@@ -1657,12 +1673,16 @@ __global__ void __launch_bounds__(64) mark_fallback_kernel(const uint16_t *frame
     }
     int nwin = 0;
     const uint16_t *src = frames;
-    double x0 = 0.0, y0 = 0.0, A = 0.0, b = 0.0;
+    double x0 = 0.0, y0 = 0.0, A = 0.0, b = 0.0, sse_pre = __builtin_nan("");
     if (examine) {
         const LzrCandidate c = cands[i];
         LaneState<5> s;
         LzrFit unused;
-        if (lane_start<5>(s, cfg, c, 0, nx, ny, nframes, LZR_MODEL_SYMMETRIC, unused)) {
+        if (!lane_start<5>(s, cfg, c, 0, nx, ny, nframes, LZR_MODEL_SYMMETRIC, unused)) {
+            /* window outside the frame: no start residual, kept */
+        } else if (patch_sse && s.n == 2 * LZR_NET_HALF_WIDTH + 1) {
+            sse_pre = patch_sse[i]; /* the same window: x_offset = floor(x - 5) */
+        } else {
             const double dm = start_distance_bound(fits[i], c, s.xoff, s.yoff, s.n, cfg.initial_psf_width);
             /* |f_start| >= chi - dm > chi (1 + 1e-6) / factor: no refit, whatever the pixels */
             cleared = factor * (chi - dm * cfg.inv_sigma) > chi * (1.0 + 1e-6);
@@ -1690,6 +1710,7 @@ __global__ void __launch_bounds__(64) mark_fallback_kernel(const uint16_t *frame
             sse = mark_sse<31>(buf, meta, src, nwin, nmax, nx, x0, y0, k, A, b);
         if (nwin > 0) chi0 = dsqrt(dmax(sse, 0.0)) * cfg.inv_sigma;
     }
+    if (sse_pre == sse_pre) chi0 = dsqrt(dmax(sse_pre, 0.0)) * cfg.inv_sigma;
     if (examine && !cleared) refit = chi > factor * chi0; /* NaN (window outside the frame): kept */
     if (i < n) {
         starts[i].width = refit ? 0.0 : -1.0;
@@ -1705,6 +1726,31 @@ __global__ void refit_live_kernel(const unsigned *nrefit, int *live, int n) {
     if (threadIdx.x == 0) *live = *nrefit ? n : 0;
 }
 
+/* The candidates the network starts: those with intensity below max_intensity, listed in
+ * sel (in no particular order: a candidate's prediction does not depend on its row), counted
+ * in *nsel (zeroed before); every other candidate gets the start of width 0, i.e. the
+ * reference start. */
+__global__ void select_net_kernel(const LzrCandidate *cands, int n, double max_intensity, LzrStart *starts,
+                                  int *sel, unsigned *nsel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool use = i < n && cands[i].intensity < max_intensity;
+    if (i < n && !use) {
+        LzrStart st;
+        st.amplitude = 0.0;
+        st.width = 0.0;
+        st.x = 0.0;
+        st.y = 0.0;
+        st.background = 0.0;
+        starts[i] = st;
+    }
+    const unsigned long long m = __ballot(use);
+    unsigned base = 0;
+    const int lane = threadIdx.x & 63;
+    if (m && lane == __ffsll((long long)m) - 1) base = atomicAdd(nsel, (unsigned)__popcll(m));
+    base = __shfl(base, __ffsll((long long)(m ? m : 1ull)) - 1, 64);
+    if (use) sel[base + __popcll(m & ((1ull << lane) - 1))] = i;
+}
+
 }  // namespace lzr
 
 LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const uint16_t *d_frames, int nx, int ny,
@@ -1714,25 +1760,51 @@ LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const 
     if (!ctx) return -1;
     if (!net) return fail(ctx, "null network");
     if (!d_work && ncands > 0) return fail(ctx, "null work buffer");
-    if (lzr_net_predict_device(net, d_frames, nx, ny, nframes, d_cands, ncands, d_work, stream) != 0) {
+    /* the context first (its start-residual buffer), then the network */
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    double *patch_sse = nullptr;
+    int *sel = nullptr;
+    unsigned *nsel = nullptr;
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        if (ncands > 0 && d_cands && d_work) {
+            hipStream_t s = (hipStream_t)stream;
+            /* the previous call on another stream may still read the buffers */
+            LZR_CHECK(ctx->order.begin(s));
+            ensure(ctx->netinit_sse, ctx->netinit_sse_cap, (size_t)ncands);
+            patch_sse = ctx->netinit_sse;
+            if (!ctx->netinit_words) LZR_CHECK(hipMalloc(&ctx->netinit_words, 3 * sizeof(unsigned)));
+            if (ctx->net_max_intensity < __builtin_inf()) {
+                ensure(ctx->netinit_sel, ctx->netinit_sel_cap, (size_t)ncands);
+                sel = ctx->netinit_sel;
+                nsel = ctx->netinit_words + 2;
+                hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, s, nsel, 1);
+                hipLaunchKernelGGL(select_net_kernel, dim3((ncands + 255) / 256), dim3(256), 0, s, d_cands, ncands,
+                                   ctx->net_max_intensity, d_work, sel, nsel);
+                LZR_CHECK(hipGetLastError());
+            }
+        }
+    } catch (const std::string &e) {
+        return fail(ctx, e);
+    }
+    if (lzr_net_predict_start_sse(net, d_frames, nx, ny, nframes, d_cands, ncands, d_work, ctx->cfg.initial_psf_width,
+                                  patch_sse, sel, nsel, stream) != 0) {
         char buf[512];
         lzr_net_last_error(net, buf, (int)sizeof buf);
         return fail(ctx, std::string("network: ") + buf);
     }
-    std::lock_guard<std::mutex> lock(ctx->mu);
     if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
     if (ncands == 0) return 0;
     if (!d_frames || !d_cands || !d_out) return fail(ctx, "null device pointer");
     try {
-        LZR_CHECK(hipSetDevice(ctx->device));
         hipStream_t s = (hipStream_t)stream;
-        if (!ctx->netinit_words) LZR_CHECK(hipMalloc(&ctx->netinit_words, 2 * sizeof(unsigned)));
         unsigned *nrefit = d_nrefit ? d_nrefit : ctx->netinit_words;
         int *live = reinterpret_cast<int *>(ctx->netinit_words + 1);
         launch(ctx, d_frames, nx, ny, nframes, d_cands, d_work, ncands, d_out, s);
         hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, s, nrefit, 1);
         hipLaunchKernelGGL(mark_fallback_kernel, dim3((ncands + 63) / 64), dim3(64), 0, s, d_frames, nx, ny, nframes,
-                           d_cands, d_out, d_work, ncands, ctx->cfg, residual_factor, refit_rejected, nrefit);
+                           d_cands, d_out, d_work, ncands, ctx->cfg, residual_factor, refit_rejected,
+                           (const double *)patch_sse, nrefit);
         hipLaunchKernelGGL(refit_live_kernel, dim3(1), dim3(64), 0, s, nrefit, live, ncands);
         launch(ctx, ctx->ws, d_frames, nx, ny, nframes, d_cands, d_work, ncands, d_out, s, live, &ctx->order);
         LZR_CHECK(hipGetLastError());
@@ -1762,6 +1834,7 @@ LZR_EXPORT int lzr_get_option(LzrContext *ctx, int option, double *value) {
     case LZR_OPT_STRAGGLER_TRIALS: *value = ctx->strag_trials; return 0;
     case LZR_OPT_DEFAULT_HALF_WIDTH: *value = ctx->cfg.default_half_width; return 0;
     case LZR_OPT_STRAGGLER_GROUP: *value = ctx->strag_group; return 0;
+    case LZR_OPT_NET_MAX_INTENSITY: *value = ctx->net_max_intensity; return 0;
     default: return fail(ctx, "unknown option");
     }
 }

@@ -41,6 +41,7 @@
 
 #include "../../include/localizer_hip.h"
 #include "lzr_order.h"
+#include "lzr_netinit.h"
 
 #define NET_CHECK(call)                                                                     \
     do {                                                                                    \
@@ -175,15 +176,27 @@ __device__ inline uint32_t half_max16(uint32_t v) {
 
 __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int nx, int ny, int nframes,
                                                    const LzrCandidate *cands, int ncands, int nrows,
-                                                   uint16_t *xin, PatchStat *stats) {
+                                                   uint16_t *xin, PatchStat *stats, double k_start,
+                                                   double *start_sse, const int *sel_idx,
+                                                   const unsigned *sel_count) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
     const int r = (blockIdx.x * 4 + wave) * 2 + half;
+    if (sel_count) { /* rows [0, *sel_count) are the candidates sel_idx names */
+        nrows = (int)*sel_count;
+        ncands = nrows;
+    }
     const bool live = r < nrows;
     if (__ballot(live) == 0) return; /* wave-uniform */
     bool valid = live && r < ncands;
+    const int ci = sel_idx && valid ? sel_idx[r] : r; /* candidate of row r */
     int xoff = 0, yoff = 0, f = 0;
+    double cx = 0.0, cy = 0.0, ca = 0.0, cb = 0.0;
     if (valid) {
-        const LzrCandidate c = cands[r];
+        const LzrCandidate c = cands[ci];
+        cx = c.x;
+        cy = c.y;
+        ca = c.intensity;
+        cb = c.background;
         const double fx = c.x - (double)kH, fy = c.y - (double)kH;
         valid = c.frame >= 0 && c.frame < nframes && fx >= 0.0 && fy >= 0.0 && fx < 2147483647.0 &&
                 fy < 2147483647.0;
@@ -217,6 +230,26 @@ __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int n
             cnt += half ? __popcll(m >> 32) : __popcll(m & 0xffffffffull);
         }
         if (cnt <= kPix / 2) lo = t;
+    }
+    if (start_sse) { /* kernel argument: uniform */
+        /* |f|^2 over the patch at the reference start point (A = intensity, b = background,
+         * width w0 = sqrt(0.5 / k_start) at the candidate's position): lanes 0..10 of the half
+         * hold the column factors, lanes 11..21 the row factors, every pixel one product */
+        const double x0 = cx - (double)xoff, y0 = cy - (double)yoff;
+        const double t = hl < kN ? (double)hl - x0 : (double)(hl - kN) - y0;
+        const double e = exp(-(t * t) * k_start);
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = hl + 32 * q, row_i = i / kN, col_i = i - row_i * kN;
+            const double ex = __shfl(e, half * 32 + col_i, 64);
+            const double ey = __shfl(e, half * 32 + kN + row_i, 64); /* row_i <= 11 */
+            const double fv = __builtin_fma(ca, ex * ey, cb - (double)u[q]);
+            acc = has[q] ? __builtin_fma(fv, fv, acc) : acc;
+        }
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (live && hl == 0) start_sse[ci] = valid ? acc : __builtin_nan("");
     }
     uint16_t *row = xin + (size_t)(live ? r : 0) * kXRow;
     if (!live) return;
@@ -275,7 +308,8 @@ __device__ inline float relu(float x) { return __int_as_float(max(__float_as_int
  * o(g, i) = (i < 4 ? 4g + i : 16 + 4g + i - 4) (the fc1 weights are packed the same way). */
 template <bool kTrain>
 __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, int n, Packed W, float *out,
-                                                           TrainStore T) {
+                                                           TrainStore T, const unsigned *n_dev) {
+    if (n_dev) n = (int)*n_dev; /* rows counted on the device (at most the launch's n) */
     __shared__ __attribute__((aligned(16))) uint16_t xs[kGroup * kXsStride];
     __shared__ __attribute__((aligned(16))) uint16_t a1s[kGroup * kA1Stride];
     __shared__ __attribute__((aligned(16))) uint16_t hs[kGroup * kHsStride];
@@ -483,8 +517,9 @@ __global__ void __launch_bounds__(kThreads) forward_kernel(const uint16_t *xin, 
 /* ---- outputs -> start points / targets -------------------------------------------- */
 
 __global__ void decode_kernel(const float *out, const PatchStat *stats, int n, double w, LzrStart *starts,
-                              float *raw) {
+                              float *raw, const int *sel_idx = nullptr, const unsigned *sel_count = nullptr) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sel_count) n = (int)*sel_count;
     if (r >= n) return;
     const PatchStat s = stats[r];
     float t[kOut];
@@ -515,7 +550,7 @@ __global__ void decode_kernel(const float *out, const PatchStat *stats, int n, d
             st.background = (double)s.median + (double)t[4] * (double)s.scale;
         }
     }
-    starts[r] = st;
+    starts[sel_idx ? sel_idx[r] : r] = st;
 }
 
 /* Normalized targets from the LM labels; weight 1 for accepted symmetric fits of a
@@ -1044,10 +1079,11 @@ static bool net_shape_ok(LzrNet *net, const void *frames, int nx, int ny, int nf
 /* prep + forward for rows [0, nrows) (rows >= ncands are padding) */
 template <bool kTrain>
 static void run_forward(LzrNet *net, const uint16_t *frames, int nx, int ny, int nframes, const LzrCandidate *cands,
-                        int ncands, int nrows, hipStream_t s) {
+                        int ncands, int nrows, hipStream_t s, double k_start = 0.0, double *start_sse = nullptr,
+                        const int *sel_idx = nullptr, const unsigned *sel_count = nullptr) {
     ensure_rows(net, (size_t)nrows);
     hipLaunchKernelGGL(prep_kernel, dim3((nrows + 7) / 8), dim3(256), 0, s, frames, nx, ny, nframes, cands, ncands,
-                       nrows, net->d_xin, net->d_stats);
+                       nrows, net->d_xin, net->d_stats, k_start, start_sse, sel_idx, sel_count);
     TrainStore T{};
     if (kTrain) {
         T.xT = net->d_xT;
@@ -1059,7 +1095,7 @@ static void run_forward(LzrNet *net, const uint16_t *frames, int nx, int ny, int
     const int ngroups = (nrows + kGroup - 1) / kGroup;
     const int grid = std::max(1, std::min(ngroups, net->num_cus));
     hipLaunchKernelGGL((forward_kernel<kTrain>), dim3(grid), dim3(kThreads), 0, s, net->d_xin, nrows, packed(net),
-                       net->d_out, T);
+                       net->d_out, T, sel_count);
 }
 
 LZR_EXPORT int lzr_net_forward_device(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,
@@ -1084,8 +1120,9 @@ LZR_EXPORT int lzr_net_forward_device(LzrNet *net, const uint16_t *d_frames, int
     return 0;
 }
 
-LZR_EXPORT int lzr_net_predict_device(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,
-                                      const LzrCandidate *d_cands, int ncands, LzrStart *d_starts, void *stream) {
+int lzr_net_predict_start_sse(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,
+                              const LzrCandidate *d_cands, int ncands, LzrStart *d_starts, double w_start,
+                              double *d_start_sse, const int *d_sel, const unsigned *d_nsel, void *stream) {
     if (!net) return -1;
     std::lock_guard<std::mutex> lock(net->mu);
     if (!net_shape_ok(net, d_frames, nx, ny, nframes, d_cands, ncands)) return -1;
@@ -1095,15 +1132,22 @@ LZR_EXPORT int lzr_net_predict_device(LzrNet *net, const uint16_t *d_frames, int
         NET_CHECK(hipSetDevice(net->device));
         hipStream_t s = (hipStream_t)stream;
         NET_CHECK(net->order.begin(s));
-        run_forward<false>(net, d_frames, nx, ny, nframes, d_cands, ncands, ncands, s);
+        run_forward<false>(net, d_frames, nx, ny, nframes, d_cands, ncands, ncands, s, 0.5 / (w_start * w_start),
+                           d_start_sse, d_sel, d_nsel);
         hipLaunchKernelGGL(decode_kernel, dim3((ncands + 255) / 256), dim3(256), 0, s, net->d_out, net->d_stats,
-                           ncands, net->w, d_starts, (float *)nullptr);
+                           ncands, net->w, d_starts, (float *)nullptr, d_sel, d_nsel);
         NET_CHECK(hipGetLastError());
         NET_CHECK(net->order.end(s));
     } catch (const std::string &e) {
         return net_fail(net, e);
     }
     return 0;
+}
+
+LZR_EXPORT int lzr_net_predict_device(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                      const LzrCandidate *d_cands, int ncands, LzrStart *d_starts, void *stream) {
+    return lzr_net_predict_start_sse(net, d_frames, nx, ny, nframes, d_cands, ncands, d_starts, 1.0, nullptr, nullptr,
+                                     nullptr, stream);
 }
 
 LZR_EXPORT int lzr_net_train_step_device(LzrNet *net, const uint16_t *d_frames, int nx, int ny, int nframes,

@@ -69,3 +69,72 @@ def train(net, workload, nrois, epochs=1.0, batch=2048, lr=2e-3, lr_final=5e-5, 
     device.synchronize()
     return dict(steps=steps, batch=batch, rois=nrois, labelled_accepted=used, losses=losses,
                 simulate_s=t_sim, train_s=time.time() - t1)
+
+
+def best_threshold(intensity, counted, seconds):
+    """The LZR_OPT_NET_MAX_INTENSITY value, among measured candidates, with the highest rate
+    of counted localizations: `intensity` holds the thresholds tried, `counted[k]` and
+    `seconds[k]` what threshold k gave.  Ties go to the lower threshold (fewer network
+    starts).  Returns (threshold, index)."""
+    rate = np.asarray(counted, dtype=np.float64) / np.asarray(seconds, dtype=np.float64)
+    k = int(np.flatnonzero(rate == rate.max())[0])
+    return float(intensity[k]), k
+
+
+def calibrate_selection(net, workload, nrois=1 << 21, stream_id=None, fractions=None, reps=3, log=None):
+    """Chooses which candidates the network starts (LZR_OPT_NET_MAX_INTENSITY): on `nrois`
+    fresh ROIs of the workload, net-initialized fits with the network on the dimmest fraction
+    f of the candidates (by start intensity; f = 0 is the plain fit, f = 1 every candidate)
+    are timed, and the threshold with the most localizations per second that agree with the
+    plain fit (accepted by both, x and y within 1e-3 px) is returned with the table.  The
+    network pays where the reference start leaves a long fit, i.e. on dim candidates; on bright
+    ones it costs more than the few trials it saves."""
+    stream_id = workload.extra.get("calibrate_stream", 45) if stream_id is None else stream_id
+    fractions = (0.0, 0.05, 0.1, 0.15, 0.2, 0.3, 0.5, 1.0) if fractions is None else fractions
+    rois, _, cands = simulate(workload, nrois, stream_id)
+    n = len(cands)
+    d_frames, d_cands = device.upload(rois), device.upload(cands)
+    d_plain, d_out = device.empty(abi.FIT_DTYPE, n), device.empty(abi.FIT_DTYPE, n)
+    d_work = device.empty(abi.START_DTYPE, n)
+    s = device.stream_handle()
+    w = 2 * abi.NET_HALF_WIDTH + 1
+    order = np.sort(cands["intensity"])
+    table = []
+    with native.Context(abi.MODEL_SYMMETRIC, workload.initial_psf_width, 1.0, abi.NET_HALF_WIDTH) as ctx:
+        ctx.fit_device(d_frames.ptr, w, w, n, d_cands.ptr, n, d_plain.ptr, s)
+        device.synchronize()
+        plain = d_plain.numpy()
+        acc = plain["status"] == abi.ACCEPTED
+        for f in fractions:
+            if f <= 0.0:
+                thr = -math.inf
+            elif f >= 1.0:
+                thr = math.inf
+            else:
+                thr = float(order[min(n - 1, int(f * n))])
+            ctx.set_option(abi.OPT_NET_MAX_INTENSITY, thr)
+
+            def call():
+                if f <= 0.0:
+                    ctx.fit_device(d_frames.ptr, w, w, n, d_cands.ptr, n, d_out.ptr, s)
+                else:
+                    ctx.fit_device_netinit(net, d_frames.ptr, w, w, n, d_cands.ptr, n, d_out.ptr, d_work.ptr, 3.0,
+                                           stream=s)
+            call()
+            device.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call()
+            device.synchronize()
+            sec = (time.perf_counter() - t0) / reps
+            out = d_out.numpy()
+            both = acc & (out["status"] == abi.ACCEPTED)
+            dxy = np.abs(out["params"][:, 2:4] - plain["params"][:, 2:4]).max(axis=1)
+            counted = int(np.sum(both & (dxy <= 1e-3)))
+            table.append(dict(fraction=float(f), threshold=thr, seconds=sec, counted=counted))
+            if log:
+                log(f"net selection: dimmest {f:.0%} (intensity < {thr:.4g}): {sec * 1e3:.2f} ms, "
+                    f"{counted} counted, {counted / sec:.4g} loc/s")
+    thr, k = best_threshold([t["threshold"] for t in table], [t["counted"] for t in table],
+                            [t["seconds"] for t in table])
+    return dict(threshold=thr, fraction=table[k]["fraction"], rois=n, table=table)

@@ -274,6 +274,50 @@ def test_gpu_corrupted_network_falls_back_to_plain_records(trained, refit_reject
         assert st_agree >= 0.95 and xy_ok >= 0.98
 
 
+def test_gpu_network_on_selected_candidates(trained):
+    """LZR_OPT_NET_MAX_INTENSITY: candidates below the threshold get exactly the records of
+    the network-on-every-candidate call (their predictions do not depend on which other
+    candidates share the network's batch), the others exactly the plain records; -inf
+    gives the plain records throughout.  The work buffer's start residuals follow the same
+    split."""
+    _, params, frames, cands = trained
+    n = len(cands)
+    thr = float(np.median(cands["intensity"]))
+    sel = cands["intensity"] < thr
+    _say(f"case: network on {int(sel.sum())} of {n} candidates (intensity < {thr:.4g})")
+    d_frames, d_cands = device.upload(frames), device.upload(cands)
+    s = device.stream_handle()
+    out = {}
+    with native.Net(W.initial_psf_width) as net, \
+            native.Context(abi.MODEL_SYMMETRIC, W.initial_psf_width, default_half_width=5) as ctx:
+        net.set_params(params)
+        d_plain = device.empty(abi.FIT_DTYPE, n)
+        ctx.fit_device(d_frames.ptr, 11, 11, len(frames), d_cands.ptr, n, d_plain.ptr, s)
+        for name, t in (("all", np.inf), ("sel", thr), ("none", -np.inf), ("sel2", thr)):
+            ctx.set_option(abi.OPT_NET_MAX_INTENSITY, t)
+            assert ctx.get_option(abi.OPT_NET_MAX_INTENSITY) == t
+            d_out, d_work = device.empty(abi.FIT_DTYPE, n), device.empty(abi.START_DTYPE, n)
+            d_nrefit = device.upload(np.array([777], np.uint32))
+            ctx.fit_device_netinit(net, d_frames.ptr, 11, 11, len(frames), d_cands.ptr, n, d_out.ptr, d_work.ptr,
+                                   3.0, refit_rejected=True, d_nrefit=d_nrefit.ptr, stream=s)
+            out[name] = (d_out.numpy(), d_work.numpy(), int(d_nrefit.numpy()[0]))
+        with pytest.raises(native.LocalizerError):
+            ctx.set_option(abi.OPT_NET_MAX_INTENSITY, np.nan)
+    plain = d_plain.numpy()
+    fa, wa, _ = out["all"]
+    fs, ws, nrs = out["sel"]
+    assert fs[sel].tobytes() == fa[sel].tobytes()
+    assert fs[~sel].tobytes() == plain[~sel].tobytes()
+    assert out["none"][0].tobytes() == plain.tobytes() and out["none"][2] == 0
+    assert out["sel2"][0].tobytes() == fs.tobytes()  # repeatable
+    assert np.all(np.isnan(ws["amplitude"][~sel]))
+    both = sel & ~np.isnan(wa["amplitude"])
+    assert np.array_equal(ws["amplitude"][both], wa["amplitude"][both])
+    assert nrs == int(np.sum(ws["width"] == 0.0))
+    assert np.array_equal(ws["width"][sel] == 0.0, wa["width"][sel] == 0.0)
+    _say(f"selected: {int(sel.sum())} net starts, {nrs} refits; unselected records equal the plain records")
+
+
 @pytest.mark.parametrize("h", [4, 8, 12])
 def test_gpu_fallback_start_residual_all_window_classes(h):
     """The fallback test's start-point residual on the device (window rows staged through

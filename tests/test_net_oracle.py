@@ -102,3 +102,13 @@ def test_rounded_forward_stays_close_to_fp64(rounding):
     t64 = pynet.forward(params, x, rounding=False)["t"]
     t = pynet.forward(params, x, rounding=rounding)["t"]
     assert np.abs(t - t64).max() <= (5e-2 if rounding else 0.0) * np.abs(t64).max()
+
+
+def test_net_selection_threshold_choice():
+    """netinit.best_threshold: the candidate threshold with the most counted localizations
+    per second wins; equal rates go to the lower threshold (fewer network starts)."""
+    from localizer_amd import netinit
+    thr = [-np.inf, 10.0, 20.0, np.inf]
+    assert netinit.best_threshold(thr, [100, 110, 120, 125], [1.0, 1.0, 1.0, 1.1]) == (20.0, 2)
+    assert netinit.best_threshold(thr, [100, 100, 100, 100], [1.0, 1.0, 1.0, 1.0]) == (-np.inf, 0)
+    assert netinit.best_threshold(thr, [100, 100, 100, 100], [2.0, 1.5, 1.0, 0.5]) == (np.inf, 3)
