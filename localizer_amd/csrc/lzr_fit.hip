@@ -197,9 +197,20 @@ __global__ void __launch_bounds__(kSortThreads) scatter_kernel(const LzrCandidat
  * covariance from a QR factor of the reference Jacobian at the final point; 2x2
  * eigen-decomposition), one thread per candidate. */
 __global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, const int *d_count, LzrFit *out,
-                                LaneConfig cfg) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= live_count(d_count, ncands)) return;
+                                LaneConfig cfg, const int *order, const unsigned *class_count) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    /* With more than one model class in the batch, thread q finishes the q-th queued fit, so
+     * that a wave's lanes share a model (the classes' records differ several-fold in cost);
+     * otherwise candidate q (coalesced records).  Candidates outside the queue were skipped
+     * and keep their records. */
+    const unsigned c0 = class_count[0], c1 = class_count[1], c2 = class_count[2];
+    int i = q;
+    if ((c0 != 0) + (c1 != 0) + (c2 != 0) > 1) {
+        if (q >= (int)(c0 + c1 + c2)) return;
+        i = order[q];
+    } else if (q >= live_count(d_count, ncands)) {
+        return;
+    }
     finish_any(out[i], cfg, cands[i]);
 }
 
@@ -1005,8 +1016,9 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
         per_cu = std::min(per_cu, 4);
         const unsigned lanes = (unsigned)(per_cu * ctx->num_cus) * 64u; /* sets the group size in both launches */
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole, PixT>), dim3(per_cu * ctx->num_cus), dim3(64), glds, stream, a,
-                           cls, lanes);
+        /* The one-lane build goes first: when it applies (many long fits) its work starts as
+         * soon as CUs free up, instead of after the group build's workgroups, each holding a
+         * quarter of a CU's LDS, have been placed only to return. */
         if constexpr (P != 7) {
             int per_cu2 = 0;
             LZR_CHECK(
@@ -1015,6 +1027,8 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
             hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane, PixT>), dim3(per_cu2 * ctx->num_cus), dim3(64), lds,
                                stream, a, cls, lanes);
         }
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole, PixT>), dim3(per_cu * ctx->num_cus), dim3(64), glds, stream, a,
+                           cls, lanes);
     }
 }
 
@@ -1133,7 +1147,7 @@ static void launch(LzrContext *ctx, Workspace &ws, const void *frames, int nx, i
     else
         launch_classes<uint16_t>(ctx, ws, a, stream);
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
-                       ctx->cfg);
+                       ctx->cfg, (const int *)ws.order, (const unsigned *)counters);
     LZR_CHECK(hipGetLastError());
     if (order) LZR_CHECK(order->end(stream));
 }
