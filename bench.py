@@ -33,10 +33,10 @@ from localizer_amd import abi, device, native, netinit, synth  # noqa: E402
 METRIC = "localizations/sec on 1 MI355X: plain LM vs net-initialized LM at 1e-3 px"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per fitting call on the default workload (lm_persistent<5> two-wave build +
-# lm_straggler<5>), from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the reading of
-# a plain 512 MiB copy kernel (fetch x2.0, write x1.0).
-MEASURED_TRAFFIC_BYTES = {"plain-lm-256px-10000frames-9x9": 2320.2e6}
+# HBM bytes per fitting call on the default workload (lm_persistent<5> two-wave build,
+# lm_straggler<5> and the epilogue), from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by
+# the reading of a plain 512 MiB copy kernel (fetch x2.0, write x1.0).
+MEASURED_TRAFFIC_BYTES = {"plain-lm-256px-10000frames-9x9": 2191.3e6}
 # Algorithmic work per ROI (symmetric model): W = N[(1+T) F_f + (1+A)(F_J + F_NE)] + T F_solve
 F_F, F_J, F_NE, F_SOLVE = 12.0, 6.0, 40.0, 300.0
 F_F7, F_J7, F_NE7, F_SOLVE7 = 16.0, 30.0, 70.0, 700.0
@@ -588,6 +588,8 @@ def dense_leg(nframes, block_frames, latency_frames, log, pipelines=DENSE_PIPELI
     net = native.Net(w.initial_psf_width, seed=5)
     tr = netinit.train(net, synth.DENSE_ROIS_11, 1 << 20, epochs=1.0, batch=2048, lr=2e-3)
     log(f"dense: network trained on {tr['rois']} ROIs in {tr['train_s']:.2f} s")
+    sel = netinit.calibrate_selection(net, synth.DENSE_ROIS_11, nrois=1 << 20, log=log)
+    log(f"dense: network on candidates with intensity < {sel['threshold']:.4g}")
     cap = block_frames * 2048
     status_off = abi.FIT_DTYPE.fields["status"][1]
 
@@ -598,6 +600,7 @@ def dense_leg(nframes, block_frames, latency_frames, log, pipelines=DENSE_PIPELI
         def __init__(self):
             self.finder = native.Finder(w.initial_psf_width, DENSE_PFA)
             self.ctx = native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, 1.0, w.half_width[0])
+            self.ctx.set_option(abi.OPT_NET_MAX_INTENSITY, sel["threshold"])
             self.d_cands = device.empty(abi.CANDIDATE_DTYPE, cap)
             self.d_fits = device.empty(abi.FIT_DTYPE, cap)
             self.d_work = device.empty(abi.START_DTYPE, cap)
@@ -633,7 +636,8 @@ def dense_leg(nframes, block_frames, latency_frames, log, pipelines=DENSE_PIPELI
     pipes = [Pipe() for _ in range(pipelines)]
 
     def stream_all(netinit_fit):
-        pipes[0].run(0, min(block_frames, nframes), netinit_fit)  # warm-up
+        for p in pipes:  # warm-up: every pipeline's buffers and work memory
+            p.run(0, min(block_frames, nframes), netinit_fit)
         torch.cuda.synchronize()
         for p in pipes:
             p.acc.zero_()
@@ -666,7 +670,8 @@ def dense_leg(nframes, block_frames, latency_frames, log, pipelines=DENSE_PIPELI
         "pipelines": pipelines,
         "candidates_per_frame": cands / nframes,
         "netinit": {"frames_per_s": nframes / sec_net, "localizations_per_s": acc_net / sec_net,
-                    "accepted": acc_net, "seconds": sec_net},
+                    "accepted": acc_net, "seconds": sec_net,
+                    "network_below_intensity": sel["threshold"], "calibration_fraction": sel["fraction"]},
         "plain": {"frames_per_s": nframes / sec_plain, "localizations_per_s": acc_plain / sec_plain,
                   "accepted": acc_plain, "seconds": sec_plain},
         "latency_ms_per_frame": {"frames": int(lat.size), "p50": float(np.percentile(lat, 50)),

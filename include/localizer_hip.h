@@ -349,11 +349,14 @@ LZR_EXPORT int lzr_net_train_step_device(LzrNet *net, const uint16_t *d_frames, 
  *   - was rejected or did not converge (LZR_LM_SUCCESS), when refit_rejected != 0, or
  *   - was accepted with a residual |f| above residual_factor times the residual at the
  *     reference start point (the normalized residuals chi/sqrt(dof) share the dof).
- * A refitted candidate's record is the plain fit's record bit for bit.  d_work:
+ * A refitted candidate's record is the plain fit's record bit for bit.  The network starts
+ * only the candidates whose intensity is below LZR_OPT_NET_MAX_INTENSITY (default: all); the
+ * others start from the reference start and get the plain record.  d_work:
  * ncandidates LzrStart of scratch; afterwards its width is 0 for refitted candidates and
  * -1 for the others, and its amplitude holds |f| at the reference start for the accepted
- * net-start fits whose test read their pixels (NaN for the others: a fit whose model lies
- * within |f| (1 - 1/residual_factor) of the start point's model passes without them).  d_nrefit (may be null):
+ * net-start fits whose test evaluated it (NaN for the others: a fit whose window is not the
+ * network's 11 x 11 patch and whose model lies within |f| (1 - 1/residual_factor) of the
+ * start point's model passes without reading its pixels).  d_nrefit (may be null):
  * one device unsigned, set to the number of refitted candidates.  No reference counterpart (the reference has no network start);
  * the records it returns are those of lzr_fit_batch_device for the refitted candidates.
  * Returns 0 or -1. */

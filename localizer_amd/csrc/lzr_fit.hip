@@ -1740,6 +1740,20 @@ __global__ void refit_live_kernel(const unsigned *nrefit, int *live, int n) {
     if (threadIdx.x == 0) *live = *nrefit ? n : 0;
 }
 
+/* The work buffer of a net-init call in which the network starts no candidate: nothing
+ * refitted (width -1), no start residual (NaN). */
+__global__ void fill_unused_starts(LzrStart *starts, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    LzrStart st;
+    st.amplitude = __builtin_nan("");
+    st.width = -1.0;
+    st.x = 0.0;
+    st.y = 0.0;
+    st.background = 0.0;
+    starts[i] = st;
+}
+
 /* The candidates the network starts: those with intensity below max_intensity, listed in
  * sel (in no particular order: a candidate's prediction does not depend on its row), counted
  * in *nsel (zeroed before); every other candidate gets the start of width 0, i.e. the
@@ -1776,6 +1790,24 @@ LZR_EXPORT int lzr_fit_batch_device_netinit(LzrContext *ctx, LzrNet *net, const 
     if (!d_work && ncands > 0) return fail(ctx, "null work buffer");
     /* the context first (its start-residual buffer), then the network */
     std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->net_max_intensity == -__builtin_inf()) {
+        /* the network starts no candidate: the plain fit, with the work buffer and the refit
+         * count the full path leaves in that case */
+        if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+        if (ncands == 0) return 0;
+        if (!d_frames || !d_cands || !d_out) return fail(ctx, "null device pointer");
+        try {
+            LZR_CHECK(hipSetDevice(ctx->device));
+            hipStream_t s = (hipStream_t)stream;
+            launch(ctx, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_out, s);
+            hipLaunchKernelGGL(fill_unused_starts, dim3((ncands + 255) / 256), dim3(256), 0, s, d_work, ncands);
+            if (d_nrefit) hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, s, d_nrefit, 1);
+            LZR_CHECK(hipGetLastError());
+        } catch (const std::string &e) {
+            return fail(ctx, e);
+        }
+        return 0;
+    }
     double *patch_sse = nullptr;
     int *sel = nullptr;
     unsigned *nsel = nullptr;

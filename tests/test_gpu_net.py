@@ -309,6 +309,9 @@ def test_gpu_network_on_selected_candidates(trained):
     assert fs[sel].tobytes() == fa[sel].tobytes()
     assert fs[~sel].tobytes() == plain[~sel].tobytes()
     assert out["none"][0].tobytes() == plain.tobytes() and out["none"][2] == 0
+    wn = out["none"][1]  # the plain shortcut leaves the work buffer the full path leaves
+    assert np.all(wn["width"] == -1.0) and np.all(np.isnan(wn["amplitude"]))
+    assert np.array_equal(wn["width"][~sel], ws["width"][~sel])
     assert out["sel2"][0].tobytes() == fs.tobytes()  # repeatable
     assert np.all(np.isnan(ws["amplitude"][~sel]))
     both = sel & ~np.isnan(wa["amplitude"])
