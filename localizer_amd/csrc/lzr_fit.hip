@@ -197,7 +197,8 @@ __global__ void __launch_bounds__(kSortThreads) scatter_kernel(const LzrCandidat
  * covariance from a QR factor of the reference Jacobian at the final point; 2x2
  * eigen-decomposition), one thread per candidate. */
 __global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, const int *d_count, LzrFit *out,
-                                LaneConfig cfg, const int *order, const unsigned *class_count) {
+                                LaneConfig cfg, const int *order, const unsigned *class_count, int *qr_list,
+                                unsigned qr_cap, unsigned *qr_count) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     /* With more than one model class in the batch, thread q finishes the q-th queued fit, so
      * that a wave's lanes share a model (the classes' records differ several-fold in cost);
@@ -211,7 +212,33 @@ __global__ void epilogue_kernel(const LzrCandidate *cands, int ncands, const int
     } else if (q >= live_count(d_count, ncands)) {
         return;
     }
-    finish_any(out[i], cfg, cands[i]);
+    /* A record whose covariance needs the QR factor (a few ill-conditioned fits, whose
+     * per-pixel factorisation would hold their whole wave) is listed for qr_finish_kernel,
+     * which runs them 64 to a wave; past the list's capacity it is finished here. */
+    finish_any(out[i], cfg, cands[i], qr_cap > 0);
+    const bool deferred = out[i].status == kDeferredQr;
+    const unsigned long long m = __ballot(deferred);
+    if (!m) return;
+    const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(qr_count, (unsigned)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!deferred) return;
+    const unsigned slot = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    if (slot < qr_cap)
+        qr_list[slot] = i;
+    else
+        finish_any_deferred(out[i], cfg, cands[i]);
+}
+
+/* The records the epilogue listed: covariance by QR, then the remaining fields. */
+__global__ void qr_finish_kernel(const LzrCandidate *cands, LzrFit *out, LaneConfig cfg, const int *qr_list,
+                                 unsigned qr_cap, const unsigned *qr_count) {
+    const unsigned n = min(*qr_count, qr_cap);
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int i = qr_list[k];
+        finish_any_deferred(out[i], cfg, cands[i]);
+    }
 }
 
 __global__ void reset_counters(unsigned *counters, int n) {
@@ -872,11 +899,14 @@ struct Workspace {
                                      hand-offs, then (count, head) x 2 buffers per class for parking */
     uint8_t *strag = nullptr;
     size_t strag_entries = 0;     /* hand-off entries of kParkedMax bytes */
+    int *qr_list = nullptr;       /* records whose covariance waits for the QR factor */
+    size_t qr_cap = 0;
     void *park[kClasses][2] = {};
     size_t park_cap[kClasses][2] = {};
     bool owned = true;
 };
-constexpr int kWorkspaceCounters = kCounterWords + 2 * kClasses + 4 * kClasses;
+constexpr int kQrCounter = kCounterWords + 2 * kClasses + 4 * kClasses; /* entries of the QR list */
+constexpr int kWorkspaceCounters = kQrCounter + 1;
 
 struct LzrContext {
     int device = 0;
@@ -1059,6 +1089,7 @@ static void reserve(LzrContext *ctx, Workspace &ws, int ncands) {
     const size_t n = (size_t)std::max(ncands, 1);
     if (ws.owned) {
         ensure(ws.order, ws.order_cap, n);
+        ensure(ws.qr_list, ws.qr_cap, n);
         if (!ws.counters) LZR_CHECK(hipMalloc((void **)&ws.counters, kWorkspaceCounters * sizeof(unsigned)));
         if (ctx->strag_trials > 0) {
             size_t bytes = ws.strag_entries * kParkedMax;
@@ -1080,6 +1111,7 @@ static void reserve(LzrContext *ctx, Workspace &ws, int ncands) {
 static void release(Workspace &ws) {
     if (!ws.owned) return;
     (void)hipFree(ws.order);
+    (void)hipFree(ws.qr_list);
     (void)hipFree(ws.counters);
     (void)hipFree(ws.strag);
     for (int c = 0; c < kClasses; ++c)
@@ -1146,8 +1178,15 @@ static void launch(LzrContext *ctx, Workspace &ws, const void *frames, int nx, i
         launch_classes<float>(ctx, ws, a, stream);
     else
         launch_classes<uint16_t>(ctx, ws, a, stream);
+    const unsigned qr_cap = (unsigned)std::min(ws.qr_cap, (size_t)ncands);
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
-                       ctx->cfg, (const int *)ws.order, (const unsigned *)counters);
+                       ctx->cfg, (const int *)ws.order, (const unsigned *)counters, ws.qr_list, qr_cap,
+                       counters + kQrCounter);
+    if (qr_cap > 0) {
+        const int qblocks = std::max(1, std::min((ncands + 63) / 64, ctx->num_cus * 4));
+        hipLaunchKernelGGL(qr_finish_kernel, dim3(qblocks), dim3(64), 0, stream, cands, out, ctx->cfg,
+                           (const int *)ws.qr_list, qr_cap, (const unsigned *)(counters + kQrCounter));
+    }
     LZR_CHECK(hipGetLastError());
     if (order) LZR_CHECK(order->end(stream));
 }

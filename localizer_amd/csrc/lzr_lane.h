@@ -94,6 +94,9 @@ struct LzrRawFit {
 static_assert(sizeof(LzrRawFit) == sizeof(LzrFit), "raw record must overlay LzrFit");
 static_assert(__builtin_offsetof(LzrRawFit, status) == __builtin_offsetof(LzrFit, status), "status overlays status");
 constexpr int32_t kPending = -1;
+/* A record whose fit passed the post-checks and whose covariance waits for the QR factor
+ * (finish_deferred): its raw parameters are kept in the deviations' slots. */
+constexpr int32_t kDeferredQr = -2;
 
 /* Per-lane LM state.  x0/y0 are kept window-local (exact integer offsets xoff/yoff);
  * every quantity the reference computes from absolute coordinates (test_delta,
@@ -1774,18 +1777,98 @@ LZR_HD LZR_INLINE void covariance_qr(const double (&t)[P], int n, double is, dou
  * by the QR factor of J as GSL computes it. */
 constexpr double kCovKappaMax = 1.0e6;
 template <int P>
-LZR_HD LZR_INLINE void covariance(const double (&t)[P], int n, double is, double w, double (&cdiag)[P]) {
+LZR_HD LZR_INLINE bool covariance(const double (&t)[P], int n, double is, double w, double (&cdiag)[P],
+                                  bool defer_qr = false) {
     double H[NP<P>::S], kappa;
     moment_normal_matrix<P>(t, n, is, w, H);
-    if (covariance_cholesky<P>(H, cdiag, kappa) && kappa <= kCovKappaMax) return;
+    if (covariance_cholesky<P>(H, cdiag, kappa) && kappa <= kCovKappaMax) return true;
+    if (defer_qr) return false; /* the caller finishes the record with the QR factor later */
     covariance_qr<P>(t, n, is, w, cdiag);
+    return true;
+}
+
+/* The fields of an accepted record that follow from the covariance diagonal cd: deviations,
+ * integral and its deviation, widths and their deviations, the symmetric-PSF filter
+ * (source/Localization.cpp:185-208 symmetric, :371-372 fixed width, :555-596 ellipsoidal).
+ * x holds the absolute parameters (out.params); sd1, sd2, theta (ellipsoidal) come from
+ * eigen2 of x. */
+template <int P>
+LZR_HD LZR_INLINE void finish_accepted(LzrFit &out, const double (&x)[P], double chi, int h, double w, int model,
+                                       const double (&cd)[P], double sd1, double sd2, double theta) {
+    LZR_NO_CONTRACT
+    if constexpr (P == 4) {
+        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 4);
+        const double q = chi / dsqrt(dof);
+        const double cs = 1.0 > q ? 1.0 : q;
+#pragma unroll
+        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
+        out.integral = 2 * kPi * w * w * x[0];
+        /* the reference scales the amplitude's relative error by 2 pi I w^2 (:371-372) */
+        const double relA = cs * dsqrt(cd[0]) / x[0];
+        out.integral_deviation = 2 * kPi * out.integral * w * w * dsqrt(relA * relA);
+        out.sd1 = out.sd2 = w;
+        out.status = LZR_ACCEPTED;
+    } else if constexpr (P == 5) {
+        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 5);
+        const double q = chi / dsqrt(dof);
+        const double cs = 1.0 > q ? 1.0 : q;
+#pragma unroll
+        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
+        const double width = x[1];
+        out.integral = 2 * kPi * width * width * x[0];
+        const double relA = cs * dsqrt(cd[0]) / x[0];
+        const double relW = out.deviations[1] / width;
+        out.integral_deviation = 2 * kPi * out.integral * dsqrt(relA * relA + 2 * relW * relW);
+        out.sd1 = out.sd2 = width;
+        out.sd1_deviation = out.sd2_deviation = out.deviations[1];
+        out.theta = 0.0;
+        out.status = LZR_ACCEPTED;
+    } else {
+        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
+        const double q = chi / dsqrt(dof);
+        const double cs = 1.0 > q ? 1.0 : q;
+#pragma unroll
+        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
+        const double rho = x[5];
+        const double corr_dev = out.deviations[5];
+        const double sxd = out.deviations[1], syd = out.deviations[2];
+        out.sd1 = sd1;
+        out.sd2 = sd2;
+        out.theta = theta;
+        out.sd1_deviation = sxd * dcos(theta) + syd * dsin(theta);
+        out.sd2_deviation = sxd * dsin(theta) + syd * dcos(theta);
+        const double omr = 1 - rho * rho;
+        out.integral = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2;
+        const double relA = out.deviations[0] / x[0];
+        out.integral_deviation = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2 *
+                                 dsqrt(relA * relA + rho * rho * corr_dev * corr_dev / omr / omr +
+                                       out.sd1_deviation * out.sd1_deviation / sd1 / sd1 +
+                                       out.sd2_deviation * out.sd2_deviation / sd2 / sd2);
+        out.status = LZR_ACCEPTED;
+        if (model == LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF) {
+            if ((0.75 * sd1 > sd2) || (1.25 * sd1 < sd2))
+                out.status = LZR_REJECT_ANISOTROPIC;
+            else if (((sd1 < w * 0.7) || (sd1 > w * 1.3)) && ((sd2 < w * 0.7) || (sd2 > w * 1.3)))
+                out.status = LZR_REJECT_PSF_WIDTH;
+        }
+    }
+}
+
+/* Marks a record whose covariance needs the QR factor: the raw parameters t go to the
+ * deviations' slots (finish_deferred reads them there). */
+template <int P>
+LZR_HD LZR_INLINE void defer_record(LzrFit &out, const double (&t)[P]) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) out.deviations[i] = t[i];
+    out.status = kDeferredQr;
 }
 
 /* Post-checks and output record of the reference drivers
  * (source/Localization.cpp:157-211 symmetric, :336-378 fixed width, :506-596 ellipsoidal,
  * :606-631 filter), from the raw end state r of candidate c. */
 template <int P>
-LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, LzrFit &out) {
+LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, const LzrCandidate &c, LzrFit &out,
+                                     bool defer_qr = false) {
     LZR_NO_CONTRACT
     const int model = r.model;
     const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
@@ -1824,18 +1907,11 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance<P>(t, n, cfg.inv_sigma, w, cd);
-        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 4);
-        const double q = chi / dsqrt(dof);
-        const double cs = 1.0 > q ? 1.0 : q;
-#pragma unroll
-        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
-        out.integral = 2 * kPi * w * w * x[0];
-        /* the reference scales the amplitude's relative error by 2 pi I w^2 (:371-372) */
-        const double relA = cs * dsqrt(cd[0]) / x[0];
-        out.integral_deviation = 2 * kPi * out.integral * w * w * dsqrt(relA * relA);
-        out.sd1 = out.sd2 = w;
-        out.status = LZR_ACCEPTED;
+        if (!covariance<P>(t, n, cfg.inv_sigma, w, cd, defer_qr)) {
+            defer_record<P>(out, t);
+            return;
+        }
+        finish_accepted<P>(out, x, chi, h, w, model, cd, 0.0, 0.0, 0.0);
         return;
     } else if constexpr (P == 5) {
         x[1] = dabs(x[1]);
@@ -1854,21 +1930,11 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance<P>(t, n, cfg.inv_sigma, w, cd);
-        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 5);
-        const double q = chi / dsqrt(dof);
-        const double cs = 1.0 > q ? 1.0 : q;
-#pragma unroll
-        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
-        const double width = x[1];
-        out.integral = 2 * kPi * width * width * x[0];
-        const double relA = cs * dsqrt(cd[0]) / x[0];
-        const double relW = out.deviations[1] / width;
-        out.integral_deviation = 2 * kPi * out.integral * dsqrt(relA * relA + 2 * relW * relW);
-        out.sd1 = out.sd2 = width;
-        out.sd1_deviation = out.sd2_deviation = out.deviations[1];
-        out.theta = 0.0;
-        out.status = LZR_ACCEPTED;
+        if (!covariance<P>(t, n, cfg.inv_sigma, w, cd, defer_qr)) {
+            defer_record<P>(out, t);
+            return;
+        }
+        finish_accepted<P>(out, x, chi, h, w, model, cd, 0.0, 0.0, 0.0);
         return;
     } else {
         x[1] = dabs(x[1]);
@@ -1893,35 +1959,33 @@ LZR_HD LZR_INLINE void finish_record(const LzrRawFit &r, const LaneConfig &cfg, 
             return;
         }
         double cd[P];
-        covariance<P>(t, n, cfg.inv_sigma, w, cd);
-        const double dof = (double)((2 * h - 1) * (2 * h - 1) - 7);
-        const double q = chi / dsqrt(dof);
-        const double cs = 1.0 > q ? 1.0 : q;
-#pragma unroll
-        for (int i = 0; i < P; ++i) out.deviations[i] = cs * dsqrt(cd[i]);
-        const double rho = x[5];
-        const double corr_dev = out.deviations[5];
-        const double sxd = out.deviations[1], syd = out.deviations[2];
-        out.sd1 = sd1;
-        out.sd2 = sd2;
-        out.theta = theta;
-        out.sd1_deviation = sxd * dcos(theta) + syd * dsin(theta);
-        out.sd2_deviation = sxd * dsin(theta) + syd * dcos(theta);
-        const double omr = 1 - rho * rho;
-        out.integral = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2;
-        const double relA = out.deviations[0] / x[0];
-        out.integral_deviation = 2.0 * kPi * x[0] * dsqrt(omr) * sd1 * sd2 *
-                                 dsqrt(relA * relA + rho * rho * corr_dev * corr_dev / omr / omr +
-                                       out.sd1_deviation * out.sd1_deviation / sd1 / sd1 +
-                                       out.sd2_deviation * out.sd2_deviation / sd2 / sd2);
-        out.status = LZR_ACCEPTED;
-        if (model == LZR_MODEL_ELLIPSOIDAL_SYMMETRIC_PSF) {
-            if ((0.75 * sd1 > sd2) || (1.25 * sd1 < sd2))
-                out.status = LZR_REJECT_ANISOTROPIC;
-            else if (((sd1 < w * 0.7) || (sd1 > w * 1.3)) && ((sd2 < w * 0.7) || (sd2 > w * 1.3)))
-                out.status = LZR_REJECT_PSF_WIDTH;
+        if (!covariance<P>(t, n, cfg.inv_sigma, w, cd, defer_qr)) {
+            defer_record<P>(out, t);
+            return;
         }
+        finish_accepted<P>(out, x, chi, h, w, model, cd, sd1, sd2, theta);
     }
+}
+
+/* Completes a record finish_record deferred (status kDeferredQr): the covariance from the QR
+ * factor of the reference Jacobian at the raw parameters kept in the deviations' slots, then
+ * the accepted record's remaining fields, exactly as finish_record computes them inline. */
+template <int P>
+LZR_HD LZR_INLINE void finish_deferred(LzrFit &out, const LaneConfig &cfg, const LzrCandidate &c) {
+    LZR_NO_CONTRACT
+    const int h = c.half_width > 0 ? c.half_width : cfg.default_half_width;
+    const int n = 2 * h + 1;
+    const double w = cfg.initial_psf_width;
+    double t[P], x[P], cd[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        t[i] = out.deviations[i];
+        x[i] = out.params[i];
+    }
+    covariance_qr<P>(t, n, cfg.inv_sigma, w, cd);
+    double sd1 = 0.0, sd2 = 0.0, theta = 0.0;
+    if constexpr (P == 7) eigen2(x[1], x[2], x[5], sd1, sd2, theta);
+    finish_accepted<P>(out, x, out.chi, h, w, out.model, cd, sd1, sd2, theta);
 }
 
 /* Separable sums of start_residual over an n x n window (n <= NMAX): the loops are unrolled
@@ -1995,16 +2059,26 @@ LZR_HD LZR_INLINE double start_residual(const LaneConfig &cfg, const LzrCandidat
 
 /* Epilogue entry: finishes a record written by store_final (no-op for records already
  * final, i.e. rejected before fitting). */
-LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c) {
+LZR_HD LZR_INLINE void finish_any(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c, bool defer_qr = false) {
     LzrRawFit r;
     __builtin_memcpy(&r, &rec, sizeof r);
     if (r.status != kPending) return;
     if (r.model == LZR_MODEL_SYMMETRIC)
-        finish_record<5>(r, cfg, c, rec);
+        finish_record<5>(r, cfg, c, rec, defer_qr);
     else if (r.model == LZR_MODEL_FIXED_WIDTH)
-        finish_record<4>(r, cfg, c, rec);
+        finish_record<4>(r, cfg, c, rec, defer_qr);
     else
-        finish_record<7>(r, cfg, c, rec);
+        finish_record<7>(r, cfg, c, rec, defer_qr);
+}
+
+/* Completes a deferred record (status kDeferredQr), whatever its model. */
+LZR_HD LZR_INLINE void finish_any_deferred(LzrFit &rec, const LaneConfig &cfg, const LzrCandidate &c) {
+    if (rec.model == LZR_MODEL_SYMMETRIC)
+        finish_deferred<5>(rec, cfg, c);
+    else if (rec.model == LZR_MODEL_FIXED_WIDTH)
+        finish_deferred<4>(rec, cfg, c);
+    else
+        finish_deferred<7>(rec, cfg, c);
 }
 
 }  // namespace lzr
