@@ -174,17 +174,13 @@ __device__ inline uint32_t half_max16(uint32_t v) {
     return v;
 }
 
-__global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int nx, int ny, int nframes,
-                                                   const LzrCandidate *cands, int ncands, int nrows,
-                                                   uint16_t *xin, PatchStat *stats, double k_start,
-                                                   double *start_sse, const int *sel_idx,
-                                                   const unsigned *sel_count) {
+/* Rows 8 rb .. 8 rb + 7 (two per wave of the block). */
+__device__ __forceinline__ void prep_rows(const uint16_t *frames, int nx, int ny, int nframes,
+                                          const LzrCandidate *cands, int ncands, int nrows, uint16_t *xin,
+                                          PatchStat *stats, double k_start, double *start_sse, const int *sel_idx,
+                                          int rb) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
-    const int r = (blockIdx.x * 4 + wave) * 2 + half;
-    if (sel_count) { /* rows [0, *sel_count) are the candidates sel_idx names */
-        nrows = (int)*sel_count;
-        ncands = nrows;
-    }
+    const int r = (rb * 4 + wave) * 2 + half;
     const bool live = r < nrows;
     if (__ballot(live) == 0) return; /* wave-uniform */
     bool valid = live && r < ncands;
@@ -284,6 +280,23 @@ __global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int n
         st.valid = 1;
         stats[r] = st;
     }
+}
+
+/* One block per 8 rows; with a device row count (sel_count: rows [0, *sel_count) are the
+ * candidates sel_idx names) the launch is sized for the most rows and its blocks stride
+ * over the rows the count leaves. */
+__global__ void __launch_bounds__(256) prep_kernel(const uint16_t *frames, int nx, int ny, int nframes,
+                                                   const LzrCandidate *cands, int ncands, int nrows,
+                                                   uint16_t *xin, PatchStat *stats, double k_start,
+                                                   double *start_sse, const int *sel_idx,
+                                                   const unsigned *sel_count) {
+    if (!sel_count) {
+        prep_rows(frames, nx, ny, nframes, cands, ncands, nrows, xin, stats, k_start, start_sse, nullptr, blockIdx.x);
+        return;
+    }
+    const int rows = (int)*sel_count;
+    for (int rb = blockIdx.x; rb * 8 < rows; rb += gridDim.x)
+        prep_rows(frames, nx, ny, nframes, cands, rows, rows, xin, stats, k_start, start_sse, sel_idx, rb);
 }
 
 /* ---- fused forward ---------------------------------------------------------------- */
@@ -1082,8 +1095,10 @@ static void run_forward(LzrNet *net, const uint16_t *frames, int nx, int ny, int
                         int ncands, int nrows, hipStream_t s, double k_start = 0.0, double *start_sse = nullptr,
                         const int *sel_idx = nullptr, const unsigned *sel_count = nullptr) {
     ensure_rows(net, (size_t)nrows);
-    hipLaunchKernelGGL(prep_kernel, dim3((nrows + 7) / 8), dim3(256), 0, s, frames, nx, ny, nframes, cands, ncands,
-                       nrows, net->d_xin, net->d_stats, k_start, start_sse, sel_idx, sel_count);
+    const int pblocks = (nrows + 7) / 8;
+    hipLaunchKernelGGL(prep_kernel, dim3(sel_count ? std::min(pblocks, net->num_cus * 32) : pblocks), dim3(256), 0, s,
+                       frames, nx, ny, nframes, cands, ncands, nrows, net->d_xin, net->d_stats, k_start, start_sse,
+                       sel_idx, sel_count);
     TrainStore T{};
     if (kTrain) {
         T.xT = net->d_xT;
