@@ -55,6 +55,7 @@ struct KernelArgs {
     const int *order;      /* queue position -> candidate index   */
     const unsigned *count; /* number of queue entries (device)    */
     unsigned *head;        /* queue head (device, zeroed per call) */
+    unsigned *seg_head;    /* heads of the queue's segments, kSegStride words apart, per class */
     LzrFit *out;
     LaneConfig cfg;
     /* long fits: a fit still running after strag_trials trials is handed off to the
@@ -68,6 +69,13 @@ struct KernelArgs {
     unsigned strag_entries; /* capacity of strag[] in entries; a class's region is its share */
     int strag_lk;           /* log2 of the lanes per handed-off fit, -1: by the hand-off count */
 };
+
+/* The first fitting kernel's queue is read through kQueueSegs interleaved segments (segment s:
+ * entries s, s + kQueueSegs, ...), each with its own head in its own 128-byte line: a wave
+ * refills from the segment of its workgroup's XCD (workgroups are dealt to the eight XCDs in
+ * turn) and moves on when that one is drained, so the refill atomics spread over eight
+ * addresses instead of queueing on one. */
+constexpr int kQueueSegs = 8, kSegStride = 32;
 
 /* What a pixel accessor returns for a frame element type: camera counts as unsigned, other
  * (non-integer) frames as float; the passes convert either to FP64. */
@@ -311,6 +319,8 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     const unsigned long long below = (1ull << lane) - 1ull;
     const unsigned count = r.resume ? *r.in_count : args.count[cls];
     unsigned *head = r.resume ? r.in_head : args.head + cls;
+    unsigned *seg_head = args.seg_head + cls * kQueueSegs * kSegStride;
+    int seg = blockIdx.x % kQueueSegs, drained = 0; /* fresh queue: this wave's segment, segments found drained */
     unsigned first = 0; /* queue entries of the classes before this one */
     for (int c = 0; c < cls; ++c) first += args.count[c];
     const int *order = args.order + first;
@@ -339,20 +349,21 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             const int cnt = __popcll(want);
             const int leader = __ffsll((long long)want) - 1;
             unsigned base = 0;
-            if (lane == leader) base = atomicAdd(head, (unsigned)cnt);
-            base = __shfl(base, leader, 64);
-            if (base + (unsigned)cnt >= count) open = false;
-            if (base >= count) break;
-            if (s.phase == kIdle) {
-                const unsigned q = base + (unsigned)__popcll(want & below);
-                if (q < count) {
-                    if (r.resume) {
-                        const Parked<P> &pk = pin[q];
-                        s = pk.s;
-                        model = pk.model;
-                        fresh = true;
-                    } else {
-                        const int idx = order[q];
+            if (!r.resume) {
+                /* segment seg holds queue entries seg + kQueueSegs k, k < size */
+                const unsigned size = count > (unsigned)seg ? (count - (unsigned)seg + kQueueSegs - 1) / kQueueSegs : 0u;
+                if (lane == leader) base = atomicAdd(seg_head + seg * kSegStride, (unsigned)cnt);
+                base = __shfl(base, leader, 64);
+                const int from = seg;
+                if (base + (unsigned)cnt >= size) { /* drained: the next refill reads the next segment */
+                    seg = seg + 1 == kQueueSegs ? 0 : seg + 1;
+                    if (++drained == kQueueSegs) open = false;
+                }
+                if (base >= size) continue;
+                if (s.phase == kIdle) {
+                    const unsigned k = base + (unsigned)__popcll(want & below);
+                    if (k < size) {
+                        const int idx = order[from + kQueueSegs * k];
                         const LzrCandidate c = args.cands[idx];
                         model = c.model >= 0 ? c.model : args.cfg.model;
                         LzrStart st;
@@ -361,10 +372,25 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
                                           args.starts ? &st : nullptr)) {
                             fresh = true;
                             stay = false;
-                        }
-                        else
+                        } else {
                             s.phase = kIdle;
+                        }
                     }
+                }
+                continue;
+            }
+            /* a later round: the fits the previous round parked, from one list */
+            if (lane == leader) base = atomicAdd(head, (unsigned)cnt);
+            base = __shfl(base, leader, 64);
+            if (base + (unsigned)cnt >= count) open = false;
+            if (base >= count) break;
+            if (s.phase == kIdle) {
+                const unsigned q = base + (unsigned)__popcll(want & below);
+                if (q < count) {
+                    const Parked<P> &pk = pin[q];
+                    s = pk.s;
+                    model = pk.model;
+                    fresh = true;
                 } else {
                     s.phase = kFinished;
                 }
@@ -906,7 +932,8 @@ struct Workspace {
     bool owned = true;
 };
 constexpr int kQrCounter = kCounterWords + 2 * kClasses + 4 * kClasses; /* entries of the QR list */
-constexpr int kWorkspaceCounters = kQrCounter + 1;
+constexpr int kSegHeads = (kQrCounter + 1 + kSegStride - 1) / kSegStride * kSegStride;
+constexpr int kWorkspaceCounters = kSegHeads + kClasses * kQueueSegs * kSegStride;
 
 struct LzrContext {
     int device = 0;
@@ -1167,6 +1194,7 @@ static void launch(LzrContext *ctx, Workspace &ws, const void *frames, int nx, i
     a.order = ws.order;
     a.count = counters;
     a.head = counters + kClasses;
+    a.seg_head = counters + kSegHeads;
     a.out = out;
     a.cfg = ctx->cfg;
     a.strag_trials = ws.strag_entries > 0 ? ctx->strag_trials : 0;
