@@ -343,8 +343,9 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             const unsigned long long want = __ballot(s.phase == kIdle);
             /* A refill costs a queue atomic, the candidates' loads and a window gather,
              * mostly latency: wait until kRefillMin lanes are idle (a lane's fit does not
-             * depend on when it starts).  500k 9x9 fits: 3.86 -> 3.73 ms. */
-            constexpr int kRefillMin = 8;
+             * depend on when it starts).  500k 9x9 fits, same box: 4.08 ms at 4 or 8 idle
+             * lanes, 4.02 at 12, 4.05 at 16. */
+            constexpr int kRefillMin = 12;
             if (!want || __popcll(want) < kRefillMin) break;
             const int cnt = __popcll(want);
             const int leader = __ffsll((long long)want) - 1;
