@@ -35,6 +35,12 @@
 
 #include "../../include/localizer_hip.h"
 
+/* lzr_fit_batch_device launching only the model classes of h_cands, the host copy of the
+ * candidates (lzr_fit.hip; internal to the library). */
+int lzr_fit_batch_device_hostcands(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                   const LzrCandidate *d_cands, const LzrCandidate *h_cands, int ncands, LzrFit *d_out,
+                                   void *stream);
+
 namespace lzr_batch {
 
 /* One packed window: its frame slot, offsets, side and first pixel in the pack. */
@@ -190,7 +196,8 @@ void run_batch(LzrBatcher *B, Slot &slot, Buf &b) {
         ok = hip_ok(hipMemcpyAsync(b.d_cand, b.h_cand, (size_t)b.nrois * sizeof(LzrCandidate), hipMemcpyHostToDevice,
                                    s),
                     err, "upload candidates");
-    if (ok && lzr_fit_batch_device(slot.ctx, b.d_frames, B->nx, B->ny, b.nframes, b.d_cand, b.nrois, b.d_out, s) != 0) {
+    if (ok && lzr_fit_batch_device_hostcands(slot.ctx, b.d_frames, B->nx, B->ny, b.nframes, b.d_cand, b.h_cand, b.nrois,
+                                             b.d_out, s) != 0) {
         char buf[512];
         lzr_last_error(slot.ctx, buf, (int)sizeof buf);
         err = std::string("fit: ") + buf;

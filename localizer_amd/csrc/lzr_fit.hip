@@ -945,6 +945,7 @@ struct LzrContext {
     int rounds = 1;          /* launches per class; fits are parked between them */
     int park_threshold = 16; /* park a wave's fits when this few lanes are still busy */
     int strag_trials = 48;   /* hand a fit to the straggler kernel after this many trials */
+    bool strag_trials_set = false; /* strag_trials chosen by the caller or the autotuner (else by call size) */
     int strag_group = 0;     /* lanes per handed-off fit (1, 2, 4, 8, 16), 0: by the hand-off count */
     double net_max_intensity = __builtin_inf(); /* net-init: the network starts candidates below this intensity */
     Workspace ws;            /* the eager calls' work memory */
@@ -1016,7 +1017,8 @@ static void ensure(T *&ptr, size_t &cap, size_t need) {
 
 struct Workspace;
 template <int P, typename Acc, int W, typename PixT>
-static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream);
+static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream,
+                      int ncap);
 
 template <int P, typename Acc, int W, typename PixT = uint16_t>
 static int grid_for(LzrContext *ctx, size_t lds) {
@@ -1043,9 +1045,14 @@ static void reserve_park(LzrContext *ctx, Workspace &ws, int cls, size_t lds) {
     }
 }
 
+/* ncap bounds the call's candidates (its count, or a graph's capacity): no launch needs more
+ * waves than fits to hold, so a small call dispatches only those (a wave visits every queue
+ * segment before it closes, and group sizes still follow the full resident capacity). */
 template <int P, typename Acc, int W, typename PixT>
-static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream) {
-    const int grid = grid_for<P, Acc, W, PixT>(ctx, lds);
+static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int cls, size_t lds, hipStream_t stream,
+                      int ncap) {
+    const int waves1 = std::max(1, (ncap + 63) / 64); /* one lane per fit */
+    const int grid = std::min(grid_for<P, Acc, W, PixT>(ctx, lds), waves1);
     reserve_park<P, Acc, W>(ctx, ws, cls, lds);
     unsigned *ctr = ws.counters + kCounterWords + 2 * kClasses + 4 * cls; /* buffer b: count ctr[2b], head ctr[2b+1] */
     for (int round = 0; round < ctx->rounds; ++round) {
@@ -1082,11 +1089,12 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
             LZR_CHECK(
                 hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, lm_straggler<P, Acc, 2, kOneLane, PixT>, 64, lds));
             if (per_cu2 < 1) throw std::string("straggler kernel cannot be resident with this window size");
-            hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane, PixT>), dim3(per_cu2 * ctx->num_cus), dim3(64), lds,
-                               stream, a, cls, lanes);
+            hipLaunchKernelGGL((lm_straggler<P, Acc, 2, kOneLane, PixT>), dim3(std::min(per_cu2 * ctx->num_cus, waves1)),
+                               dim3(64), lds, stream, a, cls, lanes);
         }
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole, PixT>), dim3(per_cu * ctx->num_cus), dim3(64), glds, stream, a,
-                           cls, lanes);
+        const int waves_g = std::max(1, (ncap + (64 / kMaxGroup) - 1) / (64 / kMaxGroup)); /* kMaxGroup lanes per fit */
+        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole, PixT>), dim3(std::min(per_cu * ctx->num_cus, waves_g)), dim3(64),
+                           glds, stream, a, cls, lanes);
     }
 }
 
@@ -1151,29 +1159,58 @@ static void release(Workspace &ws) {
  * on the device, at most ncands) on workspace ws.  Allocates nothing once reserve() covered
  * ncands; order (eager calls) waits for the previous call on ws when the stream differs. */
 template <typename PixT>
-static void launch_classes(LzrContext *ctx, Workspace &ws, const KernelArgs &a, hipStream_t stream) {
+static void launch_classes(LzrContext *ctx, Workspace &ws, const KernelArgs &a, hipStream_t stream, int ncap,
+                           unsigned class_mask) {
     const size_t lds = launch_lds(ctx, sizeof(PixT));
     /* The classes share nothing but the sorted queue: classes 1 and 2 run on the
      * context's side streams beside class 0, so one class's straggler tail overlaps
-     * another class's main kernel (fork/join by events; also inside a graph capture). */
-    LZR_CHECK(hipEventRecord(ctx->fork, stream));
-    for (int c = 0; c < kClasses - 1; ++c) LZR_CHECK(hipStreamWaitEvent(ctx->side[c], ctx->fork, 0));
-    run_class<7, AccT, 1, PixT>(ctx, ws, a, 1, lds, ctx->side[0]);
-    run_class<4, AccT, 2, PixT>(ctx, ws, a, 2, lds, ctx->side[1]);
-    if (occupancy5(ctx, sizeof(PixT)) == 1)
-        run_class<5, AccT, 1, PixT>(ctx, ws, a, 0, lds, stream);
-    else
-        run_class<5, AccT, 2, PixT>(ctx, ws, a, 0, lds, stream);
+     * another class's main kernel (fork/join by events; also inside a graph capture).
+     * Classes outside class_mask hold no candidate of this call and are not launched. */
+    const bool side[kClasses - 1] = {(class_mask & 2u) != 0, (class_mask & 4u) != 0};
+    if (side[0] || side[1]) LZR_CHECK(hipEventRecord(ctx->fork, stream));
+    for (int c = 0; c < kClasses - 1; ++c)
+        if (side[c]) LZR_CHECK(hipStreamWaitEvent(ctx->side[c], ctx->fork, 0));
+    if (side[0]) run_class<7, AccT, 1, PixT>(ctx, ws, a, 1, lds, ctx->side[0], ncap);
+    if (side[1]) run_class<4, AccT, 2, PixT>(ctx, ws, a, 2, lds, ctx->side[1], ncap);
+    if (class_mask & 1u) {
+        if (occupancy5(ctx, sizeof(PixT)) == 1)
+            run_class<5, AccT, 1, PixT>(ctx, ws, a, 0, lds, stream, ncap);
+        else
+            run_class<5, AccT, 2, PixT>(ctx, ws, a, 0, lds, stream, ncap);
+    }
     for (int c = 0; c < kClasses - 1; ++c) {
+        if (!side[c]) continue;
         LZR_CHECK(hipEventRecord(ctx->join[c], ctx->side[c]));
         LZR_CHECK(hipStreamWaitEvent(stream, ctx->join[c], 0));
     }
 }
 
+/* The model classes of host-side candidates (candidates with model < 0 take the context's). */
+static unsigned host_class_mask(const LzrContext *ctx, const LzrCandidate *cands, int ncands) {
+    unsigned mask = 0;
+    for (int i = 0; i < ncands && mask != 7u; ++i)
+        mask |= 1u << model_class(cands[i].model >= 0 ? cands[i].model : ctx->cfg.model);
+    return mask;
+}
+
+/* Trials after which a fit moves to the straggler kernel.  A call of at most kSmallCall
+ * candidates is latency work (the per-frame call pattern): its slowest fit sets the call,
+ * and a fit runs its trials faster in a straggler group than among a wave of 64 fits, so
+ * every fit moves after its first trial.  9x9 frames of 50 ROIs, same box, threshold 48 ->
+ * 1: 1 frame 0.52 -> 0.36 ms, 16 frames 2.16 -> 1.94, 256 frames 2.53 -> 2.19, 640 frames
+ * 2.59 -> 2.41, but 2000 frames 2.72 -> 3.68; records identical.  A threshold set by the
+ * caller (LZR_OPT_STRAGGLER_TRIALS) or the autotuner holds for every size. */
+constexpr int kSmallCall = 32768;
+static int hand_off_trials(const LzrContext *ctx, int ncands) {
+    if (!ctx->strag_trials_set && ctx->strag_trials > 0 && ncands <= kSmallCall) return 1;
+    return ctx->strag_trials;
+}
+
 /* frames: uint16 camera counts, or float pixels when float_frames (non-integer data). */
 static void launch(LzrContext *ctx, Workspace &ws, const void *frames, int nx, int ny, int nframes,
                    const LzrCandidate *cands, const LzrStart *starts, int ncands, LzrFit *out, hipStream_t stream,
-                   const int *d_count = nullptr, LzrStreamOrder *order = nullptr, bool float_frames = false) {
+                   const int *d_count = nullptr, LzrStreamOrder *order = nullptr, bool float_frames = false,
+                   unsigned class_mask = 7u) {
     reserve(ctx, ws, ncands);
     /* the workspace's queue, counters and hand-off buffers are reused by every call on it */
     if (order) LZR_CHECK(order->begin(stream));
@@ -1198,15 +1235,15 @@ static void launch(LzrContext *ctx, Workspace &ws, const void *frames, int nx, i
     a.seg_head = counters + kSegHeads;
     a.out = out;
     a.cfg = ctx->cfg;
-    a.strag_trials = ws.strag_entries > 0 ? ctx->strag_trials : 0;
+    a.strag_trials = ws.strag_entries > 0 ? hand_off_trials(ctx, ncands) : 0;
     a.strag = ws.strag;
     a.strag_ctr = strag_ctr;
     a.strag_entries = (unsigned)std::min(ws.strag_entries, (size_t)0xffffffffu);
     a.strag_lk = ctx->strag_group > 0 ? __builtin_ctz((unsigned)ctx->strag_group) : -1;
     if (float_frames)
-        launch_classes<float>(ctx, ws, a, stream);
+        launch_classes<float>(ctx, ws, a, stream, ncands, class_mask);
     else
-        launch_classes<uint16_t>(ctx, ws, a, stream);
+        launch_classes<uint16_t>(ctx, ws, a, stream, ncands, class_mask);
     const unsigned qr_cap = (unsigned)std::min(ws.qr_cap, (size_t)ncands);
     hipLaunchKernelGGL(epilogue_kernel, dim3((ncands + 255) / 256), dim3(256), 0, stream, cands, ncands, d_count, out,
                        ctx->cfg, (const int *)ws.order, (const unsigned *)counters, ws.qr_list, qr_cap,
@@ -1323,6 +1360,7 @@ LZR_EXPORT int lzr_clone(LzrContext *src, LzrContext **out) {
     c->rounds = src->rounds;
     c->park_threshold = src->park_threshold;
     c->strag_trials = src->strag_trials;
+    c->strag_trials_set = src->strag_trials_set;
     c->strag_group = src->strag_group;
     c->net_max_intensity = src->net_max_intensity;
     *out = c;
@@ -1399,6 +1437,7 @@ LZR_EXPORT int lzr_set_option(LzrContext *ctx, int option, double value) {
     case LZR_OPT_STRAGGLER_TRIALS:
         if (!(value >= 0 && value <= 100000)) return fail(ctx, "straggler trials must be >= 0");
         ctx->strag_trials = (int)value;
+        ctx->strag_trials_set = true;
         return 0;
     case LZR_OPT_MAX_HALF_WIDTH:
         if (!(value >= 1 && value <= LZR_MAX_HALF_WIDTH)) return fail(ctx, "max half-width out of range");
@@ -1440,6 +1479,27 @@ LZR_EXPORT int lzr_fit_batch_device(LzrContext *ctx, const uint16_t *d_frames, i
     try {
         LZR_CHECK(hipSetDevice(ctx->device));
         launch(ctx, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_out, (hipStream_t)stream);
+    } catch (const std::string &e) {
+        return fail(ctx, e);
+    }
+    return 0;
+}
+
+/* lzr_fit_batch_device for the host batcher, which also holds the candidates on the host:
+ * only the model classes they contain are launched (no side-stream work for absent
+ * classes), so concurrent batches of one class do not queue behind empty launches. */
+int lzr_fit_batch_device_hostcands(LzrContext *ctx, const uint16_t *d_frames, int nx, int ny, int nframes,
+                                   const LzrCandidate *d_cands, const LzrCandidate *h_cands, int ncands, LzrFit *d_out,
+                                   void *stream) {
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!check_shape(ctx, nx, ny, nframes, ncands)) return -1;
+    if (ncands == 0) return 0;
+    if (!d_frames || !d_cands || !h_cands || !d_out) return fail(ctx, "null pointer");
+    try {
+        LZR_CHECK(hipSetDevice(ctx->device));
+        launch(ctx, ctx->ws, d_frames, nx, ny, nframes, d_cands, nullptr, ncands, d_out, (hipStream_t)stream, nullptr,
+               &ctx->order, false, host_class_mask(ctx, h_cands, ncands));
     } catch (const std::string &e) {
         return fail(ctx, e);
     }
@@ -1495,7 +1555,8 @@ LZR_EXPORT int lzr_fit_batch(LzrContext *ctx, const uint16_t *frames, int nx, in
         LZR_CHECK(hipMemcpyAsync(ctx->d_frames, frames, npx * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
         LZR_CHECK(hipMemcpyAsync(ctx->d_cands, cands, (size_t)ncands * sizeof(LzrCandidate), hipMemcpyHostToDevice,
                                  ctx->stream));
-        launch(ctx, ctx->d_frames, nx, ny, nframes, ctx->d_cands, nullptr, ncands, ctx->d_out, ctx->stream);
+        launch(ctx, ctx->ws, ctx->d_frames, nx, ny, nframes, ctx->d_cands, nullptr, ncands, ctx->d_out, ctx->stream,
+               nullptr, &ctx->order, false, host_class_mask(ctx, cands, ncands));
         LZR_CHECK(hipMemcpyAsync(results, ctx->d_out, (size_t)ncands * sizeof(LzrFit), hipMemcpyDeviceToHost,
                                  ctx->stream));
         LZR_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1525,7 +1586,7 @@ LZR_EXPORT int lzr_fit_batch_f32(LzrContext *ctx, const float *frames, int nx, i
         LZR_CHECK(hipMemcpyAsync(ctx->d_cands, cands, (size_t)ncands * sizeof(LzrCandidate), hipMemcpyHostToDevice,
                                  ctx->stream));
         launch(ctx, ctx->ws, ctx->d_framesf, nx, ny, nframes, ctx->d_cands, nullptr, ncands, ctx->d_out, ctx->stream,
-               nullptr, &ctx->order, true);
+               nullptr, &ctx->order, true, host_class_mask(ctx, cands, ncands));
         LZR_CHECK(hipMemcpyAsync(results, ctx->d_out, (size_t)ncands * sizeof(LzrFit), hipMemcpyDeviceToHost,
                                  ctx->stream));
         LZR_CHECK(hipStreamSynchronize(ctx->stream));
@@ -2067,6 +2128,7 @@ static void tune_set(LzrContext *ctx, const TuneConfig &c) {
     ctx->rounds = c.rounds;
     ctx->park_threshold = c.park_threshold;
     ctx->strag_trials = c.strag_trials;
+    ctx->strag_trials_set = true;
     ctx->strag_group = c.strag_group;
 }
 

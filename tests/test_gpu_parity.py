@@ -118,6 +118,24 @@ def plain_100k():
     return frames, cands, ref, unsplit
 
 
+@pytest.mark.parametrize("nframes", [1, 16, 655, 656])
+def test_gpu_small_calls_hand_off_at_once(plain_100k, nframes):
+    """A call of at most 32768 candidates (frames of 50: up to 655 frames) hands every fit to
+    the straggler groups after its first trial unless the threshold was set, and launches
+    only as many waves as it has fits; 656 frames (32800 candidates) keep the default
+    threshold.  Every size gives the records of the launch without hand-off bit for bit."""
+    frames, cands, ref, unsplit = plain_100k
+    sel = cands["frame"] < nframes
+    c = cands[sel]
+    _say(f"case: symmetric 9x9, {nframes} frames, {c.size} fits, default hand-off policy")
+    with native.Context(abi.MODEL_SYMMETRIC, 1.35, default_half_width=4) as ctx:
+        got = ctx.fit(frames[:nframes], c)
+        assert ctx.get_option(abi.OPT_STRAGGLER_TRIALS) == 48
+    diff = _bytes_differ(got, unsplit[sel])
+    _say(f"records differing from the launch without hand-off: {int(diff.sum())}")
+    assert not diff.any()
+
+
 @pytest.mark.parametrize("trials", [4, 11, 17, 24, 48])
 def test_gpu_straggler_group_sizes(plain_100k, trials):
     """Fits still running after `trials` trials go to the straggler kernel, whose group size
