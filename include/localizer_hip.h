@@ -114,7 +114,7 @@ typedef struct LzrContext LzrContext;
 #define LZR_OPT_OCCUPANCY 7        /* register budget of the fitting kernel: 1 or 2 waves per SIMD; 0 (default) = 2 when the window tiles of 6 waves fit a CU's LDS */
 #define LZR_OPT_ROUNDS 8           /* fitting-kernel launches per model class; unfinished fits are parked between them (default 1) */
 #define LZR_OPT_PARK_THRESHOLD 9   /* a wave parks its fits once the queue is drained and at most this many lanes are busy (default 16) */
-#define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues in the straggler kernel, on 1..16 lanes by the count of such fits (default 48, 0 = off; until set, a call of at most 32768 candidates hands every fit over after its first trial, which shortens small calls and leaves the records unchanged) */
+#define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues in the straggler kernel, on 1..16 lanes by the count of such fits (default 48, 0 = off; until set, a call of at most 1024 candidates hands every fit over after its first trial, which shortens small calls and leaves the records unchanged) */
 #define LZR_OPT_DEFAULT_HALF_WIDTH 11 /* window half-width of candidates with half_width 0 (read-only: set at lzr_create) */
 #define LZR_OPT_STRAGGLER_GROUP 12  /* lanes per fit in the straggler kernel (64/lanes ROIs per wavefront): 1, 2, 4, 8, 16, or 0 (default) = by the number of handed-off fits */
 #define LZR_OPT_NET_MAX_INTENSITY 13 /* lzr_fit_batch_device_netinit starts from the network only the candidates whose intensity is below this value, the others from the reference start (default +inf: every candidate) */

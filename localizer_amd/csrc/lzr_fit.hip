@@ -1196,11 +1196,15 @@ static unsigned host_class_mask(const LzrContext *ctx, const LzrCandidate *cands
 /* Trials after which a fit moves to the straggler kernel.  A call of at most kSmallCall
  * candidates is latency work (the per-frame call pattern): its slowest fit sets the call,
  * and a fit runs its trials faster in a straggler group than among a wave of 64 fits, so
- * every fit moves after its first trial.  9x9 frames of 50 ROIs, same box, threshold 48 ->
- * 1: 1 frame 0.52 -> 0.36 ms, 16 frames 2.16 -> 1.94, 256 frames 2.53 -> 2.19, 640 frames
- * 2.59 -> 2.41, but 2000 frames 2.72 -> 3.68; records identical.  A threshold set by the
+ * every fit moves after its first trial.  9x9 frames of 50 ROIs, one call at a time, same
+ * box, threshold 48 -> 1: 1 frame 0.52 -> 0.36 ms, 16 frames 2.16 -> 1.94, 64 frames
+ * 2.26 -> 2.09, 256 frames 2.53 -> 2.19, but 2000 frames 2.72 -> 3.68; records identical.
+ * Groups hold 8-16 lanes per fit, so calls that overlap lose throughput to them: a stream
+ * of 9000-candidate blocks, three in flight, 25.4k -> 21.0k frames/s; per-frame batches of
+ * ~2200 ROIs from 256 threads, four in flight, 18.3k -> 14.8k frames/s.  The cut-off keeps
+ * the latency-bound batches (a few frames, from up to ~64 threads).  A threshold set by the
  * caller (LZR_OPT_STRAGGLER_TRIALS) or the autotuner holds for every size. */
-constexpr int kSmallCall = 32768;
+constexpr int kSmallCall = 1024;
 static int hand_off_trials(const LzrContext *ctx, int ncands) {
     if (!ctx->strag_trials_set && ctx->strag_trials > 0 && ncands <= kSmallCall) return 1;
     return ctx->strag_trials;

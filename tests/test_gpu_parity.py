@@ -118,12 +118,12 @@ def plain_100k():
     return frames, cands, ref, unsplit
 
 
-@pytest.mark.parametrize("nframes", [1, 16, 655, 656])
+@pytest.mark.parametrize("nframes", [1, 16, 20, 21, 300])
 def test_gpu_small_calls_hand_off_at_once(plain_100k, nframes):
-    """A call of at most 32768 candidates (frames of 50: up to 655 frames) hands every fit to
-    the straggler groups after its first trial unless the threshold was set, and launches
-    only as many waves as it has fits; 656 frames (32800 candidates) keep the default
-    threshold.  Every size gives the records of the launch without hand-off bit for bit."""
+    """A call of at most 1024 candidates (frames of 50: up to 20 frames) hands every fit to
+    the straggler groups after its first trial unless the threshold was set; 21 frames (1050
+    candidates) keep the default threshold.  Every call launches only as many waves as it has
+    fits.  Every size gives the records of the launch without hand-off bit for bit."""
     frames, cands, ref, unsplit = plain_100k
     sel = cands["frame"] < nframes
     c = cands[sel]
