@@ -619,8 +619,11 @@ __device__ __forceinline__ void sep_group_rows(const SepIn &q, int n, int nw, bo
     __syncthreads();
 }
 
-/* The straggler loop for groups of 2^LK lanes. */
-template <int P, typename Acc, int LK, typename PixT>
+/* The straggler loop for groups of 2^LK lanes.  KMax: the launch's largest window side; a
+ * build for windows up to 9 x 9 leaves out the row paths of larger windows, whose register
+ * demand (the 15-row single exchange) otherwise pushes the whole kernel into AGPR copies
+ * (500k 9x9 fits: straggler kernel 2029 -> 1865 us, records identical). */
+template <int P, typename Acc, int LK, typename PixT, int KMax = kSepRowsOne>
 __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
                                                  const Parked<P> *park, double *lds_words) {
     /* LDS: per-lane FP64 column buffer [nmax + 2][64] (as in the first kernel), the row exchange
@@ -797,9 +800,10 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                 GaussRec gy;
                 sep_columns(q, n, nw, xb, v, gy);
                 const bool guard = __any(active && n != nw);
-                if (nw <= 9) {
+                if (KMax <= 9 || nw <= 9) { /* KMax <= 9: every window is at most 9 x 9 */
                     sep_group_rows<9, lk>(q, n, nw, guard, gy, tile, xb, ey_slot, xchg, lane, v);
-                } else if (nw <= kSepRowsOne) {
+                } else if constexpr (KMax > 9) {
+                if (nw <= kSepRowsOne) {
                     sep_group_rows<kSepRowsOne, lk>(q, n, nw, guard, gy, tile, xb, ey_slot, xchg, lane, v);
                 } else for (int m = 0; m < rounds; ++m) {
                     /* the round's row factors in row order: every lane steps the recurrence and
@@ -842,6 +846,7 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                     }
                     __syncthreads();
                 }
+                }
             }
             sep_finish<P, Acc>(q, n * n, v, s.phase == kInit, s.fnorm * s.fnorm, acc);
         }
@@ -861,7 +866,7 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
  * apply exits at once).  Both derive the group size from the same lane count. */
 enum StragglerRole : int { kAllGroups = 0, kMultiLane = 1, kOneLane = 2 };
 
-template <int P, typename Acc, int W, int Role, typename PixT = uint16_t>
+template <int P, typename Acc, int W, int Role, typename PixT = uint16_t, int KMax = kSepRowsOne>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
     extern __shared__ double lds_words[];
@@ -890,9 +895,9 @@ lm_straggler(KernelArgs args, int cls, unsigned lanes_total) {
         }
     }
     switch (lk) {
-    case 4: straggler_groups<P, Acc, 4, PixT>(args, count, head, park, lds_words); break;
-    case 3: straggler_groups<P, Acc, 3, PixT>(args, count, head, park, lds_words); break;
-    case 2: straggler_groups<P, Acc, 2, PixT>(args, count, head, park, lds_words); break;
+    case 4: straggler_groups<P, Acc, 4, PixT, KMax>(args, count, head, park, lds_words); break;
+    case 3: straggler_groups<P, Acc, 3, PixT, KMax>(args, count, head, park, lds_words); break;
+    case 2: straggler_groups<P, Acc, 2, PixT, KMax>(args, count, head, park, lds_words); break;
     case 1:
         if constexpr (Role == kAllGroups) straggler_groups<P, Acc, 1, PixT>(args, count, head, park, lds_words);
         break;
@@ -1076,8 +1081,11 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
          * register file holds twice over, the occupancy query alone would allow two, and two
          * interleaved waves take ~1.4x longer per trial) */
         const size_t glds = std::max(slds, ctx->lds_per_cu / 4);
+        /* windows up to 9 x 9: the group build without the larger windows' row paths */
+        const bool small_windows = P != 7 && 2 * ctx->cfg.max_half_width + 1 <= 9;
+        auto group_kernel = small_windows ? lm_straggler<P, Acc, 1, kRole, PixT, 9> : lm_straggler<P, Acc, 1, kRole, PixT>;
         int per_cu = 0;
-        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_straggler<P, Acc, 1, kRole, PixT>, 64, glds));
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, group_kernel, 64, glds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
         per_cu = std::min(per_cu, 4);
         const unsigned lanes = (unsigned)(per_cu * ctx->num_cus) * 64u; /* sets the group size in both launches */
@@ -1093,8 +1101,8 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
                                dim3(64), lds, stream, a, cls, lanes);
         }
         const int waves_g = std::max(1, (ncap + (64 / kMaxGroup) - 1) / (64 / kMaxGroup)); /* kMaxGroup lanes per fit */
-        hipLaunchKernelGGL((lm_straggler<P, Acc, 1, kRole, PixT>), dim3(std::min(per_cu * ctx->num_cus, waves_g)), dim3(64),
-                           glds, stream, a, cls, lanes);
+        hipLaunchKernelGGL(group_kernel, dim3(std::min(per_cu * ctx->num_cus, waves_g)), dim3(64), glds, stream, a, cls,
+                           lanes);
     }
 }
 
