@@ -1082,8 +1082,9 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
          * interleaved waves take ~1.4x longer per trial) */
         const size_t glds = std::max(slds, ctx->lds_per_cu / 4);
         /* windows up to 9 x 9: the group build without the larger windows' row paths */
-        const bool small_windows = P != 7 && 2 * ctx->cfg.max_half_width + 1 <= 9;
-        auto group_kernel = small_windows ? lm_straggler<P, Acc, 1, kRole, PixT, 9> : lm_straggler<P, Acc, 1, kRole, PixT>;
+        auto group_kernel = lm_straggler<P, Acc, 1, kRole, PixT>;
+        if constexpr (P != 7)
+            if (2 * ctx->cfg.max_half_width + 1 <= 9) group_kernel = lm_straggler<P, Acc, 1, kRole, PixT, 9>;
         int per_cu = 0;
         LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, group_kernel, 64, glds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");
