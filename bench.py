@@ -650,8 +650,13 @@ def dense_leg(nframes, block_frames, latency_frames, log, pipelines=DENSE_PIPELI
         sec = time.perf_counter() - t
         return sec, cands, int(sum(int(p.acc.item()) for p in pipes))
 
-    sec_net, cands, acc_net = stream_all(True)
-    sec_plain, _, acc_plain = stream_all(False)
+    # each mode twice, alternating, best of two: the first pass over the stream also warms
+    # clocks and caches for whichever mode runs second
+    runs = {True: [], False: []}
+    for netinit_fit in (True, False, True, False):
+        runs[netinit_fit].append(stream_all(netinit_fit))
+    sec_net, cands, acc_net = min(runs[True], key=lambda r: r[0])
+    sec_plain, _, acc_plain = min(runs[False], key=lambda r: r[0])
     lat = []
     for f in range(min(latency_frames, nframes)):
         torch.cuda.synchronize()

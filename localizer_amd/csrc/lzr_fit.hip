@@ -303,10 +303,14 @@ struct RoundArgs {
     int threshold;
 };
 
-/* W = minimum waves per SIMD the register allocation must allow (occupancy target). */
-template <int P, typename Acc, int W, typename PixT = uint16_t>
+/* W = minimum waves per SIMD the register allocation must allow (occupancy target).
+ * Park = false: the build for one round (LZR_OPT_ROUNDS 1, the default), without the
+ * parking and resume paths; their register demand made the two-wave build spill 264 B per
+ * lane, the one-round build spills none.  Within one round both builds do the same work. */
+template <int P, typename Acc, int W, typename PixT = uint16_t, bool Park = true>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W)))
 lm_persistent(KernelArgs args, int cls, RoundArgs r) {
+    const bool resume = Park && r.resume;
     extern __shared__ double lds_words[];
     /* LDS: per-lane FP64 column buffer [nmax + 2][64] (column factors, then the first two row
      * factors), then the pixel tile [nmax*nmax][64] */
@@ -317,8 +321,8 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
     const PixT *frames = static_cast<const PixT *>(args.frames);
     const int lane = threadIdx.x;
     const unsigned long long below = (1ull << lane) - 1ull;
-    const unsigned count = r.resume ? *r.in_count : args.count[cls];
-    unsigned *head = r.resume ? r.in_head : args.head + cls;
+    const unsigned count = resume ? *r.in_count : args.count[cls];
+    unsigned *head = resume ? r.in_head : args.head + cls;
     unsigned *seg_head = args.seg_head + cls * kQueueSegs * kSegStride;
     int seg = blockIdx.x % kQueueSegs, drained = 0; /* fresh queue: this wave's segment, segments found drained */
     unsigned first = 0; /* queue entries of the classes before this one */
@@ -350,7 +354,7 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
             const int cnt = __popcll(want);
             const int leader = __ffsll((long long)want) - 1;
             unsigned base = 0;
-            if (!r.resume) {
+            if (!resume) {
                 /* segment seg holds queue entries seg + kQueueSegs k, k < size */
                 const unsigned size = count > (unsigned)seg ? (count - (unsigned)seg + kQueueSegs - 1) / kQueueSegs : 0u;
                 if (lane == leader) base = atomicAdd(seg_head + seg * kSegStride, (unsigned)cnt);
@@ -462,7 +466,7 @@ lm_persistent(KernelArgs args, int cls, RoundArgs r) {
         if (!amask) break;
 
         /* ---- park the stragglers of a drained queue ---- */
-        if (!open && r.threshold > 0 && __popcll(amask) <= r.threshold) {
+        if (Park && !open && r.threshold > 0 && __popcll(amask) <= r.threshold) {
             const int leader = __ffsll((long long)amask) - 1;
             unsigned base = 0;
             if (lane == leader) base = atomicAdd(r.out_count, (unsigned)__popcll(amask));
@@ -1028,7 +1032,10 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
 template <int P, typename Acc, int W, typename PixT = uint16_t>
 static int grid_for(LzrContext *ctx, size_t lds) {
     int per_cu = 0;
-    LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, W, PixT>, 64, lds));
+    if (ctx->rounds == 1)
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, W, PixT, false>, 64, lds));
+    else
+        LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lm_persistent<P, Acc, W, PixT>, 64, lds));
     if (per_cu < 1) throw std::string("fitting kernel cannot be resident with this window size");
     if (ctx->waves_per_cu > 0) per_cu = std::min(per_cu, ctx->waves_per_cu);
     return per_cu * ctx->num_cus;
@@ -1071,7 +1078,10 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
         r.out = ws.park[cls][ob];
         r.out_count = ctr + 2 * ob;
         r.threshold = round == ctx->rounds - 1 ? 0 : ctx->park_threshold;
-        hipLaunchKernelGGL((lm_persistent<P, Acc, W, PixT>), dim3(grid), dim3(64), lds, stream, a, cls, r);
+        if (ctx->rounds == 1)
+            hipLaunchKernelGGL((lm_persistent<P, Acc, W, PixT, false>), dim3(grid), dim3(64), lds, stream, a, cls, r);
+        else
+            hipLaunchKernelGGL((lm_persistent<P, Acc, W, PixT>), dim3(grid), dim3(64), lds, stream, a, cls, r);
     }
     if (a.strag_trials > 0) {
         const size_t slds = lds + (size_t)xchg_slots(P) * 64 * sizeof(double); /* + the row exchange */
