@@ -33,10 +33,10 @@ from localizer_amd import abi, device, native, netinit, synth  # noqa: E402
 METRIC = "localizations/sec on 1 MI355X: plain LM vs net-initialized LM at 1e-3 px"
 PEAK_FP64_TFLOPS = 78.6  # the fit's passes and LM logic run in FP64 (vector FP64 peak)
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per fitting call on the default workload (lm_persistent<5> two-wave build,
+# HBM bytes per fitting call on the default workload (lm_persistent<5> two-wave one-round build,
 # lm_straggler<5> and the epilogue), from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by
 # the reading of a plain 512 MiB copy kernel (fetch x2.0, write x1.0).
-MEASURED_TRAFFIC_BYTES = {"plain-lm-256px-10000frames-9x9": 2135.4e6}
+MEASURED_TRAFFIC_BYTES = {"plain-lm-256px-10000frames-9x9": 1206.4e6}
 # Algorithmic work per ROI (symmetric model): W = N[(1+T) F_f + (1+A)(F_J + F_NE)] + T F_solve
 F_F, F_J, F_NE, F_SOLVE = 12.0, 6.0, 40.0, 300.0
 F_F7, F_J7, F_NE7, F_SOLVE7 = 16.0, 30.0, 70.0, 700.0
