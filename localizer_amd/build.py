@@ -99,21 +99,32 @@ def build_sanitized():
     AddressSanitizer + UndefinedBehaviorSanitizer, and a ThreadSanitizer driver of the
     oracle's frame-parallel pool (tests/cpp/tsan_main.cpp)."""
     os.makedirs(SAN, exist_ok=True)
+    # each product is linked to a temporary name and renamed into place, so a test process that
+    # runs or maps the previous build keeps a whole file
+    def out(name):
+        return os.path.join(SAN, name + ".tmp")
+
+    def publish(name):
+        os.replace(out(name), os.path.join(SAN, name))
+
     orc = [os.path.join(ROOT, "oracle", f) for f in ("lzr_oracle.c", "lzr_find_oracle.c", "lzr_render_oracle.c")]
-    _run(["gcc", *ASAN, "-fPIC", "-std=c11", "-shared", "-o", os.path.join(SAN, "liblzr_oracle.so"), *orc,
-          "-lm", "-lpthread"])
+    _run(["gcc", *ASAN, "-fPIC", "-std=c11", "-shared", "-o", out("liblzr_oracle.so"), *orc, "-lm", "-lpthread"])
+    publish("liblzr_oracle.so")
     emu = os.path.join(ROOT, "tests", "emu")
-    _run(["g++", *ASAN, "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas", "-o",
-          os.path.join(SAN, "liblzr_emu.so"), os.path.join(emu, "lzr_emu.cpp")])
+    _run(["g++", *ASAN, "-std=c++17", "-fPIC", "-shared", "-Wno-unknown-pragmas", "-o", out("liblzr_emu.so"),
+          os.path.join(emu, "lzr_emu.cpp")])
+    publish("liblzr_emu.so")
     cpp = os.path.join(ROOT, "tests", "cpp")
-    _run(["g++", *ASAN, "-std=c++17", "-o", os.path.join(SAN, "storage_main"), os.path.join(cpp, "storage_main.cpp")])
+    _run(["g++", *ASAN, "-std=c++17", "-o", out("storage_main"), os.path.join(cpp, "storage_main.cpp")])
+    publish("storage_main")
     objs = []
     for src in orc:
         obj = os.path.join(SAN, os.path.basename(src)[:-2] + "_tsan.o")
         _run(["gcc", *TSAN, "-std=c11", "-c", "-o", obj, src])
         objs.append(obj)
-    _run(["g++", *TSAN, "-std=c++17", "-o", os.path.join(SAN, "tsan_main"), os.path.join(cpp, "tsan_main.cpp"), *objs,
+    _run(["g++", *TSAN, "-std=c++17", "-o", out("tsan_main"), os.path.join(cpp, "tsan_main.cpp"), *objs,
           "-lm", "-lpthread"])
+    publish("tsan_main")
 
 
 def build_host_libs():

@@ -4,6 +4,7 @@ CPU tests that exercise them; the oracle's frame-parallel pool (the reference co
 one-frame-per-task pattern, source/PALMAnalysis.cpp:204-252) and concurrent per-frame
 finder calls run under ThreadSanitizer.  Builds: `python -m localizer_amd.build sanitize`
 (tests/_san/, rebuilt here when a source is newer)."""
+import fcntl
 import glob
 import os
 import subprocess
@@ -33,11 +34,15 @@ def built():
     if not (_runtime("libasan.so") and _runtime("libtsan.so")):
         pytest.skip("gcc sanitizer runtimes not installed")
     newest = max(os.path.getmtime(f) for f in SOURCES)
-    stale = any(not os.path.exists(os.path.join(SAN, p)) or os.path.getmtime(os.path.join(SAN, p)) < newest
-                for p in PRODUCTS)
-    if stale:
-        subprocess.run([sys.executable, "-m", "localizer_amd.build", "sanitize"], cwd=ROOT, check=True,
-                       capture_output=True, timeout=600)
+    os.makedirs(SAN, exist_ok=True)
+    # one process rebuilds at a time (parallel test workers); the others wait, then see it fresh
+    with open(os.path.join(SAN, ".build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        stale = any(not os.path.exists(os.path.join(SAN, p)) or os.path.getmtime(os.path.join(SAN, p)) < newest
+                    for p in PRODUCTS)
+        if stale:
+            subprocess.run([sys.executable, "-m", "localizer_amd.build", "sanitize"], cwd=ROOT, check=True,
+                           capture_output=True, timeout=600)
     return SAN
 
 
