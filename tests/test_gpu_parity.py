@@ -118,6 +118,23 @@ def plain_100k():
     return frames, cands, ref, unsplit
 
 
+@pytest.mark.parametrize("rounds", [2, 3])
+def test_gpu_parking_rounds(plain_100k, rounds):
+    """LZR_OPT_ROUNDS > 1 launches the persistent kernel's build with the parking and resume
+    paths: a drained wave with few busy lanes parks their fits, and the next launch continues
+    them.  One round (the default) launches the build without those paths.  Both give the
+    records of the launch without hand-off bit for bit."""
+    frames, cands, ref, unsplit = plain_100k
+    _say(f"case: symmetric 9x9, 100k fits, {rounds} rounds")
+    with native.Context(abi.MODEL_SYMMETRIC, 1.35, default_half_width=4) as ctx:
+        ctx.set_option(abi.OPT_ROUNDS, rounds)
+        got = ctx.fit(frames, cands)
+        assert ctx.get_option(abi.OPT_ROUNDS) == rounds
+    diff = _bytes_differ(got, unsplit)
+    _say(f"records differing from the one-round launch without hand-off: {int(diff.sum())}")
+    assert not diff.any()
+
+
 @pytest.mark.parametrize("nframes", [1, 16, 20, 21, 300])
 def test_gpu_small_calls_hand_off_at_once(plain_100k, nframes):
     """A call of at most 1024 candidates (frames of 50: up to 20 frames) hands every fit to
