@@ -112,7 +112,7 @@ typedef struct LzrContext LzrContext;
 #define LZR_OPT_WAVES_PER_CU 5     /* persistent-kernel occupancy (autotuned when 0)                   */
 #define LZR_OPT_MAX_HALF_WIDTH 6   /* largest per-candidate window half-width accepted (default: the context's) */
 #define LZR_OPT_OCCUPANCY 7        /* register budget of the fitting kernel: 1 or 2 waves per SIMD; 0 (default) = 2 when the window tiles of 6 waves fit a CU's LDS */
-#define LZR_OPT_ROUNDS 8           /* fitting-kernel launches per model class; unfinished fits are parked between them (default 1) */
+#define LZR_OPT_ROUNDS 8           /* fitting-kernel launches per model class; unfinished fits are parked between them (default 1, which runs a build of the fitting kernel without the parking paths: no register spills, ~20 % faster; records identical) */
 #define LZR_OPT_PARK_THRESHOLD 9   /* a wave parks its fits once the queue is drained and at most this many lanes are busy (default 16) */
 #define LZR_OPT_STRAGGLER_TRIALS 10 /* a fit still running after this many trials continues in the straggler kernel, on 1..16 lanes by the count of such fits (default 48, 0 = off; until set, a call of at most 1024 candidates hands every fit over after its first trial, which shortens small calls and leaves the records unchanged) */
 #define LZR_OPT_DEFAULT_HALF_WIDTH 11 /* window half-width of candidates with half_width 0 (read-only: set at lzr_create) */
