@@ -623,10 +623,10 @@ __device__ __forceinline__ void sep_group_rows(const SepIn &q, int n, int nw, bo
     __syncthreads();
 }
 
-/* The straggler loop for groups of 2^LK lanes.  KMax: the launch's largest window side; a
- * build for windows up to 9 x 9 leaves out the row paths of larger windows, whose register
- * demand (the 15-row single exchange) otherwise pushes the whole kernel into AGPR copies
- * (500k 9x9 fits: straggler kernel 2029 -> 1865 us, records identical). */
+/* The straggler loop for groups of 2^LK lanes.  KMax: the launch's largest window side; the
+ * builds for windows up to 9 x 9 and 11 x 11 leave out the row paths of larger windows, whose
+ * register demand (the 15-row single exchange) otherwise pushes the whole kernel into AGPR
+ * copies (500k 9x9 fits: straggler kernel 2029 -> 1865 us, records identical). */
 template <int P, typename Acc, int LK, typename PixT, int KMax = kSepRowsOne>
 __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigned count, unsigned *head,
                                                  const Parked<P> *park, double *lds_words) {
@@ -804,9 +804,11 @@ __device__ __forceinline__ void straggler_groups(const KernelArgs &args, unsigne
                 GaussRec gy;
                 sep_columns(q, n, nw, xb, v, gy);
                 const bool guard = __any(active && n != nw);
-                if (KMax <= 9 || nw <= 9) { /* KMax <= 9: every window is at most 9 x 9 */
+                if constexpr (KMax == 11) { /* every window is at most 11 x 11; rows past n add zeros */
+                    sep_group_rows<11, lk>(q, n, nw, guard, gy, tile, xb, ey_slot, xchg, lane, v);
+                } else if (KMax <= 9 || nw <= 9) { /* KMax <= 9: every window is at most 9 x 9 */
                     sep_group_rows<9, lk>(q, n, nw, guard, gy, tile, xb, ey_slot, xchg, lane, v);
-                } else if constexpr (KMax > 9) {
+                } else if constexpr (KMax > 11) {
                 if (nw <= kSepRowsOne) {
                     sep_group_rows<kSepRowsOne, lk>(q, n, nw, guard, gy, tile, xb, ey_slot, xchg, lane, v);
                 } else for (int m = 0; m < rounds; ++m) {
@@ -1093,8 +1095,13 @@ static void run_class(LzrContext *ctx, Workspace &ws, const KernelArgs &a, int c
         const size_t glds = std::max(slds, ctx->lds_per_cu / 4);
         /* windows up to 9 x 9: the group build without the larger windows' row paths */
         auto group_kernel = lm_straggler<P, Acc, 1, kRole, PixT>;
-        if constexpr (P != 7)
-            if (2 * ctx->cfg.max_half_width + 1 <= 9) group_kernel = lm_straggler<P, Acc, 1, kRole, PixT, 9>;
+        if constexpr (P != 7) {
+            const int nmax = 2 * ctx->cfg.max_half_width + 1;
+            if (nmax <= 9)
+                group_kernel = lm_straggler<P, Acc, 1, kRole, PixT, 9>;
+            else if (nmax <= 11)
+                group_kernel = lm_straggler<P, Acc, 1, kRole, PixT, 11>;
+        }
         int per_cu = 0;
         LZR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, group_kernel, 64, glds));
         if (per_cu < 1) throw std::string("straggler kernel cannot be resident with this window size");

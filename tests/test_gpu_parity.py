@@ -118,6 +118,25 @@ def plain_100k():
     return frames, cands, ref, unsplit
 
 
+@pytest.mark.parametrize("trials", [8, 24])
+def test_gpu_straggler_groups_11x11(trials):
+    """11 x 11 windows (half-width 5) run the straggler build for windows up to 11 x 11; its
+    groups give the records of the launch without hand-off bit for bit."""
+    w = synth.LEARNED_INIT_11
+    rois, _, cands = synth.generate_rois(w, 20000, 5, 1.0)
+    _say(f"case: symmetric 11x11, 20k fits, hand-off after {trials} trials")
+    with native.Context(abi.MODEL_SYMMETRIC, w.initial_psf_width, default_half_width=5) as ctx:
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, 0)
+        unsplit = ctx.fit(rois, cands)
+        ctx.set_option(abi.OPT_STRAGGLER_TRIALS, trials)
+        got = ctx.fit(rois, cands)
+    handed = int(np.sum(unsplit["trials"] > trials))
+    diff = _bytes_differ(got, unsplit)
+    _say(f"{handed} fits handed off; records differing from the launch without hand-off: {int(diff.sum())}")
+    assert handed > 100
+    assert not diff.any()
+
+
 @pytest.mark.parametrize("rounds", [2, 3])
 def test_gpu_parking_rounds(plain_100k, rounds):
     """LZR_OPT_ROUNDS > 1 launches the persistent kernel's build with the parking and resume
